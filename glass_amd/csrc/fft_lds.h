@@ -1,0 +1,106 @@
+// fft_lds.h -- batched in-LDS complex FFTs for one workgroup (gfx950).
+//
+// NL independent length-N forward transforms stored link-major in LDS
+// (buf[l*N + n]), unnormalised, exp(-2 pi i n k / N) convention, as
+// kiss_fft forward (external/kissfft/kiss_fft.c:337-369).  Stockham
+// autosort formulation, radix 4 with one radix-2 stage when log2 N is odd.
+// Each stage reads every butterfly's inputs into registers, synchronises,
+// then writes the outputs into the same buffer, so a single LDS buffer
+// suffices (96*N bytes for the six link series of a template).
+//
+// Twiddles come from a table tw[k] = exp(-2 pi i k / NTW) of NTW entries
+// (NTW a power of two >= N) built once per device with the phase formed as
+// -2*pi*k/NTW, the kiss_fft twiddle formula.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace glass {
+
+__device__ inline double2 cmul(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// One Stockham stage of radix R over NL transforms; butterflies are spread
+// over the workgroup's NTHR threads.  Ns is the span of the previous stage.
+template <int N, int NL, int R, int NTHR>
+__device__ inline void stockham_stage(double2* buf, int Ns, const double2* __restrict__ tw, int twstride)
+{
+    constexpr int NB = NL * (N / R);                 // butterflies in the stage
+    constexpr int PER = (NB + NTHR - 1) / NTHR;      // per thread
+    double2 v[PER][R];
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int p = 0; p < PER; p++)
+    {
+        int b = tid + p * NTHR;
+        if (NB % NTHR == 0 || b < NB)
+        {
+            int l = b / (N / R), j = b % (N / R);
+            const double2* src = buf + l * N;
+#pragma unroll
+            for (int m = 0; m < R; m++) v[p][m] = src[j + m * (N / R)];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < PER; p++)
+    {
+        int b = tid + p * NTHR;
+        if (NB % NTHR == 0 || b < NB)
+        {
+            int l = b / (N / R), j = b % (N / R);
+            int k = j % Ns;
+            // twiddle exp(-2 pi i m k /(R Ns)) = tw[m*k*(NTW/(R*Ns))]
+            int step = k * twstride * (N / (R * Ns));
+#pragma unroll
+            for (int m = 1; m < R; m++) v[p][m] = cmul(v[p][m], tw[m * step]);
+            double2 y[R];
+            if (R == 4)
+            {
+                double2 a0 = v[p][0], a1 = v[p][1], a2 = v[p][2], a3 = v[p][3];
+                double2 s02 = make_double2(a0.x + a2.x, a0.y + a2.y);
+                double2 d02 = make_double2(a0.x - a2.x, a0.y - a2.y);
+                double2 s13 = make_double2(a1.x + a3.x, a1.y + a3.y);
+                double2 d13 = make_double2(a1.x - a3.x, a1.y - a3.y);
+                y[0] = make_double2(s02.x + s13.x, s02.y + s13.y);
+                y[2] = make_double2(s02.x - s13.x, s02.y - s13.y);
+                // -i * d13 = (d13.y, -d13.x)
+                y[1] = make_double2(d02.x + d13.y, d02.y - d13.x);
+                y[3] = make_double2(d02.x - d13.y, d02.y + d13.x);
+            }
+            else
+            {
+                y[0] = make_double2(v[p][0].x + v[p][1].x, v[p][0].y + v[p][1].y);
+                y[1] = make_double2(v[p][0].x - v[p][1].x, v[p][0].y - v[p][1].y);
+            }
+            double2* dst = buf + l * N;
+            int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+            for (int m = 0; m < R; m++) dst[base + m * Ns] = y[m];
+        }
+    }
+    __syncthreads();
+}
+
+// Full forward FFT of NL length-N series in LDS. twstride = NTW / N.
+template <int N, int NL, int NTHR>
+__device__ inline void lds_fft(double2* buf, const double2* __restrict__ tw, int twstride)
+{
+    static_assert((N & (N - 1)) == 0 && N >= 4, "power of two");
+    constexpr int LOG2 = __builtin_ctz(N);
+    int Ns = 1;
+    if (LOG2 & 1)
+    {
+        stockham_stage<N, NL, 2, NTHR>(buf, Ns, tw, twstride);
+        Ns *= 2;
+    }
+#pragma unroll
+    for (int s = 0; s < LOG2 / 2; s++)
+    {
+        stockham_stage<N, NL, 4, NTHR>(buf, Ns, tw, twstride);
+        Ns *= 4;
+    }
+}
+
+}  // namespace glass

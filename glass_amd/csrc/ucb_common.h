@@ -1,0 +1,183 @@
+// ucb_common.h -- shared device code for the UCB frequency-domain path on
+// gfx950 (CDNA4).  FP64 throughout.
+//
+// The slow-series sample restates ucb/src/glass_ucb_waveform.c:840-937 with
+// the LISA response helpers of utils/src/glass_lisa.c:1521-1596.  The phase
+// chain t -> k.x -> xi -> arg2 is evaluated with contraction disabled and in
+// the reference's operation order, because arg2 is a difference of ~1e6-1e7
+// rad products whose rounding (~1e-10 rad) would otherwise move the result
+// by far more than the 1e-12 parity budget.  Amplitude-only quantities
+// (detector tensors, transfer-function sinc) are free to contract.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace glass {
+
+// utils/src/glass_constants.h:31-89, utils/src/glass_lisa.h:31-49
+constexpr double kPI2 = 6.283185307179586;
+constexpr double kPI = 3.14159265358979323846;
+constexpr double kCLIGHT = 299792458.;
+constexpr double kYEAR = 3.15581497632e7;
+constexpr double kAU = 1.49597870700e11;
+constexpr double kSQ3 = 1.73205080757;
+constexpr double kLARM = 2.5e9;
+constexpr double kSQRT1_2 = 0.70710678118654752440;
+constexpr double kINVSQ6 = 0.408248290463863;  // glass_lisa.c:436
+
+// Per-template constants (glass_ucb_waveform.c:804-837).
+struct UcbConst
+{
+    double T, t0;
+    double f0, phi0, dfdt, d2fdt2;
+    double df;      // PI2*(q/T)
+    double pi2f0;   // PI2*f0, first factor of arg2
+    double pidfdt;  // M_PI*dfdt
+    double pid2;    // (M_PI/3)*d2fdt2
+    double aevc;    // 0.666...*dfdt/f0
+    double halfd2;  // 0.5*d2fdt2
+    double DPr, DPi, DCr, DCi;
+    double k[3];
+    double ep[6], ec[6];  // symmetric tensors: xx, yy, zz, xy, xz, yz
+    double invfstar, fstar, invL;
+    long q;
+    int np;
+};
+
+// glass_ucb_waveform.c:804-837 + LISA_polarization_tensor (glass_lisa.c:1566-1596)
+__host__ __device__ inline void ucb_setup(const double* p, int np, double T, double t0,
+                                          double L, double fstar, UcbConst& c)
+{
+    // k feeds the phase through k.x: no contraction anywhere in the setup
+#pragma clang fp contract(off)
+    c.T = T;
+    c.t0 = t0;
+    c.np = np;
+    c.f0 = p[0] / T;
+    double costh = p[1], phi = p[2], amp = exp(p[3]), cosi = p[4], psi = p[5];
+    c.phi0 = p[6];
+    c.dfdt = 0.0;
+    c.d2fdt2 = 0.0;
+    if (np > 7) c.dfdt = p[7] / (T * T);
+    if (np > 8) c.d2fdt2 = p[8] / (T * T * T);
+    c.q = (long)(c.f0 * T);
+    double cosps = cos(2. * psi), sinps = sin(2. * psi);
+    double Aplus = amp * (1. + cosi * cosi) * 0.5;
+    double Across = -amp * (cosi);
+    c.df = kPI2 * (((double)c.q) / T);
+    c.pi2f0 = kPI2 * c.f0;
+    c.pidfdt = kPI * c.dfdt;
+    c.pid2 = (kPI / 3.0) * c.d2fdt2;
+    c.aevc = 0.66666666666666666666 * c.dfdt / c.f0;
+    c.halfd2 = 0.5 * c.d2fdt2;
+    c.DPr = Aplus * cosps;
+    c.DPi = -Across * sinps;
+    c.DCr = -Aplus * sinps;
+    c.DCi = -Across * cosps;
+    double sinth = sqrt(1.0 - costh * costh);
+    double cosph = cos(phi), sinph = sin(phi);
+    double u0 = costh * cosph, u1 = costh * sinph, u2 = -sinth;
+    double v0 = sinph, v1 = -cosph, v2 = 0.;
+    c.k[0] = -sinth * cosph;
+    c.k[1] = -sinth * sinph;
+    c.k[2] = -costh;
+    c.ep[0] = v0 * v0 - u0 * u0;  c.ec[0] = u0 * v0 + v0 * u0;
+    c.ep[1] = v1 * v1 - u1 * u1;  c.ec[1] = u1 * v1 + v1 * u1;
+    c.ep[2] = v2 * v2 - u2 * u2;  c.ec[2] = u2 * v2 + v2 * u2;
+    c.ep[3] = v0 * v1 - u0 * u1;  c.ec[3] = u0 * v1 + v0 * u1;
+    c.ep[4] = v0 * v2 - u0 * u2;  c.ec[4] = u0 * v2 + v0 * u2;
+    c.ep[5] = v1 * v2 - u1 * u2;  c.ec[5] = u1 * v2 + v1 * u2;
+    c.fstar = fstar;
+    c.invfstar = 1. / fstar;
+    c.invL = 1. / L;
+}
+
+// r^T E r for a symmetric 3x3 tensor stored as xx,yy,zz,xy,xz,yz
+__device__ inline double quad3(const double* E, double a, double b, double cc)
+{
+    return E[0] * a * a + E[1] * b * b + E[2] * cc * cc + 2.0 * (E[3] * a * b + E[4] * a * cc + E[5] * b * cc);
+}
+
+// One slow-series sample of the six links at time t.
+// pos: spacecraft positions x1,x2,x3,y1,y2,y3,z1,z2,z3 at t (metres).
+// out: TR,TI for links 12,21,31,13,23,32 (12 doubles).
+// glass_ucb_waveform.c:843-936, LISA_detector_tensor glass_lisa.c:1521-1564.
+__device__ inline void ucb_slow_sample(const UcbConst& c, double t, const double* __restrict__ pos,
+                                       double* out)
+{
+    const double x1 = pos[0], x2 = pos[1], x3 = pos[2];
+    const double y1 = pos[3], y2 = pos[4], y3 = pos[5];
+    const double z1 = pos[6], z2 = pos[7], z3 = pos[8];
+
+    // unit arm vectors (amplitude path)
+    const double r12a = (x2 - x1) * c.invL, r12b = (y2 - y1) * c.invL, r12c = (z2 - z1) * c.invL;
+    const double r13a = (x3 - x1) * c.invL, r13b = (y3 - y1) * c.invL, r13c = (z3 - z1) * c.invL;
+    const double r23a = (x3 - x2) * c.invL, r23b = (y3 - y2) * c.invL, r23c = (z3 - z2) * c.invL;
+    const double dp12 = quad3(c.ep, r12a, r12b, r12c), dc12 = quad3(c.ec, r12a, r12b, r12c);
+    const double dp13 = quad3(c.ep, r13a, r13b, r13c), dc13 = quad3(c.ec, r13a, r13b, r13c);
+    const double dp23 = quad3(c.ep, r23a, r23b, r23c), dc23 = quad3(c.ec, r23a, r23b, r23c);
+    const double k12 = c.k[0] * r12a + c.k[1] * r12b + c.k[2] * r12c;
+    const double k13 = c.k[0] * r13a + c.k[1] * r13b + c.k[2] * r13c;
+    const double k23 = c.k[0] * r23a + c.k[1] * r23b + c.k[2] * r23c;
+
+    double arg2[3], fon[3], aev[3];
+    {
+#pragma clang fp contract(off)
+        const double px[3] = {x1, x2, x3}, py[3] = {y1, y2, y3}, pz[3] = {z1, z2, z3};
+        const double dft = c.df * t;
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+        {
+            double kdotx = (px[i] * c.k[0] + py[i] * c.k[1] + pz[i] * c.k[2]) / kCLIGHT;
+            double xi = t - kdotx;
+            double fi = c.f0;
+            double a2 = c.pi2f0 * xi - c.phi0 - dft;
+            double ae = 1.0;
+            if (c.np > 7)
+            {
+                fi += c.dfdt * xi;
+                a2 += c.pidfdt * xi * xi;
+                ae += c.aevc * xi;
+            }
+            if (c.np > 8)
+            {
+                fi += c.halfd2 * xi * xi;
+                a2 += c.pid2 * xi * xi * xi;
+            }
+            arg2[i] = a2;
+            fon[i] = fi * c.invfstar;
+            aev[i] = ae;
+        }
+    }
+    // links: (i,j) with kdotr[i][j] antisymmetric, dplus/dcross symmetric
+    const int li[6] = {0, 1, 2, 0, 1, 2};
+    const double kr[6] = {k12, -k12, -k13, k13, k23, -k23};
+    const double dpl[6] = {dp12, dp12, dp13, dp13, dp23, dp23};
+    const double dcl[6] = {dc12, dc12, dc13, dc13, dc23, dc23};
+#pragma unroll
+    for (int l = 0; l < 6; l++)
+    {
+        const int i = li[l];
+        const double arg1 = 0.5 * fon[i] * (1.0 + kr[l]);
+        const double sinc = 0.25 * sin(arg1) / arg1;
+        const double t1r = aev[i] * (dpl[l] * c.DPr + dcl[l] * c.DCr);
+        const double t1i = aev[i] * (dpl[l] * c.DPi + dcl[l] * c.DCi);
+        double s2, c2;
+        sincos(arg1 - arg2[i], &s2, &c2);
+        out[2 * l] = sinc * (t1r * c2 + t1i * s2);
+        out[2 * l + 1] = sinc * (-t1r * s2 + t1i * c2);
+    }
+}
+
+// Time of slow sample n (0-based) of a BW-sample series:
+// t = t0 + T*(double)(n)/(double)BW  (glass_ucb_waveform.c:843)
+__host__ __device__ inline double ucb_sample_time(double T, double t0, int n, int BW)
+{
+#pragma clang fp contract(off)
+    return t0 + T * (double)(n) / (double)BW;
+}
+
+// TDI format codes (glass_ucb_waveform.c:965-975)
+enum TdiFormat : int { TDI_PHASE = 0, TDI_FREQUENCY = 1, TDI_SANGRIA = 2 };
+
+}  // namespace glass

@@ -1,0 +1,111 @@
+"""Synthetic inputs for the UCB likelihood path (parity tests and bench).
+
+Parameter arrays follow the reference layout (map_ucb_params_to_array,
+ucb/src/glass_ucb_model.c:40-53):
+    [f0*T, cos(theta), phi, ln A, cos(iota), psi, phi0, fdot*T^2]
+Noise follows GetNoiseModel's 3-channel "sangria" branch without confusion
+noise (utils/src/glass_data.c:1156-1242) and AddNoise's per-bin Cholesky
+draw with variance C/2 on the real and imaginary parts
+(utils/src/glass_data.c:1323-1403); the random numbers come from numpy's
+PCG64, not the reference's rand_r, so realisations differ while the
+statistics agree.
+"""
+import numpy as np
+
+YEAR_T = 31457280.0          # 2^21 * 15 s, the reference default Tobs
+T_2YR = 62914560.0
+T_4YR = 125829120.0
+CLIGHT = 299792458.0
+PI2 = 6.283185307179586
+LARM = 2.5e9
+FSTAR = CLIGHT / (2.0 * np.pi * LARM)
+
+
+def draw_params(rng, n, T, f_lo=1e-3, f_hi=1e-2, fdot=(-1e-17, 1e-16), log10A=(-23.0, -21.0),
+                log_f=False):
+    """UCB parameter vectors with the configs' distributions."""
+    if log_f:
+        f0 = np.exp(rng.uniform(np.log(f_lo), np.log(f_hi), n))
+    else:
+        f0 = rng.uniform(f_lo, f_hi, n)
+    p = np.empty((n, 8))
+    p[:, 0] = f0 * T
+    p[:, 1] = rng.uniform(-1, 1, n)
+    p[:, 2] = rng.uniform(0, 2 * np.pi, n)
+    p[:, 3] = np.log(10.0 ** rng.uniform(log10A[0], log10A[1], n))
+    p[:, 4] = rng.uniform(-1, 1, n)
+    p[:, 5] = rng.uniform(0, np.pi, n)
+    p[:, 6] = rng.uniform(0, 2 * np.pi, n)
+    p[:, 7] = rng.uniform(fdot[0], fdot[1], n) * T * T
+    return p
+
+
+def noise_levels_sangria(f):
+    """get_noise_levels("sangria") (utils/src/glass_lisa.c:1107-1112)."""
+    Spm = 5.76e-30 / (PI2 * f * CLIGHT) / (PI2 * f * CLIGHT) * (1.0 + (0.4e-3 / f) ** 2) * \
+        (1.0 + (f / 8.0e-3) ** 4)
+    Sop = 1.28e-22 * (PI2 * f / CLIGHT) * (PI2 * f / CLIGHT) * (1.0 + (2.0e-3 / f) ** 4)
+    return Spm, Sop
+
+
+def xyz_noise_cov(T, fmin, N, fstar=FSTAR):
+    """C[3,3,N] for bins f = fmin + n/T: XYZnoise_FF / XYZcross_FF over 4
+    (utils/src/glass_lisa.c:1121-1135, utils/src/glass_data.c:1206-1242)."""
+    f = fmin + np.arange(N) / T
+    Spm, Sop = noise_levels_sangria(f)
+    x = f / fstar
+    cx = np.cos(x)
+    tr = np.sin(x) ** 2
+    diag = 16.0 * tr * (2.0 * (1.0 + cx * cx) * Spm + Sop) / 4.0
+    cross = -8.0 * tr * cx * (4.0 * Spm + Sop) / 4.0
+    C = np.empty((3, 3, N))
+    for a in range(3):
+        for b in range(3):
+            C[a, b] = diag if a == b else cross
+    return C
+
+
+def noise_draw(rng, C):
+    """Complex Gaussian noise [3, N] with E|n|^2 = C (re/im variance C/2)."""
+    N = C.shape[-1]
+    Cm = np.moveaxis(C, -1, 0)                   # [N,3,3]
+    Lc = np.linalg.cholesky(Cm)                  # lower
+    u = rng.standard_normal((N, 3)) + 1j * rng.standard_normal((N, 3))
+    n = np.einsum("nij,nj->ni", Lc, u) / np.sqrt(2.0)
+    return n.T.copy()
+
+
+def window_for(f0, T, nfft):
+    """Data window centred on f0 (ucb/src/glass_ucb_data.c:317-326)."""
+    fmin = f0 - (nfft // 2) / T
+    qmin = int(fmin * T)
+    return qmin
+
+
+def scatter(window, tmpl, imin):
+    """add_ucb_model (ucb/src/glass_ucb_model.c:192-232) for one template:
+    window [C, W] += tmpl [C, BW] at offset imin, clipped to the window."""
+    W = window.shape[-1]
+    BW = tmpl.shape[-1]
+    lo, hi = max(0, imin), min(W, imin + BW)
+    if hi > lo:
+        window[..., lo:hi] += tmpl[..., lo - imin:hi - imin]
+
+
+def config0(seed=20240, n=64, T=YEAR_T, BW=128, W=256, waveform=None):
+    """Plain-method case: n vectors, BW=128, one injected source in a window
+    of W bins plus noise.  waveform(params, T, BW) -> complex [5, BW]."""
+    rng = np.random.default_rng(seed)
+    P = draw_params(rng, n, T)
+    qmin = np.array([window_for(P[i, 0] / T, T, W) for i in range(n)], dtype=np.int64)
+    C = np.stack([xyz_noise_cov(T, qmin[i] / T, W) for i in range(n)])
+    data = np.zeros((n, 3, W), dtype=np.complex128)
+    imin = np.zeros(n, dtype=np.int64)
+    for i in range(n):
+        q = int(P[i, 0] / T * T)
+        imin[i] = (q - BW // 2) - qmin[i]
+        if waveform is not None:
+            h = waveform(P[i], T, BW)[:3]
+            scatter(data[i], h, imin[i])
+        data[i] += noise_draw(rng, C[i])
+    return dict(params=P, T=T, BW=BW, W=W, qmin=qmin, imin=imin, C=C, data=data, seed=seed)

@@ -1,0 +1,169 @@
+/*
+ * glass_oracle.h -- CPU restatement of the GLASS galactic-binary (UCB)
+ * likelihood path, used ONLY as test infrastructure: the parity checker for
+ * the HIP library and the timed CPU baseline.  Nothing in the product links
+ * or loads this library.
+ *
+ * Every function restates one reference function; the citation on each
+ * declaration names the reference file:line it follows (paths relative to the
+ * reference repository root).  Arithmetic is FP64 with the reference's
+ * operation order; the library is compiled with -ffp-contract=off so no
+ * multiply-add is fused (the reference's parity build, see oracle/Makefile).
+ *
+ * Pinning (what checks each function independently of this restatement):
+ *   orc_fft_*            -> bitwise vs kissfft built from the reference sources
+ *                           (oracle/_ref), brute-force O(N^2) DFT, impulse
+ *                           known answers (apps/src/fft_wdm_unit_tests.c:454-545)
+ *   orc_fourier_nwip,
+ *   orc_snr, orc_gaussian_log_likelihood,
+ *   orc_wavelet_nwip     -> closed-form sums evaluated by numpy on small inputs
+ *   orc_ucb_waveform     -> invariants the formulas fix: linearity in A, the
+ *                           phi0 global phase, psi -> psi+pi/2 sign flip, A/E as
+ *                           linear combinations of X/Y/Z, carrier-bin
+ *                           placement, and a brute-force time-domain evaluation
+ *                           of the same one-arm response (tests/)
+ *   orc_ucb_bandwidth    -> closed form by hand at chosen inputs
+ *   orc_invert_noise_covariance -> C * invC = I
+ *   orc_ucb_fisher_fd    -> symmetric, and equal to the Gram matrix of the
+ *                           finite-difference waveform derivatives
+ *   libm (glibc sin/cos/exp/log/pow): unpinned, ulp level only.
+ */
+#ifndef GLASS_ORACLE_H
+#define GLASS_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Analytic (rigid, equal-arm) LISA orbit constants.
+ * utils/src/glass_lisa.c:125-136 (initialize_analytic_orbit). */
+typedef struct
+{
+    double L;      /* arm length [m]                 */
+    double fstar;  /* transfer frequency c/(2 pi L)  */
+    double R;      /* AU * eccentricity              */
+} orc_orbit;
+
+void orc_init_analytic_orbit(orc_orbit *orbit);
+
+/* utils/src/glass_lisa.c:77-124; x,y,z are 1-indexed arrays of length 4 */
+void orc_analytic_orbits(const orc_orbit *orbit, double t, double *x, double *y, double *z);
+
+/* ---------------------------------------------------------------- FFT ---- */
+/* Mixed-radix decimation-in-time FFT with kissfft's factorisation and
+ * butterflies (external/kissfft/kiss_fft.c:15-330, twiddles :356-363).
+ * Complex data interleaved re,im.  Unnormalised in both directions. */
+int  orc_fft(const double *in, double *out, int n, int inverse);
+/* glass_forward_complex_fft, utils/src/glass_math.c:703-730 (in place) */
+void orc_forward_complex_fft(double *data, int n);
+/* glass_inverse_complex_fft, utils/src/glass_math.c:740-766 (in place, /N) */
+void orc_inverse_complex_fft(double *data, int n);
+/* kiss_fftr forward real FFT, external/kissfft/kiss_fftr.c:20-117:
+ * n real samples in, n/2+1 complex bins out (interleaved). */
+int  orc_fftr(const double *timedata, double *freqdata, int n);
+/* kiss_fftri inverse real FFT (unnormalised), kiss_fftr.c:119-153 */
+int  orc_fftri(const double *freqdata, double *timedata, int n);
+
+/* ------------------------------------------------------- FD waveform ---- */
+/* ucb/src/glass_ucb_waveform.c:738-1001.  format: "sangria" (default
+ * production TDI, utils/src/glass_lisa.c:645-710), "phase" (:458-549) or
+ * "frequency" (:551-643).  X..E receive 2*BW doubles each (template-local
+ * bins; bin m <-> absolute bin q - BW/2 + m).  Returns 0, or -1 for an
+ * unsupported format (the reference prints and exits, :971-975). */
+int orc_ucb_waveform(const orc_orbit *orbit, const char *format, double T, double t0,
+                     const double *params, int NParams,
+                     double *X, double *Y, double *Z, double *A, double *E,
+                     int BW, int NI);
+
+/* The 6 slow link series before TDI, for tests: d[link][2*BW] after the
+ * FFT, fftshift and 1/(2BW) normalisation (glass_ucb_waveform.c:940-962).
+ * link order: 12, 21, 31, 13, 23, 32. */
+void orc_ucb_slow_series(const orc_orbit *orbit, double T, double t0, const double *params,
+                         int NParams, int BW, double *d);
+
+/* ucb/src/glass_ucb_waveform.c:698-725 */
+int orc_ucb_bandwidth(double L, double fstar, double f, double fdot, double costheta,
+                      double A, double T, int N);
+/* ucb/src/glass_ucb_waveform.c:727-736 with map_array_to_ucb_params
+ * (ucb/src/glass_ucb_model.c:25-38).  out: BW, qmin, imin */
+void orc_ucb_alignment(const orc_orbit *orbit, double T, int NFFT, int data_qmin,
+                       const double *params, int *BW, int *qmin, int *imin);
+
+/* utils/src/glass_lisa.c:1188-1198 */
+double orc_AEnoise(double L, double fstar, double f);
+/* utils/src/glass_lisa.c:1082-1119 ("sangria" levels), :1121-1135 */
+void   orc_noise_levels_sangria(double f, double *Spm, double *Sop);
+double orc_XYZnoise_FF(double fstar, double f, double Spm, double Sop);
+double orc_XYZcross_FF(double fstar, double f, double Spm, double Sop);
+/* 3-channel sangria covariance of utils/src/glass_data.c:1156-1242 (no
+ * confusion noise): C[9][N] row-major 3x3 planes for bins fmin + n/T. */
+void orc_xyz_noise_covariance(const orc_orbit *orbit, double T, double fmin, int N, double *C);
+/* utils/src/glass_math.c:240-289, Nchannel = 3.  C, invC: 9 planes of N. */
+void orc_invert_noise_covariance(const double *C, double *invC, double *logdetC, int N);
+
+/* ------------------------------------------------- inner products ------- */
+/* utils/src/glass_math.c:362-380 (factor 2, not 4) */
+double orc_fourier_nwip(const double *a, const double *b, const double *invC, int N);
+/* utils/src/glass_math.c:382-391 */
+double orc_wavelet_nwip(const double *a, const double *b, const double *invC, const int *list, int N);
+/* utils/src/glass_math.c:403-426, Nchannel = 3: template-local X,Y,Z
+ * against window-indexed invC (the reference's alignment), N bins. */
+double orc_snr(const double *X, const double *Y, const double *Z, const double *invC, int Ninv, int N);
+/* utils/src/glass_model.c:726-774.  data/model: X,Y,Z,(A,E) arrays of 2*NFFT;
+ * invC 9 planes of NFFT (row-major 3x3).  Nchannel 1 (X), 2 (A,E), 3 (XYZ). */
+double orc_gaussian_log_likelihood(const double *dX, const double *dY, const double *dZ,
+                                   const double *dA, const double *dE,
+                                   const double *mX, const double *mY, const double *mZ,
+                                   const double *mA, const double *mE,
+                                   const double *invC, int NFFT, int Nchannel);
+
+/* ucb/src/glass_ucb_model.c:192-232 (Nchannel 3): model[j] += template[i],
+ * j = i + imin, 0 <= j < NFFT */
+void orc_add_ucb_model(double *mX, double *mY, double *mZ, const double *X, const double *Y,
+                       const double *Z, int BW, int imin, int NFFT);
+
+/* ----------------------------------------------------------- Fisher ----- */
+/* ucb/src/glass_ucb_waveform.c:364-522 up to (not including) the
+ * eigensolve: forward difference eps=1e-6 on the parameter array, cosine
+ * clipping, per-step ucb_alignment, Gram over wave_p->BW bins of 9
+ * fourier_nwip channel-pair terms with invC taken from window bin 0,
+ * NaN -> 10.  Nchannel 3, "sangria" unless format given. fisher: 8x8. */
+void orc_ucb_fisher_fd(const orc_orbit *orbit, const char *format, double T, double t0,
+                       int NFFT, int data_qmin, const double *params,
+                       const double *invC, double *fisher);
+
+/* ------------------------------------------------------ CPU baseline ---- */
+/* One MCMC-step likelihood evaluation the way the reference's sampler
+ * performs it (ucb/src/glass_ucb_sampler.c:139-155 -> generate_ucb_model,
+ * ucb/src/glass_ucb_model.c:55-103, then gaussian_log_likelihood):
+ * zero the window model, regenerate source `sid` from `params`, add every
+ * source's stored template, return the full-window logL.  Templates of the
+ * other sources are taken from `tmpl` (3 channels x 2*BW each, laid out as
+ * described in orc_batch.c).  Evaluations are independent and run under
+ * OpenMP with nthreads threads (0 = runtime default). */
+typedef struct
+{
+    int     nsrc;         /* live sources of the chain                       */
+    int     W;            /* window bins (NFFT)                              */
+    int     qmin;         /* absolute bin of window bin 0                    */
+    const double *data;   /* 3 x 2W: X,Y,Z data                              */
+    const double *invC;   /* 9 x W                                           */
+    const double *src_params; /* nsrc x 8 current parameters                 */
+    const double *tmpl;   /* nsrc x 3 x 2*BWmax current templates (bins 0..BW) */
+    const int    *src_bw;   /* nsrc current BW                              */
+    const int    *src_imin; /* nsrc current imin                            */
+    int     bwmax;        /* template stride                                 */
+} orc_chain;
+
+double orc_eval_logl(const orc_orbit *orbit, double T, double t0, const orc_chain *chain,
+                     int sid, const double *params);
+/* Evaluate n proposals (chain index, source index, params[8] each);
+ * returns logL in out.  nthreads 0 = default. Returns wall seconds. */
+double orc_eval_batch(const orc_orbit *orbit, double T, double t0, const orc_chain *chains,
+                      int n, const int *chain_of, const int *sid_of, const double *params,
+                      double *out, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
