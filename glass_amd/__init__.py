@@ -68,6 +68,14 @@ def lib():
             raise GlassHipError(
                 f"{LIB_PATH} is missing: build it with `python -m glass_amd.build` "
                 "(there is no CPU fallback)")
+        # torch ships its own HIP runtime under the same soname
+        # (libamdhip64.so.7).  Loading it first makes this library bind to
+        # that one runtime instead of pulling in a second copy from
+        # /opt/rocm, which would leave two HIP runtimes in one process.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         _declare(L)
         _lib = L

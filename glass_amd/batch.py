@@ -1,0 +1,146 @@
+"""Python binding of the batched device API (include/glass_hip_batch.h).
+
+Device memory and streams come from torch (ROCm build); the compute is the
+HIP library's.  FdBatch mirrors one PT-MCMC sweep's likelihood work: many
+chains x temperatures x sources, one proposal per work item.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _EXTRA, _f64, check, lib
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+_i, _d = ctypes.c_int, ctypes.c_double
+
+_EXTRA += [
+    ("glass_hip_fd_batch_create", [ctypes.POINTER(_vp), _vp, ctypes.c_char_p, _d, _d, _i, _ip, _ip, _dp, _dp,
+                                   _i, _ip, _ip, _dp, _i], _i),
+    ("glass_hip_fd_batch_destroy", [_vp], _i),
+    ("glass_hip_fd_batch_buffers", [_vp, _vp, _vp, _vp, _vp, _vp], _i),
+    ("glass_hip_fd_batch_eval", [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp], _i),
+    ("glass_hip_fd_batch_eval_graph", [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp], _i),
+    ("glass_hip_fd_batch_accept", [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp], _i),
+    ("glass_hip_fd_batch_refresh", [_vp, _vp], _i),
+    ("glass_hip_fd_batch_state", [_vp, _dp, _dp], _i),
+    ("glass_hip_ucb_bandwidth", [_d] * 7 + [_i], _i),
+    ("glass_hip_ucb_alignment", [_vp, _d, _i, _i, _dp, _ip, _ip, _ip], _i),
+]
+
+
+def _L():
+    L = lib()
+    # _declare ran before this module extended _EXTRA if lib() was loaded
+    # earlier: (re)declare the entry points here
+    for name, args, res in _EXTRA:
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    return L
+
+
+def ucb_alignment(orbit, T, NFFT, data_qmin, params):
+    """ucb_alignment (ucb/src/glass_ucb_waveform.c:727-736) -> (BW, qmin, imin)."""
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    bw, qm, im = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    check(_L().glass_hip_ucb_alignment(ctypes.cast(ctypes.byref(orbit), _vp), float(T), int(NFFT),
+                                       int(data_qmin), p.ctypes.data_as(_dp), ctypes.byref(bw),
+                                       ctypes.byref(qm), ctypes.byref(im)), "ucb_alignment")
+    return bw.value, qm.value, im.value
+
+
+def pack_invC(invC33):
+    """[3,3,W] -> [6,W] planes XX, YY, ZZ, XY, XZ, YZ."""
+    return np.stack([invC33[0, 0], invC33[1, 1], invC33[2, 2], invC33[0, 1], invC33[0, 2], invC33[1, 2]])
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class FdBatch:
+    """A device-resident set of data windows and chains (see the header)."""
+
+    def __init__(self, orbit, fmt, T, t0, group_qmin, data, invC, chain_group, chain_nsrc, src_params,
+                 max_prop):
+        """data: list of complex [3, W_g]; invC: list of [3,3,W_g];
+        src_params: [nchain, 8, 8]."""
+        import torch
+        self.torch = torch
+        L = _L()
+        W = np.array([d.shape[-1] for d in data], dtype=np.int32)
+        dflat = np.concatenate([np.ascontiguousarray(d, dtype=np.complex128).reshape(-1) for d in data])
+        iflat = np.concatenate([pack_invC(np.asarray(c)).reshape(-1) for c in invC])
+        gq = np.ascontiguousarray(group_qmin, dtype=np.int32)
+        cg = np.ascontiguousarray(chain_group, dtype=np.int32)
+        cn = np.ascontiguousarray(chain_nsrc, dtype=np.int32)
+        sp = np.ascontiguousarray(src_params, dtype=np.float64)
+        assert sp.shape == (len(cg), 8, 8)
+        self.nchain = len(cg)
+        self.max_prop = int(max_prop)
+        h = _vp()
+        dv = _f64(dflat)
+        check(L.glass_hip_fd_batch_create(ctypes.byref(h), ctypes.cast(ctypes.byref(orbit), _vp),
+                                          fmt.encode(), float(T), float(t0), len(W),
+                                          gq.ctypes.data_as(_ip), W.ctypes.data_as(_ip),
+                                          dv.ctypes.data_as(_dp), _f64(iflat).ctypes.data_as(_dp),
+                                          len(cg), cg.ctypes.data_as(_ip), cn.ctypes.data_as(_ip),
+                                          sp.ctypes.data_as(_dp), self.max_prop), "fd_batch_create")
+        self.h = h
+        dev = torch.device("cuda")
+        self.chain = torch.zeros(self.max_prop, dtype=torch.int32, device=dev)
+        self.src = torch.zeros(self.max_prop, dtype=torch.int32, device=dev)
+        self.params = torch.zeros((self.max_prop, 8), dtype=torch.float64, device=dev)
+        self.logL = torch.zeros(self.max_prop, dtype=torch.float64, device=dev)
+        self.snr = torch.zeros(self.max_prop, dtype=torch.float64, device=dev)
+
+    def close(self):
+        if getattr(self, "h", None):
+            _L().glass_hip_fd_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_proposals(self, chain, src, params):
+        n = len(chain)
+        t = self.torch
+        self.chain[:n].copy_(t.as_tensor(np.asarray(chain, dtype=np.int32)))
+        self.src[:n].copy_(t.as_tensor(np.asarray(src, dtype=np.int32)))
+        self.params[:n].copy_(t.as_tensor(np.asarray(params, dtype=np.float64)))
+        return n
+
+    def eval(self, n, graph=False):
+        """Evaluates the first n proposals in the plan's torch buffers on the
+        current torch stream; returns (logL, snr) views (device)."""
+        s = ctypes.c_void_p(self.torch.cuda.current_stream().cuda_stream)
+        fn = _L().glass_hip_fd_batch_eval_graph if graph else _L().glass_hip_fd_batch_eval
+        check(fn(self.h, int(n), _ptr(self.chain), _ptr(self.src), _ptr(self.params), _ptr(self.logL),
+                 _ptr(self.snr), s), "fd_batch_eval")
+        return self.logL[:n], self.snr[:n]
+
+    def accept(self, idx):
+        """Accept proposals idx (indices into the last evaluated buffers)."""
+        t = self.torch
+        d_idx = t.as_tensor(np.asarray(idx, dtype=np.int32)).to(self.chain.device)
+        s = ctypes.c_void_p(t.cuda.current_stream().cuda_stream)
+        check(_L().glass_hip_fd_batch_accept(self.h, len(idx), _ptr(d_idx), _ptr(self.chain), _ptr(self.src),
+                                             _ptr(self.params), _ptr(self.logL), s), "fd_batch_accept")
+        t.cuda.current_stream().synchronize()
+
+    def refresh(self):
+        s = ctypes.c_void_p(self.torch.cuda.current_stream().cuda_stream)
+        check(_L().glass_hip_fd_batch_refresh(self.h, s), "fd_batch_refresh")
+
+    def state(self):
+        self.torch.cuda.current_stream().synchronize()
+        lg = np.zeros(self.nchain)
+        sp = np.zeros((self.nchain, 8, 8))
+        check(_L().glass_hip_fd_batch_state(self.h, lg.ctypes.data_as(_dp), sp.ctypes.data_as(_dp)),
+              "fd_batch_state")
+        return lg, sp

@@ -1,0 +1,764 @@
+// fd_batch.hip -- batched, device-resident UCB waveform + delta-logL
+// (include/glass_hip_batch.h).
+//
+// One evaluation = one MCMC proposal of the reference's ucb_mcmc step
+// (ucb/src/glass_ucb_sampler.c:139-155): regenerate one source's template
+// and evaluate the chain's likelihood and the template's snr.  The sweep is
+// a fixed launch sequence with no host synchronisation:
+//   1. zero the BW-class counters (a kernel, not a memset: graph-safe)
+//   2. align_kernel: ucb_alignment per proposal on the device; bucket the
+//      proposal into its BW class (32..1024) with an atomic counter
+//   3. per BW class, a persistent eval_kernel<N>: each workgroup takes
+//      proposals from its class list, builds the template in LDS (slow
+//      series, 6 FFTs, TDI) and reduces (r|dh) - (dh|dh)/2 over the union of
+//      the old and new supports only, reading the chain residual, the
+//      group's invC and the source's stored template with coalesced loads.
+// The full-window likelihood is therefore never re-read per proposal: the
+// HBM traffic per proposal is ~144 bytes per template bin.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../../include/glass_hip_batch.h"
+#include "fd_waveform.h"
+#include "reduce.h"
+#include "runtime.h"
+
+namespace glass {
+
+int parse_format(const char* format);
+
+constexpr int kMaxSrc = 8;
+constexpr int kNClass = 8;     // BW = 32 << k, k = 0..7 (32..4096)
+constexpr int kPosN = 4096;    // positions table samples (largest BW)
+
+struct BatchDev
+{
+    // groups
+    const int* g_qmin;
+    const int* g_W;
+    const long long* g_off;  // bins before group g
+    const double2* data;     // [3][W_g] per group
+    const double* invC;      // [6][W_g] per group
+    // chains
+    const int* c_group;
+    const int* c_nsrc;
+    const long long* c_roff;  // bins before chain c's residual
+    double2* resid;           // [3][W] per chain
+    double* logl;             // [C]
+    double* sparams;          // [C][8][8]
+    int* s_bw;                // [C*8]
+    int* s_imin;              // [C*8]
+    double2* tmpl;            // [C*8][3][bwcap]
+    int bwcap;                // template slot size: largest BW any window allows
+    // constants
+    double T, t0, L, fstar;
+    const double* pos;  // positions table, kPosN samples
+    const double2* tw;
+    int fmt;
+};
+
+// Work items: item e refers to entry q = sel ? sel[e] : e of the
+// (chain, src, params[8]) arrays.
+struct Items
+{
+    const int* chain;
+    const int* src;
+    const double* params;
+    const int* sel;
+};
+
+__device__ inline int item_q(const Items& it, int e) { return it.sel ? it.sel[e] : e; }
+__device__ inline const double* item_params(const Items& it, int e) { return it.params + (size_t)item_q(it, e) * 8; }
+__device__ inline int item_chain(const Items& it, int e) { return it.chain[item_q(it, e)]; }
+__device__ inline int item_src(const Items& it, int e) { return it.src[item_q(it, e)]; }
+
+__global__ void zero_int_kernel(int* p, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// ucb_alignment on the device (ucb/src/glass_ucb_waveform.c:727-736) and
+// bucketing by BW class.  A BW above the cap marks the item with class -1.
+__global__ void align_kernel(int n, Items it, BatchDev B, int* __restrict__ bw_out, int* __restrict__ imin_out,
+                             int* __restrict__ cls_count, int* __restrict__ cls_list, int cls_stride,
+                             int* __restrict__ err)
+{
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const double* p = item_params(it, e);
+    int c = item_chain(it, e);
+    int g = B.c_group[c];
+    int W = B.g_W[g];
+    double f0 = p[0] / B.T;
+    double costh = p[1], amp = exp(p[3]);
+    double dfdt = p[7] / (B.T * B.T);
+    int BW = 2 * ucb_bandwidth(B.L, B.fstar, f0, dfdt, costh, amp, B.T, W);
+    int qmin = (int)(f0 * B.T) - BW / 2;
+    bw_out[e] = BW;
+    imin_out[e] = qmin - B.g_qmin[g];
+    int k = 31 - __clz(BW) - 5;  // log2(BW) - 5
+    if (BW > B.bwcap || k < 0 || k >= kNClass || (BW & (BW - 1)))
+    {
+        atomicAdd(err, 1);
+        return;
+    }
+    int pos = atomicAdd(&cls_count[k], 1);
+    cls_list[k * cls_stride + pos] = e;
+}
+
+// symmetric 3x3 weighted bilinear form of complex 3-vectors:
+// 2 * sum_ab Re(u_a conj v_b) w_ab, w planes XX,YY,ZZ,XY,XZ,YZ
+__device__ inline double bform(const double2* u, const double2* v, const double* w)
+{
+    auto re = [](double2 a, double2 b) { return a.x * b.x + a.y * b.y; };
+    double d = w[0] * re(u[0], v[0]) + w[1] * re(u[1], v[1]) + w[2] * re(u[2], v[2]);
+    double o = w[3] * (re(u[0], v[1]) + re(u[1], v[0])) + w[4] * (re(u[0], v[2]) + re(u[2], v[0])) +
+               w[5] * (re(u[1], v[2]) + re(u[2], v[1]));
+    return 2.0 * (d + o);
+}
+
+__device__ inline void load_w(const double* __restrict__ iC, int W, int j, double* w)
+{
+#pragma unroll
+    for (int q = 0; q < 6; q++) w[q] = iC[(size_t)q * W + j];
+}
+
+// Fused waveform + delta-logL + snr for the proposals of one BW class.
+template <int N, int NTHR>
+__global__ __launch_bounds__(NTHR) void eval_kernel(const int* __restrict__ list, const int* __restrict__ count,
+                                                    Items it, BatchDev B, const int* __restrict__ p_imin,
+                                                    double* __restrict__ logl_out, double* __restrict__ snr_out)
+{
+    using S = FdShape<N, NTHR>;
+    __shared__ double2 lds[S::LDS];
+    const int cnt = *count;
+    for (int w = blockIdx.x; w < cnt; w += gridDim.x)
+    {
+        const int e = list[w];
+        const int c = item_chain(it, e), s = item_src(it, e);
+        const int g = B.c_group[c];
+        const int W = B.g_W[g];
+        UcbConst k;
+        ucb_setup(item_params(it, e), 8, B.T, B.t0, B.L, B.fstar, k);
+        double2 dl[S::PER][6];
+        fd_link_spectra<N, NTHR>(k, lds, B.pos, kPosN / N, B.tw, kTwN / N, dl);
+
+        const int so = c * kMaxSrc + s;
+        const int imin_n = p_imin[e];
+        const int imin_o = B.s_imin[so], bw_o = B.s_bw[so];
+        const double2* ho = B.tmpl + (size_t)so * 3 * B.bwcap;
+        const double2* R = B.resid + (size_t)B.c_roff[c] * 3;
+        const double* iC = B.invC + (size_t)B.g_off[g] * 6;
+        double acc = 0.0, s2 = 0.0;
+        // new-template bins (and the old template where they overlap)
+#pragma unroll
+        for (int kk = 0; kk < S::PER; kk++)
+        {
+            const int m = threadIdx.x + kk * NTHR;
+            if (m >= N) continue;
+            double2 v[5];
+            fd_tdi<N>(k, m, B.fmt, dl[kk], v);
+            double wq[6];
+            load_w(iC, W, m, wq);  // snr: template bin m against window bin m
+            s2 += bform(v, v, wq);
+            const int j = imin_n + m;
+            if (j >= 0 && j < W)
+            {
+                double2 d[3], r[3];
+                const int mo = j - imin_o;
+                const bool inold = mo >= 0 && mo < bw_o;
+#pragma unroll
+                for (int ch = 0; ch < 3; ch++)
+                {
+                    double2 h0 = inold ? ho[ch * B.bwcap + mo] : make_double2(0.0, 0.0);
+                    d[ch] = make_double2(v[ch].x - h0.x, v[ch].y - h0.y);
+                    r[ch] = R[(size_t)ch * W + j];
+                }
+                load_w(iC, W, j, wq);
+                acc += bform(r, d, wq) - 0.5 * bform(d, d, wq);
+            }
+        }
+        // old-template bins outside the new support: dh = -h_old
+        for (int mo = threadIdx.x; mo < bw_o; mo += NTHR)
+        {
+            const int j = imin_o + mo;
+            if (j < 0 || j >= W || (j >= imin_n && j < imin_n + N)) continue;
+            double2 d[3], r[3];
+            double wq[6];
+#pragma unroll
+            for (int ch = 0; ch < 3; ch++)
+            {
+                double2 h0 = ho[ch * B.bwcap + mo];
+                d[ch] = make_double2(-h0.x, -h0.y);
+                r[ch] = R[(size_t)ch * W + j];
+            }
+            load_w(iC, W, j, wq);
+            acc += bform(r, d, wq) - 0.5 * bform(d, d, wq);
+        }
+        acc = block_sum<NTHR < 64 ? 64 : NTHR>(acc);
+        s2 = block_sum<NTHR < 64 ? 64 : NTHR>(s2);
+        if (threadIdx.x == 0)
+        {
+            logl_out[e] = B.logl[c] + acc;
+            snr_out[e] = sqrt(s2);
+        }
+    }
+}
+
+// Template of each listed item into dst slot (slot index = dslot[e] or e).
+template <int N, int NTHR>
+__global__ __launch_bounds__(NTHR) void tmpl_kernel(const int* __restrict__ list, const int* __restrict__ count,
+                                                    Items it, BatchDev B, double2* __restrict__ dst,
+                                                    const int* __restrict__ dslot)
+{
+    using S = FdShape<N, NTHR>;
+    __shared__ double2 lds[S::LDS];
+    const int cnt = *count;
+    for (int w = blockIdx.x; w < cnt; w += gridDim.x)
+    {
+        const int e = list[w];
+        UcbConst k;
+        ucb_setup(item_params(it, e), 8, B.T, B.t0, B.L, B.fstar, k);
+        double2 dl[S::PER][6];
+        fd_link_spectra<N, NTHR>(k, lds, B.pos, kPosN / N, B.tw, kTwN / N, dl);
+        double2* o = dst + (size_t)(dslot ? dslot[e] : e) * 3 * B.bwcap;
+#pragma unroll
+        for (int kk = 0; kk < S::PER; kk++)
+        {
+            const int m = threadIdx.x + kk * NTHR;
+            if (m >= N) continue;
+            double2 v[5];
+            fd_tdi<N>(k, m, B.fmt, dl[kk], v);
+#pragma unroll
+            for (int ch = 0; ch < 3; ch++) o[ch * B.bwcap + m] = v[ch];
+        }
+    }
+}
+
+// r_c = d_g - sum_s h_cs (add_ucb_model, ucb/src/glass_ucb_model.c:192-232)
+__global__ void resid_kernel(BatchDev B)
+{
+    const int c = blockIdx.y;
+    const int g = B.c_group[c], W = B.g_W[g];
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= W) return;
+    const double2* D = B.data + (size_t)B.g_off[g] * 3;
+    double2* R = B.resid + (size_t)B.c_roff[c] * 3;
+    for (int ch = 0; ch < 3; ch++)
+    {
+        double2 v = D[(size_t)ch * W + j];
+        for (int s = 0; s < B.c_nsrc[c]; s++)
+        {
+            const int so = c * kMaxSrc + s;
+            const int m = j - B.s_imin[so];
+            if (m >= 0 && m < B.s_bw[so])
+            {
+                double2 h = B.tmpl[(size_t)so * 3 * B.bwcap + ch * B.bwcap + m];
+                v.x -= h.x;
+                v.y -= h.y;
+            }
+        }
+        R[(size_t)ch * W + j] = v;
+    }
+}
+
+// logL_c = -1/2 (r_c | r_c) over the whole window
+__global__ __launch_bounds__(256) void chain_logl_kernel(BatchDev B)
+{
+    const int c = blockIdx.x;
+    const int g = B.c_group[c], W = B.g_W[g];
+    const double2* R = B.resid + (size_t)B.c_roff[c] * 3;
+    const double* iC = B.invC + (size_t)B.g_off[g] * 6;
+    double acc = 0.0;
+    for (int j = threadIdx.x; j < W; j += 256)
+    {
+        double2 r[3];
+        double w[6];
+        for (int ch = 0; ch < 3; ch++) r[ch] = R[(size_t)ch * W + j];
+        load_w(iC, W, j, w);
+        acc += bform(r, r, w);
+    }
+    acc = block_sum<256>(acc);
+    if (threadIdx.x == 0) B.logl[c] = -0.5 * acc;
+}
+
+// Accept: residual += h_old - h_new over the union of supports.
+__global__ void accept_resid_kernel(int nacc, Items it, BatchDev B,
+                                    const double2* __restrict__ hnew, const int* __restrict__ bw_new,
+                                    const int* __restrict__ imin_new)
+{
+    const int a = blockIdx.y;
+    if (a >= nacc) return;
+    const int c = item_chain(it, a), s = item_src(it, a), so = c * kMaxSrc + s;
+    const int g = B.c_group[c], W = B.g_W[g];
+    const int in = imin_new[a], bn = bw_new[a], io = B.s_imin[so], bo = B.s_bw[so];
+    const int lo = max(0, min(in, io)), hi = min(W, max(in + bn, io + bo));
+    const int j = lo + blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= hi) return;
+    double2* R = B.resid + (size_t)B.c_roff[c] * 3;
+    const double2* ho = B.tmpl + (size_t)so * 3 * B.bwcap;
+    const double2* hn = hnew + (size_t)a * 3 * B.bwcap;
+    for (int ch = 0; ch < 3; ch++)
+    {
+        double2 v = R[(size_t)ch * W + j];
+        int mo = j - io, mn = j - in;
+        if (mo >= 0 && mo < bo)
+        {
+            v.x += ho[ch * B.bwcap + mo].x;
+            v.y += ho[ch * B.bwcap + mo].y;
+        }
+        if (mn >= 0 && mn < bn)
+        {
+            v.x -= hn[ch * B.bwcap + mn].x;
+            v.y -= hn[ch * B.bwcap + mn].y;
+        }
+        R[(size_t)ch * W + j] = v;
+    }
+}
+
+__global__ void accept_commit_kernel(int nacc, Items it, BatchDev B, const double2* __restrict__ hnew,
+                                     const int* __restrict__ bw_new, const int* __restrict__ imin_new,
+                                     const double* __restrict__ logl_new)
+{
+    const int a = blockIdx.x;
+    if (a >= nacc) return;
+    const int c = item_chain(it, a), s = item_src(it, a), so = c * kMaxSrc + s;
+    const double2* hn = hnew + (size_t)a * 3 * B.bwcap;
+    double2* ho = B.tmpl + (size_t)so * 3 * B.bwcap;
+    for (int i = threadIdx.x; i < 3 * B.bwcap; i += blockDim.x) ho[i] = hn[i];
+    if (threadIdx.x < 8) B.sparams[(size_t)so * 8 + threadIdx.x] = item_params(it, a)[threadIdx.x];
+    if (threadIdx.x == 0)
+    {
+        B.s_bw[so] = bw_new[a];
+        B.s_imin[so] = imin_new[a];
+        B.logl[c] = logl_new[item_q(it, a)];
+    }
+}
+
+}  // namespace glass
+
+using namespace glass;
+
+struct glass_hip_fd_batch
+{
+    int device = 0;
+    glass_hip_orbit orbit{};
+    int fmt = TDI_SANGRIA;
+    double T = 0, t0 = 0;
+    int G = 0, C = 0, maxprop = 0, ninit = 0;
+    int Wmax = 0;
+    std::vector<void*> allocs;
+    BatchDev B{};
+    // proposal buffers
+    int *p_chain = nullptr, *p_src = nullptr;
+    double *p_params = nullptr, *p_logl = nullptr, *p_snr = nullptr;
+    int *p_bw = nullptr, *p_imin = nullptr;
+    int* cls = nullptr;       // [kNClass] counts + [1] error
+    int* cls_list = nullptr;  // [kNClass][cap]
+    int cls_cap = 0;
+    // init items: (chain, src) of every live source, its params gathered
+    int *i_chain = nullptr, *i_src = nullptr, *i_slot = nullptr;
+    double* i_params = nullptr;
+    double2* hnew = nullptr;  // accept scratch [C][3][bwcap]
+    int grid[kNClass] = {0};
+    int tgrid[kNClass] = {0};
+    // graph cache
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int g_nprop = -1;
+    const void* g_key[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+};
+
+template <typename T>
+static int dalloc(glass_hip_fd_batch* P, T** p, size_t n)
+{
+    void* q = nullptr;
+    GH_CHECK(hipMalloc(&q, sizeof(T) * (n ? n : 1)));
+    P->allocs.push_back(q);
+    *p = (T*)q;
+    return GLASS_HIP_OK;
+}
+
+#define GH_NTHR_(N) ((N) < 64 ? 64 : ((N) > 1024 ? 1024 : (N)))
+#define GH_NTHR(N) GH_NTHR_(N)
+
+// Persistent grid: workgroups resident per CU as bounded by LDS (160 KB)
+// and by 32 waves per CU, times the CU count.  A larger grid than actually
+// fits only delays some workgroups; the loop bounds stay correct.
+template <int N>
+static int occ_grid(const void* fn, int nthr, int* grid)
+{
+    (void)fn;
+    int dev = 0, ncu = 0;
+    GH_CHECK(hipGetDevice(&dev));
+    GH_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const int lds = (int)sizeof(double2) * FdShape<N, GH_NTHR_(N)>::LDS + 512;
+    int per = (160 * 1024) / lds;
+    int wv = 32 / ((nthr + 63) / 64);
+    if (wv < per) per = wv;
+    if (per < 1) per = 1;
+    *grid = ncu * per;
+    return GLASS_HIP_OK;
+}
+
+
+static int setup_grids(glass_hip_fd_batch* P)
+{
+    int rc;
+#define GH_G(k, N)                                                                                          \
+    if ((rc = occ_grid<N>((const void*)eval_kernel<N, GH_NTHR(N)>, GH_NTHR(N), &P->grid[k]))) return rc;  \
+    if ((rc = occ_grid<N>((const void*)tmpl_kernel<N, GH_NTHR(N)>, GH_NTHR(N), &P->tgrid[k]))) return rc;
+    GH_G(0, 32) GH_G(1, 64) GH_G(2, 128) GH_G(3, 256) GH_G(4, 512) GH_G(5, 1024) GH_G(6, 2048) GH_G(7, 4096)
+#undef GH_G
+    return GLASS_HIP_OK;
+}
+
+// alignment + per-class launches for items; mode 0: eval, 1: templates
+static int launch_items(glass_hip_fd_batch* P, int n, const Items& it, int mode, double* logl_out, double* snr_out,
+                        double2* dst, const int* dslot, hipStream_t s)
+{
+    if (n <= 0) return GLASS_HIP_OK;
+    if (n > P->cls_cap)
+    {
+        set_error("batch: %d items exceed capacity %d", n, P->cls_cap);
+        return GLASS_HIP_EINVAL;
+    }
+    zero_int_kernel<<<1, 64, 0, s>>>(P->cls, kNClass + 1);
+    align_kernel<<<(n + 255) / 256, 256, 0, s>>>(n, it, P->B, P->p_bw, P->p_imin, P->cls, P->cls_list, P->cls_cap,
+                                                 P->cls + kNClass);
+    for (int k = kNClass - 1; k >= 0; k--)
+    {
+        const int* list = P->cls_list + (size_t)k * P->cls_cap;
+        const int* cnt = P->cls + k;
+        int g = (mode == 0 ? P->grid[k] : P->tgrid[k]);
+        if (g > n) g = n;
+#define GH_L(kk, N)                                                                                             \
+    case kk:                                                                                                    \
+        if (mode == 0)                                                                                          \
+            eval_kernel<N, GH_NTHR(N)><<<g, GH_NTHR(N), 0, s>>>(list, cnt, it, P->B, P->p_imin, logl_out, snr_out); \
+        else                                                                                                    \
+            tmpl_kernel<N, GH_NTHR(N)><<<g, GH_NTHR(N), 0, s>>>(list, cnt, it, P->B, dst, dslot);              \
+        break;
+        switch (k)
+        {
+            GH_L(0, 32) GH_L(1, 64) GH_L(2, 128) GH_L(3, 256) GH_L(4, 512) GH_L(5, 1024) GH_L(6, 2048) GH_L(7, 4096)
+        }
+#undef GH_L
+    }
+    GH_CHECK(hipGetLastError());
+    return GLASS_HIP_OK;
+}
+
+// slot alignment from an alignment pass over the init items
+__global__ void slot_align_kernel(int n, const int* __restrict__ slot, const int* __restrict__ bw,
+                                  const int* __restrict__ imin, int* __restrict__ s_bw, int* __restrict__ s_imin)
+{
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    s_bw[slot[e]] = bw[e];
+    s_imin[slot[e]] = imin[e];
+}
+
+__global__ void gather_params_kernel(int n, const int* __restrict__ slot, const double* __restrict__ sparams,
+                                     double* __restrict__ out)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * 8) return;
+    out[i] = sparams[(size_t)slot[i / 8] * 8 + (i % 8)];
+}
+
+static int rebuild(glass_hip_fd_batch* P, hipStream_t s)
+{
+    const int n = P->ninit;
+    gather_params_kernel<<<(n * 8 + 255) / 256, 256, 0, s>>>(n, P->i_slot, P->B.sparams, P->i_params);
+    Items it{P->i_chain, P->i_src, P->i_params, nullptr};
+    int rc;
+    if ((rc = launch_items(P, n, it, 1, nullptr, nullptr, P->B.tmpl, P->i_slot, s))) return rc;
+    slot_align_kernel<<<(n + 255) / 256, 256, 0, s>>>(n, P->i_slot, P->p_bw, P->p_imin, P->B.s_bw, P->B.s_imin);
+    dim3 rg((P->Wmax + 255) / 256, P->C);
+    resid_kernel<<<rg, 256, 0, s>>>(P->B);
+    chain_logl_kernel<<<P->C, 256, 0, s>>>(P->B);
+    GH_CHECK(hipGetLastError());
+    return GLASS_HIP_OK;
+}
+
+static int check_err(glass_hip_fd_batch* P, hipStream_t s)
+{
+    int h = 0;
+    GH_CHECK(hipMemcpyAsync(&h, P->cls + kNClass, sizeof(int), hipMemcpyDeviceToHost, s));
+    GH_CHECK(hipStreamSynchronize(s));
+    if (h)
+    {
+        set_error("batch: %d items with BW outside 32..%d", h, P->B.bwcap);
+        return GLASS_HIP_EINVAL;
+    }
+    return GLASS_HIP_OK;
+}
+
+extern "C" int glass_hip_fd_batch_create(glass_hip_fd_batch** plan, const glass_hip_orbit* orbit, const char* format,
+                                         double T, double t0, int ngroup, const int* group_qmin, const int* group_W,
+                                         const double* data, const double* invC, int nchain, const int* chain_group,
+                                         const int* chain_nsrc, const double* src_params, int max_prop)
+{
+    if (!plan || !orbit || ngroup <= 0 || nchain <= 0 || max_prop < 0 || !group_qmin || !group_W || !data ||
+        !invC || !chain_group || !chain_nsrc || !src_params)
+    {
+        set_error("fd_batch_create: bad arguments");
+        return GLASS_HIP_EINVAL;
+    }
+    int fmt = parse_format(format);
+    if (fmt < 0)
+    {
+        set_error("Unsupported data format %s", format ? format : "(null)");
+        return GLASS_HIP_EINVAL;
+    }
+    std::vector<long long> goff(ngroup + 1, 0), roff(nchain + 1, 0);
+    int Wmax = 0;
+    for (int g = 0; g < ngroup; g++)
+    {
+        if (group_W[g] < 32)
+        {
+            set_error("fd_batch_create: window %d has W=%d < 32", g, group_W[g]);
+            return GLASS_HIP_EINVAL;
+        }
+        goff[g + 1] = goff[g] + group_W[g];
+        if (group_W[g] > Wmax) Wmax = group_W[g];
+    }
+    std::vector<int> ic, is, islot;
+    for (int c = 0; c < nchain; c++)
+    {
+        int g = chain_group[c];
+        if (g < 0 || g >= ngroup || chain_nsrc[c] < 0 || chain_nsrc[c] > kMaxSrc)
+        {
+            set_error("fd_batch_create: chain %d has group %d / nsrc %d", c, g, chain_nsrc[c]);
+            return GLASS_HIP_EINVAL;
+        }
+        roff[c + 1] = roff[c] + group_W[g];
+        for (int s = 0; s < chain_nsrc[c]; s++)
+        {
+            ic.push_back(c);
+            is.push_back(s);
+            islot.push_back(c * kMaxSrc + s);
+        }
+    }
+    ThreadCtx* cx;
+    int rc;
+    if ((rc = ctx_get(&cx))) return rc;
+    glass_hip_fd_batch* P = new glass_hip_fd_batch();
+    P->orbit = *orbit;
+    P->fmt = fmt;
+    P->T = T;
+    P->t0 = t0;
+    P->G = ngroup;
+    P->C = nchain;
+    P->maxprop = max_prop;
+    P->ninit = (int)ic.size();
+    P->Wmax = Wmax;
+    P->cls_cap = std::max(max_prop, std::max(P->ninit, nchain));
+    GH_CHECK(hipGetDevice(&P->device));
+    // largest BW any window allows: 2 * 2^floor(log2(W/2)) <= W
+    P->B.bwcap = 32;
+    while (P->B.bwcap * 2 <= Wmax && P->B.bwcap < kPosN) P->B.bwcap *= 2;
+    auto fail = [&](int r) {
+        glass_hip_fd_batch_destroy(P);
+        return r;
+    };
+    int *g_qmin, *g_W, *c_group, *c_nsrc;
+    long long *g_off, *c_roff;
+    double2 *ddata, *resid, *tmpl;
+    double *dinvC, *logl, *sparams;
+    int *s_bw, *s_imin;
+    const size_t nslot = (size_t)nchain * kMaxSrc;
+    if ((rc = dalloc(P, &g_qmin, ngroup)) || (rc = dalloc(P, &g_W, ngroup)) || (rc = dalloc(P, &g_off, ngroup)) ||
+        (rc = dalloc(P, &ddata, 3 * (size_t)goff[ngroup])) || (rc = dalloc(P, &dinvC, 6 * (size_t)goff[ngroup])) ||
+        (rc = dalloc(P, &c_group, nchain)) || (rc = dalloc(P, &c_nsrc, nchain)) || (rc = dalloc(P, &c_roff, nchain)) ||
+        (rc = dalloc(P, &resid, 3 * (size_t)roff[nchain])) || (rc = dalloc(P, &logl, nchain)) ||
+        (rc = dalloc(P, &sparams, nslot * 8)) || (rc = dalloc(P, &s_bw, nslot)) || (rc = dalloc(P, &s_imin, nslot)) ||
+        (rc = dalloc(P, &tmpl, nslot * 3 * (size_t)P->B.bwcap)) || (rc = dalloc(P, &P->p_chain, P->cls_cap)) ||
+        (rc = dalloc(P, &P->p_src, P->cls_cap)) || (rc = dalloc(P, &P->p_params, (size_t)P->cls_cap * 8)) ||
+        (rc = dalloc(P, &P->p_logl, P->cls_cap)) || (rc = dalloc(P, &P->p_snr, P->cls_cap)) ||
+        (rc = dalloc(P, &P->p_bw, P->cls_cap)) || (rc = dalloc(P, &P->p_imin, P->cls_cap)) ||
+        (rc = dalloc(P, &P->cls, kNClass + 1)) || (rc = dalloc(P, &P->cls_list, (size_t)kNClass * P->cls_cap)) ||
+        (rc = dalloc(P, &P->i_chain, P->ninit)) || (rc = dalloc(P, &P->i_src, P->ninit)) ||
+        (rc = dalloc(P, &P->i_slot, P->ninit)) || (rc = dalloc(P, &P->i_params, (size_t)P->ninit * 8)) ||
+        (rc = dalloc(P, &P->hnew, (size_t)nchain * 3 * P->B.bwcap)))
+        return fail(rc);
+    auto up = [&](void* d, const void* h, size_t b) { return hipMemcpy(d, h, b, hipMemcpyHostToDevice); };
+    if (up(g_qmin, group_qmin, sizeof(int) * ngroup) || up(g_W, group_W, sizeof(int) * ngroup) ||
+        up(g_off, goff.data(), sizeof(long long) * ngroup) || up(ddata, data, 48 * (size_t)goff[ngroup]) ||
+        up(dinvC, invC, 48 * (size_t)goff[ngroup]) || up(c_group, chain_group, sizeof(int) * nchain) ||
+        up(c_nsrc, chain_nsrc, sizeof(int) * nchain) || up(c_roff, roff.data(), sizeof(long long) * nchain) ||
+        up(sparams, src_params, sizeof(double) * nslot * 8) ||
+        (P->ninit && (up(P->i_chain, ic.data(), sizeof(int) * P->ninit) ||
+                      up(P->i_src, is.data(), sizeof(int) * P->ninit) ||
+                      up(P->i_slot, islot.data(), sizeof(int) * P->ninit))))
+    {
+        set_error("fd_batch_create: upload failed");
+        return fail(GLASS_HIP_ERUNTIME);
+    }
+    const double* pos;
+    const double2* tw;
+    if ((rc = positions(*orbit, T, t0, kPosN, &pos)) || (rc = twiddles(&tw))) return fail(rc);
+    BatchDev& B = P->B;
+    B.g_qmin = g_qmin;
+    B.g_W = g_W;
+    B.g_off = g_off;
+    B.data = ddata;
+    B.invC = dinvC;
+    B.c_group = c_group;
+    B.c_nsrc = c_nsrc;
+    B.c_roff = c_roff;
+    B.resid = resid;
+    B.logl = logl;
+    B.sparams = sparams;
+    B.s_bw = s_bw;
+    B.s_imin = s_imin;
+    B.tmpl = tmpl;
+    B.T = T;
+    B.t0 = t0;
+    B.L = orbit->L;
+    B.fstar = orbit->fstar;
+    B.pos = pos;
+    B.tw = tw;
+    B.fmt = fmt;
+    if ((rc = setup_grids(P))) return fail(rc);
+    if ((rc = rebuild(P, cx->stream)) || (rc = check_err(P, cx->stream))) return fail(rc);
+    *plan = P;
+    return GLASS_HIP_OK;
+}
+
+extern "C" int glass_hip_fd_batch_destroy(glass_hip_fd_batch* P)
+{
+    if (!P) return GLASS_HIP_OK;
+    (void)hipSetDevice(P->device);
+    if (P->gexec) (void)hipGraphExecDestroy(P->gexec);
+    if (P->graph) (void)hipGraphDestroy(P->graph);
+    (void)hipDeviceSynchronize();
+    for (void* p : P->allocs) (void)hipFree(p);
+    delete P;
+    return GLASS_HIP_OK;
+}
+
+extern "C" int glass_hip_fd_batch_buffers(glass_hip_fd_batch* P, int** d_chain, int** d_src, double** d_params,
+                                          double** d_logL, double** d_snr)
+{
+    if (!P)
+    {
+        set_error("fd_batch_buffers: null plan");
+        return GLASS_HIP_EINVAL;
+    }
+    if (d_chain) *d_chain = P->p_chain;
+    if (d_src) *d_src = P->p_src;
+    if (d_params) *d_params = P->p_params;
+    if (d_logL) *d_logL = P->p_logl;
+    if (d_snr) *d_snr = P->p_snr;
+    return GLASS_HIP_OK;
+}
+
+extern "C" int glass_hip_fd_batch_eval(glass_hip_fd_batch* P, int nprop, const int* d_chain, const int* d_src,
+                                       const double* d_params, double* d_logL, double* d_snr, void* stream)
+{
+    if (!P || nprop < 0 || nprop > P->maxprop || (nprop && (!d_chain || !d_src || !d_params || !d_logL || !d_snr)))
+    {
+        set_error("fd_batch_eval: bad arguments (nprop %d, max %d)", nprop, P ? P->maxprop : 0);
+        return GLASS_HIP_EINVAL;
+    }
+    GH_CHECK(hipSetDevice(P->device));
+    Items it{d_chain, d_src, d_params, nullptr};
+    return launch_items(P, nprop, it, 0, d_logL, d_snr, nullptr, nullptr, (hipStream_t)stream);
+}
+
+extern "C" int glass_hip_fd_batch_eval_graph(glass_hip_fd_batch* P, int nprop, const int* d_chain, const int* d_src,
+                                             const double* d_params, double* d_logL, double* d_snr, void* stream)
+{
+    if (!P)
+    {
+        set_error("fd_batch_eval_graph: null plan");
+        return GLASS_HIP_EINVAL;
+    }
+    GH_CHECK(hipSetDevice(P->device));
+    hipStream_t s = (hipStream_t)stream;
+    const void* key[5] = {d_chain, d_src, d_params, d_logL, d_snr};
+    bool same = P->gexec && P->g_nprop == nprop;
+    for (int i = 0; i < 5 && same; i++) same = key[i] == P->g_key[i];
+    if (!same)
+    {
+        if (P->gexec) GH_CHECK(hipGraphExecDestroy(P->gexec));
+        if (P->graph) GH_CHECK(hipGraphDestroy(P->graph));
+        P->gexec = nullptr;
+        P->graph = nullptr;
+        // capture on a private stream so the caller's stream may be the
+        // legacy default stream
+        hipStream_t cs;
+        GH_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        GH_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        int rc = glass_hip_fd_batch_eval(P, nprop, d_chain, d_src, d_params, d_logL, d_snr, cs);
+        hipGraph_t g = nullptr;
+        hipError_t ec = hipStreamEndCapture(cs, &g);
+        (void)hipStreamDestroy(cs);
+        if (rc) return rc;
+        if (ec != hipSuccess)
+        {
+            set_error("graph capture failed: %s", hipGetErrorString(ec));
+            return GLASS_HIP_ERUNTIME;
+        }
+        P->graph = g;
+        GH_CHECK(hipGraphInstantiate(&P->gexec, g, nullptr, nullptr, 0));
+        P->g_nprop = nprop;
+        for (int i = 0; i < 5; i++) P->g_key[i] = key[i];
+    }
+    GH_CHECK(hipGraphLaunch(P->gexec, s));
+    return GLASS_HIP_OK;
+}
+
+extern "C" int glass_hip_fd_batch_accept(glass_hip_fd_batch* P, int nacc, const int* d_idx, const int* d_chain,
+                                         const int* d_src, const double* d_params, const double* d_logL, void* stream)
+{
+    if (!P || nacc < 0 || nacc > P->C || (nacc && (!d_idx || !d_chain || !d_src || !d_params || !d_logL)))
+    {
+        set_error("fd_batch_accept: bad arguments");
+        return GLASS_HIP_EINVAL;
+    }
+    if (nacc == 0) return GLASS_HIP_OK;
+    GH_CHECK(hipSetDevice(P->device));
+    hipStream_t s = (hipStream_t)stream;
+    Items it{d_chain, d_src, d_params, d_idx};
+    int rc;
+    // new templates into the scratch slots 0..nacc (item a -> slot a)
+    if ((rc = launch_items(P, nacc, it, 1, nullptr, nullptr, P->hnew, nullptr, s))) return rc;
+    dim3 g((2 * P->B.bwcap + 255) / 256, nacc);
+    accept_resid_kernel<<<g, 256, 0, s>>>(nacc, it, P->B, P->hnew, P->p_bw, P->p_imin);
+    accept_commit_kernel<<<nacc, 256, 0, s>>>(nacc, it, P->B, P->hnew, P->p_bw, P->p_imin, d_logL);
+    GH_CHECK(hipGetLastError());
+    return GLASS_HIP_OK;
+}
+
+extern "C" int glass_hip_fd_batch_refresh(glass_hip_fd_batch* P, void* stream)
+{
+    if (!P)
+    {
+        set_error("fd_batch_refresh: null plan");
+        return GLASS_HIP_EINVAL;
+    }
+    GH_CHECK(hipSetDevice(P->device));
+    return rebuild(P, (hipStream_t)stream);
+}
+
+extern "C" int glass_hip_fd_batch_state(glass_hip_fd_batch* P, double* logL, double* src_params)
+{
+    if (!P || !logL)
+    {
+        set_error("fd_batch_state: bad arguments");
+        return GLASS_HIP_EINVAL;
+    }
+    GH_CHECK(hipSetDevice(P->device));
+    GH_CHECK(hipDeviceSynchronize());
+    int rc;
+    if ((rc = check_err(P, nullptr))) return rc;
+    GH_CHECK(hipMemcpy(logL, P->B.logl, sizeof(double) * P->C, hipMemcpyDeviceToHost));
+    if (src_params)
+        GH_CHECK(hipMemcpy(src_params, P->B.sparams, sizeof(double) * P->C * kMaxSrc * 8, hipMemcpyDeviceToHost));
+    return GLASS_HIP_OK;
+}

@@ -16,15 +16,20 @@ __global__ __launch_bounds__(NTHR) void fd_waveform_kernel(const double* __restr
                                                            const double2* __restrict__ tw, int twstride,
                                                            int fmt, double2* __restrict__ out)
 {
-    __shared__ double2 lds[6 * N];
+    using S = FdShape<N, NTHR>;
+    __shared__ double2 lds[S::LDS];
     UcbConst c;
     ucb_setup(params + (size_t)blockIdx.x * np, np, T, t0, L, fstar, c);
-    fd_slow_fft<N, NTHR>(c, lds, pos, posstride, tw, twstride);
+    double2 d[S::PER][6];
+    fd_link_spectra<N, NTHR>(c, lds, pos, posstride, tw, twstride, d);
     double2* o = out + (size_t)blockIdx.x * 5 * N;
-    for (int m = threadIdx.x; m < N; m += NTHR)
+#pragma unroll
+    for (int k = 0; k < S::PER; k++)
     {
+        const int m = threadIdx.x + k * NTHR;
+        if (m >= N) continue;
         double2 v[5];
-        fd_tdi_bin<N>(c, lds, m, fmt, v);
+        fd_tdi<N>(c, m, fmt, d[k], v);
 #pragma unroll
         for (int ch = 0; ch < 5; ch++) o[ch * N + m] = v[ch];
     }
@@ -52,9 +57,9 @@ int fd_waveform_launch(int ntmpl, const double* d_params, int np, double T, doub
     {
 #define GH_CASE(n) \
     case n: launch_fd<n>(ntmpl, d_params, np, T, t0, o.L, o.fstar, pos, 1, tw, fmt, d_out, s); break;
-        GH_CASE(16) GH_CASE(32) GH_CASE(64) GH_CASE(128) GH_CASE(256) GH_CASE(512) GH_CASE(1024)
+        GH_CASE(16) GH_CASE(32) GH_CASE(64) GH_CASE(128) GH_CASE(256) GH_CASE(512) GH_CASE(1024) GH_CASE(2048) GH_CASE(4096)
 #undef GH_CASE
-        default: set_error("BW=%d unsupported (power of two 16..1024)", BW); return GLASS_HIP_EINVAL;
+        default: set_error("BW=%d unsupported (power of two 16..4096)", BW); return GLASS_HIP_EINVAL;
     }
     GH_CHECK(hipGetLastError());
     return GLASS_HIP_OK;
@@ -109,5 +114,30 @@ extern "C" int glass_hip_ucb_waveform(const glass_hip_orbit* orbit, const char* 
     GH_CHECK(hipStreamSynchronize(cx->stream));
     double* outs[5] = {X, Y, Z, A, E};
     for (int ch = 0; ch < (all ? 5 : 1); ch++) memcpy(outs[ch], hp + (size_t)ch * 2 * BW, sizeof(double) * 2 * BW);
+    return GLASS_HIP_OK;
+}
+
+extern "C" int glass_hip_ucb_bandwidth(double L, double fstar, double f, double fdot, double costheta, double A,
+                                       double T, int N)
+{
+    return glass::ucb_bandwidth(L, fstar, f, fdot, costheta, A, T, N);
+}
+
+extern "C" int glass_hip_ucb_alignment(const glass_hip_orbit* orbit, double T, int NFFT, int data_qmin,
+                                       const double* params, int* BW, int* qmin, int* imin)
+{
+    if (!orbit || !params || !BW || !qmin || !imin || NFFT < 2)
+    {
+        glass::set_error("ucb_alignment: bad arguments");
+        return GLASS_HIP_EINVAL;
+    }
+    // map_array_to_ucb_params (ucb/src/glass_ucb_model.c:25-38)
+    double f0 = params[0] / T;
+    double costheta = params[1];
+    double amp = exp(params[3]);
+    double dfdt = params[7] / (T * T);
+    *BW = 2 * glass::ucb_bandwidth(orbit->L, orbit->fstar, f0, dfdt, costheta, amp, T, NFFT);
+    *qmin = (int)(f0 * T) - *BW / 2;
+    *imin = *qmin - data_qmin;
     return GLASS_HIP_OK;
 }

@@ -180,4 +180,61 @@ __host__ __device__ inline double ucb_sample_time(double T, double t0, int n, in
 // TDI format codes (glass_ucb_waveform.c:965-975)
 enum TdiFormat : int { TDI_PHASE = 0, TDI_FREQUENCY = 1, TDI_SANGRIA = 2 };
 
+// utils/src/glass_lisa.h:43-49
+constexpr double kSPS = 8.321000e-23;
+constexpr double kSACC = 9.000000e-30;
+constexpr double kSLOC = 2.89e-24;
+
+__host__ __device__ inline double ipow_(double x, int n)
+{
+    double xn = 1.0;
+    for (int i = 0; i < n; i++) xn *= x;
+    return xn;
+}
+
+// AEnoise with rednoise (utils/src/glass_lisa.c:1183-1198)
+__host__ __device__ inline double ae_noise(double L, double fstar, double f)
+{
+    double fon = f / fstar;
+    double trans = ipow_(sin(fon), 2);
+    double red = 1. + 16.0 * (pow((2.0e-5 / f), 10.0) + ipow_(1.0e-4 / f, 2));
+    return (16.0 / 3.0) * trans *
+           ((2.0 + cos(fon)) * (kSPS + kSLOC) +
+            2.0 * (3.0 + 2.0 * cos(fon) + cos(2.0 * fon)) * (kSLOC / 2.0 + kSACC / ipow_(kPI2 * f, 4) * red)) /
+           ipow_(2.0 * L, 2);
+}
+
+// 2^((int)log2(x) + 1) clamped to [nmin, nmax], as ucb_bandwidth forms its
+// Doppler and SNR spreads (ucb/src/glass_ucb_waveform.c:709-722).  Values
+// below 8 (including negative or NaN x) land on the floor nmin = 16.
+__host__ __device__ inline int pow2_spread(double x, int nmin, int nmax)
+{
+    if (!(x >= 8.0)) return nmin;
+    int e = (int)log2(x) + 1;
+    int v = e >= 30 ? (1 << 30) : (1 << e);
+    if (v > nmax) v = nmax;
+    if (v < nmin) v = nmin;
+    return v;
+}
+
+// ucb_bandwidth (ucb/src/glass_ucb_waveform.c:698-725)
+__host__ __device__ inline int ucb_bandwidth(double L, double fstar, double f, double fdot, double costheta,
+                                             double A, double T, int N)
+{
+#pragma clang fp contract(off)
+    const int Nmin = 16;
+    int half = N / 2, lg = 0;
+    while ((2 << lg) <= half) lg++;  // (int)log2(N/2)
+    const int Nmax = 1 << lg;
+    double sqT = sqrt(T);
+    double sf = sin(f / fstar);
+    double sn = ae_noise(L, fstar, f);
+    double sintheta = sin(acos(costheta));
+    double bw = 2 * T * ((4. + kPI2 * f * (kAU / kCLIGHT) * sintheta) / kYEAR + fabs(fdot) * T);
+    int DS = pow2_spread(bw - 1, Nmin, Nmax);
+    double SNRm = A * sqT * sf / sqrt(sn);  // analytic_snr, glass_ucb_waveform.c:22-25
+    int SS = pow2_spread(SNRm - 1, Nmin, Nmax);
+    return (DS > SS) ? DS : SS;
+}
+
 }  // namespace glass

@@ -109,3 +109,62 @@ def config0(seed=20240, n=64, T=YEAR_T, BW=128, W=256, waveform=None):
             scatter(data[i], h, imin[i])
         data[i] += noise_draw(rng, C[i])
     return dict(params=P, T=T, BW=BW, W=W, qmin=qmin, imin=imin, C=C, data=data, seed=seed)
+
+
+# proposal jump scales per parameter (f0*T, cos th, phi, ln A, cos i, psi,
+# phi0, fdot*T^2): small moves of the kind a Fisher proposal makes
+JUMP = np.array([0.05, 0.01, 0.01, 0.02, 0.01, 0.01, 0.02, 0.01])
+
+
+def _clip_cos(p):
+    p[..., 1] = np.clip(p[..., 1], -1.0, 1.0)
+    p[..., 4] = np.clip(p[..., 4], -1.0, 1.0)
+    return p
+
+
+def config1(seed=1, ngroup=256, ntemp=16, nprop_per_chain=80, T=T_2YR, f_lo=0.5e-3, f_hi=20e-3,
+            windows=(512, 1024, 2048, 4096), max_src=8, waveform=None, alignment=None):
+    """Batched PT-MCMC case: ngroup data windows x ntemp chains, 1..max_src
+    sources per chain, nprop_per_chain proposals per chain.
+
+    waveform(params, T, BW) -> complex [5, BW] and
+    alignment(params, T, NFFT, qmin) -> (BW, qmin, imin) build the injected
+    data (any FP64 implementation; the data are only an input)."""
+    rng = np.random.default_rng(seed)
+    G = ngroup
+    Wg = rng.choice(np.asarray(windows), G).astype(np.int32)
+    fc = rng.uniform(f_lo, f_hi, G)
+    gqmin = (fc * T).astype(np.int64) - Wg // 2
+    nsrc = rng.integers(1, max_src + 1, G)
+    data, invC33, inj = [], [], []
+    for g in range(G):
+        W = int(Wg[g])
+        P = draw_params(rng, int(nsrc[g]), T)
+        bins = rng.uniform(gqmin[g] + W / 8, gqmin[g] + 7 * W / 8, int(nsrc[g]))
+        P[:, 0] = bins
+        inj.append(P)
+        C = xyz_noise_cov(T, gqmin[g] / T, W)
+        d = noise_draw(rng, C)
+        if waveform is not None:
+            for p in P:
+                BW, _, imin = alignment(p, T, W, int(gqmin[g]))
+                scatter(d, waveform(p, T, BW)[:3], imin)
+        data.append(d)
+        invC33.append(C)  # inverted by the caller
+    C = G * ntemp
+    chain_group = np.repeat(np.arange(G), ntemp).astype(np.int32)
+    chain_nsrc = nsrc[chain_group].astype(np.int32)
+    src_params = np.zeros((C, 8, 8))
+    for c in range(C):
+        g = chain_group[c]
+        n = chain_nsrc[c]
+        src_params[c, :n] = _clip_cos(inj[g] + JUMP * rng.standard_normal((n, 8)))
+    # proposals: nprop_per_chain per chain, cycling over its sources
+    E = C * nprop_per_chain
+    p_chain = np.repeat(np.arange(C), nprop_per_chain).astype(np.int32)
+    k = np.tile(np.arange(nprop_per_chain), C)
+    p_src = (k % chain_nsrc[p_chain]).astype(np.int32)
+    p_params = _clip_cos(src_params[p_chain, p_src] + 0.5 * JUMP * rng.standard_normal((E, 8)))
+    return dict(T=T, group_qmin=gqmin.astype(np.int32), group_W=Wg, data=data, C=invC33,
+                chain_group=chain_group, chain_nsrc=chain_nsrc, src_params=src_params,
+                p_chain=p_chain, p_src=p_src, p_params=p_params, injected=inj, seed=seed)

@@ -64,6 +64,20 @@ int glass_hip_ucb_waveform(const glass_hip_orbit *orbit, const char *format, dou
                            const double *params, int NParams, double *X, double *Y, double *Z,
                            double *A, double *E, int BW, int NI);
 
+/* Replaces int ucb_bandwidth(double L, double fstar, double f, double fdot,
+ *   double costheta, double A, double T, int N)
+ * (ucb/src/glass_ucb_waveform.h, ucb/src/glass_ucb_waveform.c:698-725).
+ * Host computation (integer result that sizes the template). */
+int glass_hip_ucb_bandwidth(double L, double fstar, double f, double fdot, double costheta,
+                            double A, double T, int N);
+
+/* Replaces void ucb_alignment(struct Orbit *orbit, struct Data *data,
+ *   struct Source *source)  (ucb/src/glass_ucb_waveform.c:727-736):
+ * from params[8] (the reference's parameter array, NFFT = data->NFFT,
+ * data_qmin = data->qmin) returns BW, qmin and imin (source fields). */
+int glass_hip_ucb_alignment(const glass_hip_orbit *orbit, double T, int NFFT, int data_qmin,
+                            const double *params, int *BW, int *qmin, int *imin);
+
 /* Replaces double fourier_nwip(const double *a, const double *b,
  *   const double *invC, int N)  (utils/src/glass_math.h:243,
  *   utils/src/glass_math.c:362-380): 2 * sum_n Re(a_n conj b_n) invC_n. */

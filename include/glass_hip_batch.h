@@ -1,0 +1,97 @@
+/*
+ * glass_hip_batch.h -- batched, device-resident UCB waveform + likelihood
+ * evaluation on gfx950.  No counterpart in the reference: the reference
+ * evaluates one proposal per OpenMP thread (apps/src/ucb_mcmc.c:230,
+ * ucb/src/glass_ucb_sampler.c:139-155); this API evaluates a whole sweep of
+ * proposals (chains x temperatures x sources) in one stream-ordered,
+ * graph-capturable launch sequence.
+ *
+ * Model.  A plan holds G data windows ("groups": the frequency segment a
+ * set of parallel-tempered chains analyses) and C chains.  Chain c belongs
+ * to group g(c) and carries nsrc(c) <= 8 galactic binaries; the plan keeps,
+ * on the device, every source's current template (X,Y,Z, "sangria" unless
+ * another format is given), every chain's residual r_c = d_g - sum_s h_cs
+ * and its logL_c = -1/2 (r_c|r_c).
+ *
+ * A proposal is (chain c, source s, params[8]).  Its evaluation returns
+ *   logL  = log-likelihood of chain c with source s moved to params:
+ *           logL_c + (r_c|dh) - 1/2 (dh|dh),  dh = h(params) - h_cs,
+ *           equal to generate_ucb_model(source_id=s) followed by
+ *           gaussian_log_likelihood (ucb/src/glass_ucb_model.c:55-103,
+ *           utils/src/glass_model.c:726-774) over the whole window;
+ *   snr   = snr() of the proposed template as the sampler's rejection test
+ *           computes it (utils/src/glass_math.c:403-426: template-local bins
+ *           against window-indexed invC).
+ * ucb_alignment (BW, qmin; ucb/src/glass_ucb_waveform.c:698-736) is
+ * evaluated on the device for every proposal.
+ *
+ * Layouts (all FP64, complex = interleaved re,im):
+ *   data:  per group, 3 channels x W_g complex, groups concatenated
+ *   invC:  per group, 6 planes x W_g: XX, YY, ZZ, XY, XZ, YZ
+ *   src_params: C x 8 x 8 (unused source slots ignored)
+ * Entry points taking `stream` accept a hipStream_t (NULL = default
+ * stream) and only enqueue work: they neither allocate nor synchronise, so
+ * a caller may capture them in a hipGraph.  The plan's own graph path is
+ * glass_hip_fd_batch_eval_graph.
+ */
+#ifndef GLASS_HIP_BATCH_H
+#define GLASS_HIP_BATCH_H
+
+#include "glass_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct glass_hip_fd_batch glass_hip_fd_batch;
+
+/* Creates a plan and uploads the data (host arrays, copied).  format:
+ * "sangria" | "phase" | "frequency".  max_prop: largest proposal count a
+ * single eval will be given.  Computes every source's template, the
+ * residuals and logL_c on the device before returning. */
+int glass_hip_fd_batch_create(glass_hip_fd_batch **plan, const glass_hip_orbit *orbit,
+                              const char *format, double T, double t0, int ngroup,
+                              const int *group_qmin, const int *group_W, const double *data,
+                              const double *invC, int nchain, const int *chain_group,
+                              const int *chain_nsrc, const double *src_params, int max_prop);
+
+int glass_hip_fd_batch_destroy(glass_hip_fd_batch *plan);
+
+/* Device buffers owned by the plan, for callers that fill proposals on the
+ * device: chain [max_prop] int, src [max_prop] int, params [max_prop*8],
+ * logL [max_prop], snr [max_prop]. */
+int glass_hip_fd_batch_buffers(glass_hip_fd_batch *plan, int **d_chain, int **d_src,
+                               double **d_params, double **d_logL, double **d_snr);
+
+/* Evaluates nprop proposals held in device memory (any device pointers;
+ * the plan's own buffers work).  Results to d_logL, d_snr (device). */
+int glass_hip_fd_batch_eval(glass_hip_fd_batch *plan, int nprop, const int *d_chain,
+                            const int *d_src, const double *d_params, double *d_logL,
+                            double *d_snr, void *stream);
+
+/* Same, replaying a hipGraph of the launch sequence built on first use for
+ * this (nprop, pointer) combination. */
+int glass_hip_fd_batch_eval_graph(glass_hip_fd_batch *plan, int nprop, const int *d_chain,
+                                  const int *d_src, const double *d_params, double *d_logL,
+                                  double *d_snr, void *stream);
+
+/* Accepts proposals: for each index i in d_idx[0..nacc) (indices into the
+ * last evaluated proposal arrays, at most one per chain), moves the source
+ * to its proposed parameters, updates the chain residual and sets logL_c to
+ * the proposal's logL. */
+int glass_hip_fd_batch_accept(glass_hip_fd_batch *plan, int nacc, const int *d_idx,
+                              const int *d_chain, const int *d_src, const double *d_params,
+                              const double *d_logL, void *stream);
+
+/* Recomputes every template, residual and logL_c from the current
+ * parameters (removes drift accumulated by accepts). */
+int glass_hip_fd_batch_refresh(glass_hip_fd_batch *plan, void *stream);
+
+/* Copies the chains' current logL (host array of nchain) and, if not
+ * NULL, their current source parameters (nchain*8*8). Synchronous. */
+int glass_hip_fd_batch_state(glass_hip_fd_batch *plan, double *logL, double *src_params);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,149 @@
+"""GPU parity of the batched waveform + delta-logL evaluation.
+
+Every proposal's logL is compared with the oracle's restatement of the
+reference's per-step work (generate_ucb_model for the moved source, all
+other sources' stored templates, full-window gaussian_log_likelihood),
+and its snr with the reference's snr() (template-local bins against
+window-indexed invC).  Gate: relative error <= 1e-10 (logL) and 1e-10
+(snr)."""
+import numpy as np
+import pytest
+
+from glass_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def make_cfg(orc, seed=3, ngroup=6, ntemp=3, nprop=6, **kw):
+    o = orc.analytic_orbit()
+    cfg = synth.config1(seed=seed, ngroup=ngroup, ntemp=ntemp, nprop_per_chain=nprop,
+                        waveform=lambda p, T, BW: orc.ucb_waveform(p, T, BW),
+                        alignment=lambda p, T, W, q: orc.ucb_alignment(p, T, W, q, orbit=o), **kw)
+    cfg["invC"] = [orc.invert_noise(C)[0] for C in cfg["C"]]
+    return cfg
+
+
+def oracle_chains(orc, cfg, src_params):
+    """orc_chain structs for the chains' current state."""
+    T = cfg["T"]
+    chains = []
+    bwmax = 4096
+    for c in range(len(cfg["chain_group"])):
+        g = cfg["chain_group"][c]
+        W = int(cfg["group_W"][g])
+        qmin = int(cfg["group_qmin"][g])
+        n = int(cfg["chain_nsrc"][c])
+        tm = np.zeros((n, 3, 2 * bwmax))
+        bw = np.zeros(n, np.int32)
+        im = np.zeros(n, np.int32)
+        for s in range(n):
+            b, _, i = orc.ucb_alignment(src_params[c, s], T, W, qmin)
+            h = orc.ucb_waveform(src_params[c, s], T, b)
+            tm[s, :, :2 * b] = h[:3].view(np.float64).reshape(3, -1)
+            bw[s], im[s] = b, i
+        chains.append(dict(nsrc=n, W=W, qmin=qmin, bwmax=bwmax,
+                           data=np.ascontiguousarray(cfg["data"][g]).view(np.float64).reshape(-1).copy(),
+                           invC=np.ascontiguousarray(cfg["invC"][g]).reshape(-1).copy(),
+                           src_params=np.ascontiguousarray(src_params[c, :max(n, 1)]).copy(),
+                           tmpl=tm.reshape(-1).copy(), src_bw=bw, src_imin=im))
+    return orc.ChainSet(chains), chains
+
+
+def oracle_snr(orc, cfg, c, params):
+    T = cfg["T"]
+    g = cfg["chain_group"][c]
+    W, qmin = int(cfg["group_W"][g]), int(cfg["group_qmin"][g])
+    b, _, _ = orc.ucb_alignment(params, T, W, qmin)
+    h = orc.ucb_waveform(params, T, b)
+    return orc.snr(h[0], h[1], h[2], cfg["invC"][g], b)
+
+
+def new_plan(gl, cfg, max_prop):
+    from glass_amd.batch import FdBatch
+    return FdBatch(gl.analytic_orbit(), "sangria", cfg["T"], 0.0, cfg["group_qmin"], cfg["data"], cfg["invC"],
+                   cfg["chain_group"], cfg["chain_nsrc"], cfg["src_params"], max_prop)
+
+
+def check_props(orc, cfg, cs, logL, snr, idx):
+    worst = 0.0
+    for e in idx:
+        c, s, p = int(cfg["p_chain"][e]), int(cfg["p_src"][e]), cfg["p_params"][e]
+        import ctypes
+        ref = orc.lib().orc_eval_logl(ctypes.byref(orc.analytic_orbit()), cfg["T"], 0.0,
+                                      ctypes.byref(cs.arr[c]), s, p.ctypes.data_as(orc._dp))
+        worst = max(worst, abs(logL[e] - ref) / abs(ref))
+        sref = oracle_snr(orc, cfg, c, p)
+        assert abs(snr[e] - sref) <= TOL * sref, (e, snr[e], sref)
+    return worst
+
+
+def test_batch_eval_matches_oracle(gl, orc):
+    import torch
+    cfg = make_cfg(orc)
+    E = len(cfg["p_chain"])
+    plan = new_plan(gl, cfg, E)
+    # initial chain logL == full-window logL of the current state
+    lg, sp = plan.state()
+    assert np.array_equal(sp, cfg["src_params"])
+    cs, _ = oracle_chains(orc, cfg, cfg["src_params"])
+    for c in range(len(lg)):
+        n = int(cfg["chain_nsrc"][c])
+        import ctypes
+        ref = orc.lib().orc_eval_logl(ctypes.byref(orc.analytic_orbit()), cfg["T"], 0.0, ctypes.byref(cs.arr[c]),
+                                      -1, cfg["src_params"][c, 0].ctypes.data_as(orc._dp)) if n else None
+        if n:
+            assert abs(lg[c] - ref) <= TOL * abs(ref)
+    plan.set_proposals(cfg["p_chain"], cfg["p_src"], cfg["p_params"])
+    logL, snr = plan.eval(E)
+    logL, snr = logL.cpu().numpy(), snr.cpu().numpy()
+    worst = check_props(orc, cfg, cs, logL, snr, range(E))
+    print("worst logL rel err", worst)
+    assert worst <= TOL
+    # graph replay: same numbers, every replay, with eager work in between
+    for r in range(3):
+        plan.logL.zero_()
+        l2, s2 = plan.eval(E, graph=True)
+        torch.cuda.synchronize()
+        x = torch.randn(1000, device="cuda").sum().item()  # eager work between replays
+        assert np.array_equal(l2.cpu().numpy(), logL), r
+        assert np.array_equal(s2.cpu().numpy(), snr), r
+    plan.close()
+
+
+def test_batch_accept_then_eval(gl, orc):
+    cfg = make_cfg(orc, seed=5, ngroup=4, ntemp=2, nprop=4)
+    E = len(cfg["p_chain"])
+    plan = new_plan(gl, cfg, E)
+    plan.set_proposals(cfg["p_chain"], cfg["p_src"], cfg["p_params"])
+    logL, _ = plan.eval(E)
+    logL = logL.cpu().numpy()
+    # accept the first proposal of every chain
+    nchain = len(cfg["chain_group"])
+    acc = [int(np.nonzero(cfg["p_chain"] == c)[0][0]) for c in range(nchain)]
+    plan.accept(acc)
+    lg, sp = plan.state()
+    cur = cfg["src_params"].copy()
+    for e in acc:
+        cur[cfg["p_chain"][e], cfg["p_src"][e]] = cfg["p_params"][e]
+        assert lg[cfg["p_chain"][e]] == logL[e]
+    assert np.array_equal(sp, cur)
+    # evaluate again from the updated state
+    cs, _ = oracle_chains(orc, cfg, cur)
+    l2, s2 = plan.eval(E)
+    worst = check_props(orc, cfg, cs, l2.cpu().numpy(), s2.cpu().numpy(), range(E))
+    assert worst <= TOL
+    # refresh recomputes from parameters: same state to rounding
+    plan.refresh()
+    lg2, _ = plan.state()
+    assert np.max(np.abs(lg2 - lg) / np.abs(lg)) <= TOL
+    plan.close()
+
+
+def test_batch_bad_sizes(gl, orc):
+    cfg = make_cfg(orc, seed=7, ngroup=2, ntemp=1, nprop=2)
+    plan = new_plan(gl, cfg, 4)
+    with pytest.raises(gl.GlassHipError):
+        plan.eval(5)
+    plan.close()

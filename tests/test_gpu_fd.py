@@ -35,7 +35,7 @@ def test_waveform_config0(gl, orc, cfg0):
 
 
 @pytest.mark.parametrize("fmt", ["sangria", "phase", "frequency"])
-@pytest.mark.parametrize("BW", [16, 32, 64, 256, 512, 1024])
+@pytest.mark.parametrize("BW", [16, 32, 64, 256, 512, 1024, 2048])
 def test_waveform_formats_sizes(gl, orc, fmt, BW):
     rng = np.random.default_rng(BW)
     T = synth.T_2YR
