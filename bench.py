@@ -1,0 +1,258 @@
+#!/usr/bin/env python3
+"""bench.py -- batched UCB waveform + log-likelihood throughput on one MI355X.
+
+One step = one parallel-tempering sweep of likelihood evaluations of the
+batched PTMCMC case: 256 data windows (512-4096 complex128 bins, Tobs = 2 yr)
+x 16 temperatures = 4096 chains, each with 1-8 galactic binaries
+(f0 ~ U[0.5, 20] mHz), 80 proposals per chain = 327,680 evaluations.  One
+evaluation = the reference's per-proposal likelihood work: regenerate one
+source's template (ucb_alignment + ucb_waveform) and return the chain's
+logL and the template's snr (ucb/src/glass_ucb_sampler.c:139-155).  All
+inputs are resident in HBM before the timed region; the proposals are
+evaluated by the library's per-BW-class kernels.
+
+Printed: one JSON line with the throughput, the plain per-call drop-in rate
+(the "plain" method) for comparison, a parity check of a sample of the
+timed evaluations against the FP64 oracle, the roofline of the dominant
+kernel and the oracle's CPU throughput on the host cores.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "UCB waveform+logL evals/s, batched vs plain, at logL rel err ≤1e-10 vs FP64"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def wave_flops(N):
+    """Algorithmic FP64 work of one template of N bins plus its delta-logL:
+    940 N + 30 N log2 N for the slow series, six FFTs and TDI (each sin/cos
+    counted as 20 operations), + 36 N for the 3-channel weighted inner
+    products."""
+    return 940.0 * N + 30.0 * N * math.log2(N) + 36.0 * N
+
+
+def eval_bytes(N):
+    """Algorithmic HBM bytes of one fused evaluation over N bins: chain
+    residual 48 N + invC planes 48 N + the source's stored template 48 N +
+    parameters 64 + logL/snr out 16."""
+    return 144.0 * N + 80.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def build_inputs(gl, synth, ngroup, ntemp, nprop, seed):
+    from glass_amd.batch import ucb_alignment
+    o = gl.analytic_orbit()
+    cfg = synth.config1(seed=seed, ngroup=ngroup, ntemp=ntemp, nprop_per_chain=nprop,
+                        waveform=lambda p, T, BW: gl.ucb_waveform(o, "sangria", T, 0.0, p, 8, BW),
+                        alignment=lambda p, T, W, q: ucb_alignment(o, T, W, q, p))
+    cfg["invC"] = [gl.invert_noise_covariance_matrix(C)[0] for C in cfg["C"]]
+    return o, cfg
+
+
+def plain_rate(gl, synth, seconds=2.0):
+    """The plain method: per-call drop-ins ucb_waveform + gaussian_log_likelihood
+    on host memory (64 vectors, BW 128, W 256, Tobs 1 yr)."""
+    cfg = synth.config0(seed=7, waveform=None)
+    o = gl.analytic_orbit()
+    T, W, BW = cfg["T"], cfg["W"], cfg["BW"]
+    invCs = [gl.invert_noise_covariance_matrix(C)[0] for C in cfg["C"]]
+    datas = []
+    for i in range(len(cfg["params"])):
+        d = np.zeros((5, W), complex)
+        d[:3] = cfg["data"][i]
+        datas.append(d)
+    n = 0
+    t0 = time.perf_counter()
+    while True:
+        i = n % len(cfg["params"])
+        h = gl.ucb_waveform(o, "sangria", T, 0.0, cfg["params"][i], 8, BW)
+        m = np.zeros((5, W), complex)
+        synth.scatter(m[:3], h[:3], int(cfg["imin"][i]))
+        gl.gaussian_log_likelihood(datas[i], m, invCs[i])
+        n += 1
+        if n % 64 == 0 and time.perf_counter() - t0 > seconds:
+            break
+    return n / (time.perf_counter() - t0)
+
+
+def oracle_leg(cfg, logL_gpu, seconds, nthreads, seed):
+    """CPU baseline (the oracle restatement of the reference, OpenMP over
+    proposals) on a bounded sample of the same workload, and the parity of
+    the GPU results for those proposals."""
+    import ctypes
+    from oracle import orc
+    orc.lib()
+    rng = np.random.default_rng(seed)
+    C = len(cfg["chain_group"])
+    chains = rng.choice(C, size=min(48, C), replace=False)
+    T = cfg["T"]
+    bwmax = 4096
+    structs, local = [], {}
+    for j, c in enumerate(chains):
+        g = cfg["chain_group"][c]
+        W, qmin = int(cfg["group_W"][g]), int(cfg["group_qmin"][g])
+        n = int(cfg["chain_nsrc"][c])
+        tm = np.zeros((n, 3, 2 * bwmax))
+        bw = np.zeros(n, np.int32)
+        im = np.zeros(n, np.int32)
+        for s in range(n):
+            b, _, i = orc.ucb_alignment(cfg["src_params"][c, s], T, W, qmin)
+            tm[s, :, :2 * b] = orc.ucb_waveform(cfg["src_params"][c, s], T, b)[:3].view(np.float64).reshape(3, -1)
+            bw[s], im[s] = b, i
+        structs.append(dict(nsrc=n, W=W, qmin=qmin, bwmax=bwmax,
+                            data=np.ascontiguousarray(cfg["data"][g]).view(np.float64).reshape(-1).copy(),
+                            invC=np.ascontiguousarray(cfg["invC"][g]).reshape(-1).copy(),
+                            src_params=cfg["src_params"][c, :n].copy(), tmpl=tm.reshape(-1).copy(),
+                            src_bw=bw, src_imin=im))
+        local[int(c)] = j
+    cs = orc.ChainSet(structs)
+    sel = np.nonzero(np.isin(cfg["p_chain"], chains))[0]
+    rng.shuffle(sel)
+    ch = np.array([local[int(c)] for c in cfg["p_chain"][sel]], np.int32)
+    # one pass over the sample (also the parity check), then repeat it to
+    # about `seconds` of CPU work in total
+    n = len(sel)
+    out, secs = orc.eval_batch(cs, T, ch, cfg["p_src"][sel], cfg["p_params"][sel], nthreads=nthreads)
+    rel = np.abs(logL_gpu[sel] - out) / np.abs(out)
+    reps = max(0, int(round(seconds / max(secs, 1e-3))) - 1)
+    for _ in range(reps):
+        _, s = orc.eval_batch(cs, T, ch, cfg["p_src"][sel], cfg["p_params"][sel], nthreads=nthreads)
+        secs += s
+    return dict(rate=n * (reps + 1) / secs, n=n, reps=reps + 1, secs=secs, nchains=len(chains),
+                max_rel=float(rel.max()))
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--groups", type=int, default=256, help="data windows (256 = the full case)")
+    ap.add_argument("--temps", type=int, default=16)
+    ap.add_argument("--props", type=int, default=80, help="proposals per chain per sweep")
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs on a single MI355X and has no multi-GPU "
+              "path; run it with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import glass_amd as gl
+    from glass_amd import synth
+    from glass_amd.batch import FdBatch
+
+    gl.lib()
+    t0 = time.perf_counter()
+    o, cfg = build_inputs(gl, synth, args.groups, args.temps, args.props, args.seed)
+    E = len(cfg["p_chain"])
+    plan = FdBatch(o, "sangria", cfg["T"], 0.0, cfg["group_qmin"], cfg["data"], cfg["invC"],
+                   cfg["chain_group"], cfg["chain_nsrc"], cfg["src_params"], E)
+    plan.set_proposals(cfg["p_chain"], cfg["p_src"], cfg["p_params"])
+    torch.cuda.synchronize()
+    log(f"inputs + plan: {time.perf_counter() - t0:.1f} s, {E} evaluations per step")
+
+    for _ in range(args.warmup):
+        plan.eval(E, graph=True)
+    torch.cuda.synchronize()
+    # graph replay (the production path), for reference
+    g0, g1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    g0.record()
+    for _ in range(max(1, min(args.steps, 5))):
+        plan.eval(E, graph=True)
+    g1.record()
+    torch.cuda.synchronize()
+    graph_ms = g0.elapsed_time(g1) / max(1, min(args.steps, 5))
+
+    # timed region: K sweeps with per-kernel HIP events on the launch stream
+    plan.profile(True)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        plan.eval(E)
+    e1.record()
+    torch.cuda.synchronize()
+    total_ms = e0.elapsed_time(e1)
+    ms_step = total_ms / args.steps
+    value = E * args.steps / (total_ms * 1e-3)
+    kms, klaunch, kcount = plan.profile_read()
+    plan.profile(False)
+    logL = plan.logL[:E].cpu().numpy()
+    log(f"sweep {ms_step:.3f} ms ({value:.4e} evals/s), graph replay {graph_ms:.3f} ms")
+    for k in range(8):
+        if klaunch[k]:
+            log(f"  BW {32 << k:5d}: {kcount[k]:7d} evals, {kms[k] / klaunch[k]:8.3f} ms/launch")
+
+    kd = int(np.argmax(kms))
+    Nd = 32 << kd
+    avg_ms = kms[kd] / klaunch[kd]
+    flops = kcount[kd] * wave_flops(Nd)
+    byts = kcount[kd] * eval_bytes(Nd)
+    ach_tf = flops / (avg_ms * 1e-3) / 1e12
+    ach_gbs = byts / (avg_ms * 1e-3) / 1e9
+    roofline = {"bound": "mfma", "pipe": "fp64 VALU (no MFMA: nothing on the path is a dense contraction)",
+                "kernel": f"eval_kernel<{Nd},{min(max(Nd, 64), 1024)}>", "achieved": round(ach_tf, 4),
+                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(ach_tf / FP64_PEAK_TFLOPS, 5),
+                "traffic": None, "avg_launch_ms": round(avg_ms, 4), "evals_per_launch": int(kcount[kd]),
+                "hbm_algorithmic_GBs": round(ach_gbs, 2), "hbm_frac": round(ach_gbs / HBM_PEAK_GBS, 5)}
+
+    plain = None
+    if not args.no_plain:
+        plain = plain_rate(gl, synth)
+        log(f"plain per-call drop-ins: {plain:.1f} evals/s")
+
+    cpu = None
+    parity = None
+    if not args.no_cpu_baseline:
+        nthr = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+        leg = oracle_leg(cfg, logL, args.cpu_seconds, nthr, args.seed + 17)
+        cpu = {"value": round(leg["rate"], 2), "unit": "evals/s", "cores": nthr, "kind": "port",
+               "sample": f"{leg['n']} proposals (all of {leg['nchains']} randomly chosen chains) of the "
+                         f"same sweep, evaluated {leg['reps']}x by the oracle restatement of "
+                         f"generate_ucb_model + gaussian_log_likelihood, OpenMP over proposals, "
+                         f"{leg['secs']:.1f} s"}
+        parity = {"checked": leg["n"], "max_logL_rel_err": leg["max_rel"], "gate": 1e-10,
+                  "pass": bool(leg["max_rel"] <= 1e-10)}
+        log(f"cpu baseline {leg['rate']:.1f} evals/s on {nthr} threads; parity max rel {leg['max_rel']:.3e}")
+
+    line = {
+        "metric": METRIC, "value": round(value, 1), "unit": "evals/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "batched PTMCMC likelihood sweep: %d windows x %d temperatures, 1-8 sources "
+                               "per chain, %d proposals per chain, Tobs 2 yr, windows 512-4096 bins" %
+                               (args.groups, args.temps, args.props),
+                   "evals_per_step": E, "chains": len(cfg["chain_group"]), "format": "sangria",
+                   "graph_replay_ms_per_step": round(graph_ms, 4)},
+        "plain": None if plain is None else {"value": round(plain, 1), "unit": "evals/s",
+                                             "method": "per-call ucb_waveform + gaussian_log_likelihood "
+                                                       "drop-ins, host memory, BW 128, W 256"},
+        "batched_vs_plain": None if plain is None else round(value / plain, 1),
+        "parity": parity,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    plan.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

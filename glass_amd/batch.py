@@ -25,6 +25,8 @@ _EXTRA += [
     ("glass_hip_fd_batch_accept", [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp], _i),
     ("glass_hip_fd_batch_refresh", [_vp, _vp], _i),
     ("glass_hip_fd_batch_state", [_vp, _dp, _dp], _i),
+    ("glass_hip_fd_batch_profile", [_vp, _i], _i),
+    ("glass_hip_fd_batch_profile_read", [_vp, _dp, _ip, _ip], _i),
     ("glass_hip_ucb_bandwidth", [_d] * 7 + [_i], _i),
     ("glass_hip_ucb_alignment", [_vp, _d, _i, _i, _dp, _ip, _ip, _ip], _i),
 ]
@@ -136,6 +138,19 @@ class FdBatch:
     def refresh(self):
         s = ctypes.c_void_p(self.torch.cuda.current_stream().cuda_stream)
         check(_L().glass_hip_fd_batch_refresh(self.h, s), "fd_batch_refresh")
+
+    def profile(self, on=True):
+        """Per-BW-class kernel timing of eager evals (makes them synchronous)."""
+        check(_L().glass_hip_fd_batch_profile(self.h, int(bool(on))), "fd_batch_profile")
+
+    def profile_read(self):
+        """-> (ms[8], launches[8], count[8]) per BW class (BW = 32 << k)."""
+        ms = np.zeros(8)
+        la = np.zeros(8, dtype=np.int32)
+        cn = np.zeros(8, dtype=np.int32)
+        check(_L().glass_hip_fd_batch_profile_read(self.h, ms.ctypes.data_as(_dp), la.ctypes.data_as(_ip),
+                                                   cn.ctypes.data_as(_ip)), "fd_batch_profile_read")
+        return ms, la, cn
 
     def state(self):
         self.torch.cuda.current_stream().synchronize()

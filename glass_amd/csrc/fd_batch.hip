@@ -368,6 +368,12 @@ struct glass_hip_fd_batch
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int g_nprop = -1;
+    // profiling (eager evals only): events around each class kernel
+    bool prof = false;
+    hipEvent_t ev[kNClass][2] = {};
+    double prof_ms[kNClass] = {0};
+    int prof_launches[kNClass] = {0};
+    int prof_count[kNClass] = {0};
     const void* g_key[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
 };
 
@@ -434,6 +440,8 @@ static int launch_items(glass_hip_fd_batch* P, int n, const Items& it, int mode,
         const int* cnt = P->cls + k;
         int g = (mode == 0 ? P->grid[k] : P->tgrid[k]);
         if (g > n) g = n;
+        const bool pr = P->prof && mode == 0;
+        if (pr) GH_CHECK(hipEventRecord(P->ev[k][0], s));
 #define GH_L(kk, N)                                                                                             \
     case kk:                                                                                                    \
         if (mode == 0)                                                                                          \
@@ -446,8 +454,24 @@ static int launch_items(glass_hip_fd_batch* P, int n, const Items& it, int mode,
             GH_L(0, 32) GH_L(1, 64) GH_L(2, 128) GH_L(3, 256) GH_L(4, 512) GH_L(5, 1024) GH_L(6, 2048) GH_L(7, 4096)
         }
 #undef GH_L
+        if (pr) GH_CHECK(hipEventRecord(P->ev[k][1], s));
     }
     GH_CHECK(hipGetLastError());
+    if (P->prof && mode == 0)
+    {
+        // synchronous in profiling mode: accumulate the class kernels' times
+        int cnt[kNClass + 1];
+        GH_CHECK(hipMemcpyAsync(cnt, P->cls, sizeof(int) * (kNClass + 1), hipMemcpyDeviceToHost, s));
+        GH_CHECK(hipStreamSynchronize(s));
+        for (int k = 0; k < kNClass; k++)
+        {
+            float ms = 0.f;
+            GH_CHECK(hipEventElapsedTime(&ms, P->ev[k][0], P->ev[k][1]));
+            P->prof_ms[k] += ms;
+            P->prof_launches[k] += 1;
+            P->prof_count[k] = cnt[k];
+        }
+    }
     return GLASS_HIP_OK;
 }
 
@@ -634,6 +658,9 @@ extern "C" int glass_hip_fd_batch_destroy(glass_hip_fd_batch* P)
     if (!P) return GLASS_HIP_OK;
     (void)hipSetDevice(P->device);
     if (P->gexec) (void)hipGraphExecDestroy(P->gexec);
+    for (int k = 0; k < kNClass; k++)
+        for (int j = 0; j < 2; j++)
+            if (P->ev[k][j]) (void)hipEventDestroy(P->ev[k][j]);
     if (P->graph) (void)hipGraphDestroy(P->graph);
     (void)hipDeviceSynchronize();
     for (void* p : P->allocs) (void)hipFree(p);
@@ -744,6 +771,43 @@ extern "C" int glass_hip_fd_batch_refresh(glass_hip_fd_batch* P, void* stream)
     }
     GH_CHECK(hipSetDevice(P->device));
     return rebuild(P, (hipStream_t)stream);
+}
+
+extern "C" int glass_hip_fd_batch_profile(glass_hip_fd_batch* P, int on)
+{
+    if (!P)
+    {
+        set_error("fd_batch_profile: null plan");
+        return GLASS_HIP_EINVAL;
+    }
+    GH_CHECK(hipSetDevice(P->device));
+    if (on && !P->ev[0][0])
+        for (int k = 0; k < kNClass; k++)
+            for (int j = 0; j < 2; j++) GH_CHECK(hipEventCreate(&P->ev[k][j]));
+    P->prof = on != 0;
+    for (int k = 0; k < kNClass; k++)
+    {
+        P->prof_ms[k] = 0;
+        P->prof_launches[k] = 0;
+        P->prof_count[k] = 0;
+    }
+    return GLASS_HIP_OK;
+}
+
+extern "C" int glass_hip_fd_batch_profile_read(glass_hip_fd_batch* P, double* ms, int* launches, int* count)
+{
+    if (!P || !ms || !launches || !count)
+    {
+        set_error("fd_batch_profile_read: bad arguments");
+        return GLASS_HIP_EINVAL;
+    }
+    for (int k = 0; k < kNClass; k++)
+    {
+        ms[k] = P->prof_ms[k];
+        launches[k] = P->prof_launches[k];
+        count[k] = P->prof_count[k];
+    }
+    return GLASS_HIP_OK;
 }
 
 extern "C" int glass_hip_fd_batch_state(glass_hip_fd_batch* P, double* logL, double* src_params)

@@ -87,6 +87,18 @@ int glass_hip_fd_batch_accept(glass_hip_fd_batch *plan, int nacc, const int *d_i
  * parameters (removes drift accumulated by accepts). */
 int glass_hip_fd_batch_refresh(glass_hip_fd_batch *plan, void *stream);
 
+/* Profiling of the eager glass_hip_fd_batch_eval (not the graph path):
+ * with on != 0 every eval records HIP events around each BW-class kernel
+ * on the launch stream, waits for them and accumulates the elapsed time
+ * per class (evals become synchronous).  Turning it on or off clears the
+ * totals.  BW class k covers BW = 32 << k, k = 0..7. */
+int glass_hip_fd_batch_profile(glass_hip_fd_batch *plan, int on);
+
+/* ms[8]: summed kernel time per class; launches[8]: evals timed;
+ * count[8]: proposals per class in the last timed eval. */
+int glass_hip_fd_batch_profile_read(glass_hip_fd_batch *plan, double *ms, int *launches,
+                                    int *count);
+
 /* Copies the chains' current logL (host array of nchain) and, if not
  * NULL, their current source parameters (nchain*8*8). Synchronous. */
 int glass_hip_fd_batch_state(glass_hip_fd_batch *plan, double *logL, double *src_params);
