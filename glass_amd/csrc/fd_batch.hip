@@ -142,6 +142,7 @@ __global__ __launch_bounds__(NTHR) void eval_kernel(const int* __restrict__ list
         const int W = B.g_W[g];
         UcbConst k;
         ucb_setup(item_params(it, e), 8, B.T, B.t0, B.L, B.fstar, k);
+        make_uniform(k);
         double2 dl[S::PER][6];
         fd_link_spectra<N, NTHR>(k, lds, B.pos, kPosN / N, B.tw, kTwN / N, dl);
 
@@ -221,6 +222,7 @@ __global__ __launch_bounds__(NTHR) void tmpl_kernel(const int* __restrict__ list
         const int e = list[w];
         UcbConst k;
         ucb_setup(item_params(it, e), 8, B.T, B.t0, B.L, B.fstar, k);
+        make_uniform(k);
         double2 dl[S::PER][6];
         fd_link_spectra<N, NTHR>(k, lds, B.pos, kPosN / N, B.tw, kTwN / N, dl);
         double2* o = dst + (size_t)(dslot ? dslot[e] : e) * 3 * B.bwcap;

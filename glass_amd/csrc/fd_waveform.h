@@ -2,13 +2,16 @@
 // template: slow series -> six LDS FFTs -> fftshift/normalise -> TDI.
 // ucb/src/glass_ucb_waveform.c:738-1001; TDI utils/src/glass_lisa.c:458-710.
 //
-// Work split: one workgroup of NTHR threads per template of N bins.  Thread
-// t generates slow samples n = t + k*NTHR and afterwards owns output bins
-// m = t + k*NTHR (k < PER = N/NTHR).  The six link series go through LDS in
-// passes of NLP links (6 when 96*N bytes fit in 96 KB of the 160 KB LDS,
-// i.e. N <= 1024; 3 at N = 2048; 2 at N = 4096); after each pass's FFT every thread
-// copies its bins' link values into registers, so the TDI combination reads
-// registers only.
+// Work split: one workgroup of NTHR threads per template of N bins; thread
+// t owns output bins m = t + k*NTHR (k < PER).
+//  * N <= 1024: NTHR = max(N, 64) threads, one slow sample each, written
+//    straight into LDS (6 links x N complex = 96 N bytes <= 96 KB), six
+//    FFTs, then every thread reads its bin's six link values.
+//  * N = P*1024 (P = 2, 4): decimation in time.  The samples n = P*t + p of
+//    each phase p form a 1024-point series; its six FFTs run in LDS in turn
+//    and each thread folds X_{t+1024 r} += exp(-2 pi i p (t+1024 r)/N) S_p[t]
+//    into registers, so LDS stays at 96 KB and no sample is held in
+//    registers across passes.
 #pragma once
 #include "fft_lds.h"
 #include "ucb_common.h"
@@ -18,11 +21,45 @@ namespace glass {
 template <int N, int NTHR>
 struct FdShape
 {
-    static constexpr int PER = (N + NTHR - 1) / NTHR;  // samples (and bins) per thread
-    // links per LDS pass: all six while they fit in 96 KB, else 3, 2, 1
-    static constexpr int NLP = (6 * N * 16 <= 98304) ? 6 : (3 * N * 16 <= 98304) ? 3 : (2 * N * 16 <= 131072) ? 2 : 1;
-    static constexpr int LDS = NLP * N;                          // double2 elements
+    static constexpr int PER = (N + NTHR - 1) / NTHR;  // bins per thread
+    static constexpr int NS = N <= 1024 ? N : 1024;     // FFT length in LDS
+    static constexpr int LDS = 6 * NS;                  // double2 elements
 };
+
+// Move a value that every lane holds identically into a scalar register.
+__device__ inline double uniform(double x)
+{
+    int lo = __builtin_amdgcn_readfirstlane(__double2loint(x));
+    int hi = __builtin_amdgcn_readfirstlane(__double2hiint(x));
+    return __hiloint2double(hi, lo);
+}
+
+__device__ inline void make_uniform(UcbConst& c)
+{
+    double* f = &c.T;
+    // every double member up to (not including) q
+    constexpr int nd = (int)((offsetof(UcbConst, q)) / sizeof(double));
+#pragma unroll
+    for (int i = 0; i < nd; i++) f[i] = uniform(f[i]);
+    c.q = ((long)__builtin_amdgcn_readfirstlane((int)(c.q >> 32)) << 32) |
+          (unsigned)__builtin_amdgcn_readfirstlane((int)(c.q & 0xffffffff));
+    c.np = __builtin_amdgcn_readfirstlane(c.np);
+}
+
+// Slow samples of one phase into LDS: sample n = P*j + p at LDS slot j.
+template <int N, int NTHR, int P>
+__device__ inline void fd_samples_to_lds(const UcbConst& c, double2* lds, const double* __restrict__ pos,
+                                         int posstride, int p)
+{
+    constexpr int NS = N / P;
+    for (int j = threadIdx.x; j < NS; j += NTHR)
+    {
+        const int n = P * j + p;
+        double t = ucb_sample_time(c.T, c.t0, n, N);
+        ucb_slow_sample(c, t, pos + (size_t)n * posstride * 9, lds, NS, j);
+    }
+    __syncthreads();
+}
 
 // Build the six shifted, normalised link spectra of one template: on return
 // d[k][l] holds link l at output bin m = threadIdx.x + k*NTHR, i.e.
@@ -35,34 +72,11 @@ __device__ inline void fd_link_spectra(const UcbConst& c, double2* lds, const do
                                        double2 (&d)[FdShape<N, NTHR>::PER][6])
 {
     using S = FdShape<N, NTHR>;
-    double2 sv[S::PER][6];
-#pragma unroll
-    for (int k = 0; k < S::PER; k++)
-    {
-        const int n = threadIdx.x + k * NTHR;
-        if (n < N)
-        {
-            double t = ucb_sample_time(c.T, c.t0, n, N);
-            double s[12];
-            ucb_slow_sample(c, t, pos + (size_t)n * posstride * 9, s);
-#pragma unroll
-            for (int l = 0; l < 6; l++) sv[k][l] = make_double2(s[2 * l], s[2 * l + 1]);
-        }
-    }
     const double inv2N = 1. / (double)(2 * N);
-#pragma unroll
-    for (int pass = 0; pass < 6 / S::NLP; pass++)
+    if constexpr (N <= 1024)
     {
-#pragma unroll
-        for (int k = 0; k < S::PER; k++)
-        {
-            const int n = threadIdx.x + k * NTHR;
-            if (n < N)
-#pragma unroll
-                for (int l = 0; l < S::NLP; l++) lds[l * N + n] = sv[k][pass * S::NLP + l];
-        }
-        __syncthreads();
-        lds_fft<N, S::NLP, NTHR>(lds, tw, twstride);
+        fd_samples_to_lds<N, NTHR, 1>(c, lds, pos, posstride, 0);
+        lds_fft<N, 6, NTHR>(lds, tw, twstride);
 #pragma unroll
         for (int k = 0; k < S::PER; k++)
         {
@@ -70,13 +84,56 @@ __device__ inline void fd_link_spectra(const UcbConst& c, double2* lds, const do
             const int s = (m + N / 2) & (N - 1);
             if (m < N)
 #pragma unroll
-                for (int l = 0; l < S::NLP; l++)
+                for (int l = 0; l < 6; l++)
                 {
                     double2 v = lds[l * N + s];
-                    d[k][pass * S::NLP + l] = make_double2(v.x * inv2N, v.y * inv2N);
+                    d[k][l] = make_double2(v.x * inv2N, v.y * inv2N);
                 }
         }
         __syncthreads();
+    }
+    else
+    {
+        static_assert(NTHR == 1024 && (N == 2048 || N == 4096), "decimated path: N = 2048 or 4096");
+        constexpr int P = N / 1024;
+        const int t = threadIdx.x;
+        double2 X[P][6];
+#pragma unroll
+        for (int r = 0; r < P; r++)
+#pragma unroll
+            for (int l = 0; l < 6; l++) X[r][l] = make_double2(0.0, 0.0);
+#pragma unroll 1
+        for (int p = 0; p < P; p++)
+        {
+            fd_samples_to_lds<N, NTHR, P>(c, lds, pos, posstride, p);
+            lds_fft<1024, 6, NTHR>(lds, tw, twstride * P);
+            // w_N^{p t} and the radix-P rotations w_P^{p r}
+            const double2 wt = tw[(p * t) * twstride];
+#pragma unroll
+            for (int l = 0; l < 6; l++)
+            {
+                double2 v = cmul(lds[l * 1024 + t], wt);
+#pragma unroll
+                for (int r = 0; r < P; r++)
+                {
+                    // exp(-2 pi i p r / P) = tw[p*r*(kTw/P)], kTw = N*twstride
+                    double2 w = tw[((p * r) % P) * (N / P) * twstride];
+                    double2 u = cmul(v, w);
+                    X[r][l].x += u.x;
+                    X[r][l].y += u.y;
+                }
+            }
+            __syncthreads();
+        }
+        // FFT index s = t + 1024 r  <->  output bin m = (s + N/2) mod N,
+        // which is bin k = (r + P/2) mod P of this thread
+#pragma unroll
+        for (int r = 0; r < P; r++)
+        {
+            const int k = (r + P / 2) % P;
+#pragma unroll
+            for (int l = 0; l < 6; l++) d[k][l] = make_double2(X[r][l].x * inv2N, X[r][l].y * inv2N);
+        }
     }
 }
 
@@ -91,7 +148,7 @@ __device__ inline void fd_tdi(const UcbConst& c, int m, int fmt, const double2* 
     const double f = ((double)(c.q + m - N / 2)) / c.T;
     const double fonfs = f * c.invfstar;
     double s1, c1;
-    sincos(fonfs, &s1, &c1);
+    sincos_red(fonfs, &s1, &c1);
     const double c2 = 2. * c1 * c1 - 1.0, s2 = 2. * s1 * c1;
     const double sqT = sqrt(c.T);
     // link combinations a = d_ij - d_ik, b = d_ji - d_ki per channel

@@ -10,7 +10,10 @@
 // (detector tensors, transfer-function sinc) are free to contract.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
+
+#include "fastmath.h"
 
 namespace glass {
 
@@ -98,12 +101,14 @@ __device__ inline double quad3(const double* E, double a, double b, double cc)
     return E[0] * a * a + E[1] * b * b + E[2] * cc * cc + 2.0 * (E[3] * a * b + E[4] * a * cc + E[5] * b * cc);
 }
 
-// One slow-series sample of the six links at time t.
+// One slow-series sample of the six links at time t, written to
+// lds[l*stride + j] for link l in the order 12, 21, 31, 13, 23, 32.
 // pos: spacecraft positions x1,x2,x3,y1,y2,y3,z1,z2,z3 at t (metres).
-// out: TR,TI for links 12,21,31,13,23,32 (12 doubles).
 // glass_ucb_waveform.c:843-936, LISA_detector_tensor glass_lisa.c:1521-1564.
+// The loop runs over the sending spacecraft i; its two links (i,j) share
+// xi_i, arg2_i and the frequency at spacecraft i.
 __device__ inline void ucb_slow_sample(const UcbConst& c, double t, const double* __restrict__ pos,
-                                       double* out)
+                                       double2* lds, int stride, int j)
 {
     const double x1 = pos[0], x2 = pos[1], x3 = pos[2];
     const double y1 = pos[3], y2 = pos[4], y3 = pos[5];
@@ -119,20 +124,25 @@ __device__ inline void ucb_slow_sample(const UcbConst& c, double t, const double
     const double k12 = c.k[0] * r12a + c.k[1] * r12b + c.k[2] * r12c;
     const double k13 = c.k[0] * r13a + c.k[1] * r13b + c.k[2] * r13c;
     const double k23 = c.k[0] * r23a + c.k[1] * r23b + c.k[2] * r23c;
-
-    double arg2[3], fon[3], aev[3];
+    double dft;
     {
 #pragma clang fp contract(off)
-        const double px[3] = {x1, x2, x3}, py[3] = {y1, y2, y3}, pz[3] = {z1, z2, z3};
-        const double dft = c.df * t;
-#pragma unroll
-        for (int i = 0; i < 3; i++)
+        dft = c.df * t;
+    }
+#pragma unroll 1
+    for (int i = 0; i < 3; i++)
+    {
+        const double px = i == 0 ? x1 : (i == 1 ? x2 : x3);
+        const double py = i == 0 ? y1 : (i == 1 ? y2 : y3);
+        const double pz = i == 0 ? z1 : (i == 1 ? z2 : z3);
+        double a2, fon, ae;
         {
-            double kdotx = (px[i] * c.k[0] + py[i] * c.k[1] + pz[i] * c.k[2]) / kCLIGHT;
+#pragma clang fp contract(off)
+            double kdotx = (px * c.k[0] + py * c.k[1] + pz * c.k[2]) / kCLIGHT;
             double xi = t - kdotx;
             double fi = c.f0;
-            double a2 = c.pi2f0 * xi - c.phi0 - dft;
-            double ae = 1.0;
+            a2 = c.pi2f0 * xi - c.phi0 - dft;
+            ae = 1.0;
             if (c.np > 7)
             {
                 fi += c.dfdt * xi;
@@ -144,28 +154,27 @@ __device__ inline void ucb_slow_sample(const UcbConst& c, double t, const double
                 fi += c.halfd2 * xi * xi;
                 a2 += c.pid2 * xi * xi * xi;
             }
-            arg2[i] = a2;
-            fon[i] = fi * c.invfstar;
-            aev[i] = ae;
+            fon = fi * c.invfstar;
         }
-    }
-    // links: (i,j) with kdotr[i][j] antisymmetric, dplus/dcross symmetric
-    const int li[6] = {0, 1, 2, 0, 1, 2};
-    const double kr[6] = {k12, -k12, -k13, k13, k23, -k23};
-    const double dpl[6] = {dp12, dp12, dp13, dp13, dp23, dp23};
-    const double dcl[6] = {dc12, dc12, dc13, dc13, dc23, dc23};
+        // links (i, j) of spacecraft i: kdotr antisymmetric, d+/dx symmetric
+        const double kra = i == 0 ? k12 : (i == 1 ? -k12 : -k13);
+        const double krb = i == 0 ? k13 : (i == 1 ? k23 : -k23);
+        const double dpa = i == 0 ? dp12 : (i == 1 ? dp12 : dp13);
+        const double dpb = i == 0 ? dp13 : (i == 1 ? dp23 : dp23);
+        const double dca = i == 0 ? dc12 : (i == 1 ? dc12 : dc13);
+        const double dcb = i == 0 ? dc13 : (i == 1 ? dc23 : dc23);
 #pragma unroll
-    for (int l = 0; l < 6; l++)
-    {
-        const int i = li[l];
-        const double arg1 = 0.5 * fon[i] * (1.0 + kr[l]);
-        const double sinc = 0.25 * sin(arg1) / arg1;
-        const double t1r = aev[i] * (dpl[l] * c.DPr + dcl[l] * c.DCr);
-        const double t1i = aev[i] * (dpl[l] * c.DPi + dcl[l] * c.DCi);
-        double s2, c2;
-        sincos(arg1 - arg2[i], &s2, &c2);
-        out[2 * l] = sinc * (t1r * c2 + t1i * s2);
-        out[2 * l + 1] = sinc * (-t1r * s2 + t1i * c2);
+        for (int h = 0; h < 2; h++)
+        {
+            const double kr = h ? krb : kra, dp = h ? dpb : dpa, dc = h ? dcb : dca;
+            const double arg1 = 0.5 * fon * (1.0 + kr);
+            const double sinc = 0.25 * sinc_ratio(arg1);
+            const double t1r = ae * (dp * c.DPr + dc * c.DCr);
+            const double t1i = ae * (dp * c.DPi + dc * c.DCi);
+            double s2, c2;
+            sincos_red(arg1 - a2, &s2, &c2);
+            lds[(i + 3 * h) * stride + j] = make_double2(sinc * (t1r * c2 + t1i * s2), sinc * (-t1r * s2 + t1i * c2));
+        }
     }
 }
 
