@@ -147,6 +147,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--copy-calibrate", action="store_true",
+                    help="also run a 256 MiB device copy, the reference point that scales the "
+                         "FETCH_SIZE/WRITE_SIZE counters of a profiled run")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs on a single MI355X and has no multi-GPU "
@@ -212,6 +215,15 @@ def main():
                 "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(ach_tf / FP64_PEAK_TFLOPS, 5),
                 "traffic": None, "avg_launch_ms": round(avg_ms, 4), "evals_per_launch": int(kcount[kd]),
                 "hbm_algorithmic_GBs": round(ach_gbs, 2), "hbm_frac": round(ach_gbs / HBM_PEAK_GBS, 5)}
+
+    if args.copy_calibrate:
+        a = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")  # 256 MiB
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log("copy calibration: 256 MiB read + 256 MiB written")
+        del a, b
 
     plain = None
     if not args.no_plain:

@@ -16,13 +16,18 @@ FLAGS = [
     # packed-FP32 instructions are kept out of every kernel
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
+    # keep loop-invariant values inside the loops that use them: hoisting
+    # them (FP64 constants, LDS addresses) out of the per-template loops
+    # costs ~60 VGPRs and halves occupancy of the template kernels
+    "-mllvm", "-disable-machine-licm",
 ]
+EXTRA = os.environ.get("GLASS_HIP_EXTRA_FLAGS", "").split()
 
 
 def build(verbose=False):
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = LIB + ".tmp"
-    cmd = [HIPCC] + FLAGS + srcs + ["-o", tmp]
+    cmd = [HIPCC] + FLAGS + EXTRA + srcs + ["-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

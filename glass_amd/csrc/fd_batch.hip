@@ -80,32 +80,69 @@ __global__ void zero_int_kernel(int* p, int n)
 }
 
 // ucb_alignment on the device (ucb/src/glass_ucb_waveform.c:727-736) and
-// bucketing by BW class.  A BW above the cap marks the item with class -1.
-__global__ void align_kernel(int n, Items it, BatchDev B, int* __restrict__ bw_out, int* __restrict__ imin_out,
-                             int* __restrict__ cls_count, int* __restrict__ cls_list, int cls_stride,
-                             int* __restrict__ err)
+// bucketing by BW class.  Class slots are reserved per block (an LDS
+// histogram, then one global atomic per class per block) so the class
+// counters are not hammered by every item.  Items whose BW falls outside
+// the classes are counted in *err and left out.
+__global__ __launch_bounds__(256) void align_kernel(int n, Items it, BatchDev B, int* __restrict__ bw_out,
+                                                    int* __restrict__ imin_out, int* __restrict__ cls_count,
+                                                    int* __restrict__ cls_list, int cls_stride, int* __restrict__ err)
 {
-    int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    const double* p = item_params(it, e);
-    int c = item_chain(it, e);
-    int g = B.c_group[c];
-    int W = B.g_W[g];
-    double f0 = p[0] / B.T;
-    double costh = p[1], amp = exp(p[3]);
-    double dfdt = p[7] / (B.T * B.T);
-    int BW = 2 * ucb_bandwidth(B.L, B.fstar, f0, dfdt, costh, amp, B.T, W);
-    int qmin = (int)(f0 * B.T) - BW / 2;
-    bw_out[e] = BW;
-    imin_out[e] = qmin - B.g_qmin[g];
-    int k = 31 - __clz(BW) - 5;  // log2(BW) - 5
-    if (BW > B.bwcap || k < 0 || k >= kNClass || (BW & (BW - 1)))
+    __shared__ int hist[kNClass + 1], base[kNClass + 1];
+    if (threadIdx.x <= kNClass) hist[threadIdx.x] = 0;
+    __syncthreads();
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    int k = -1, slot = 0;
+    if (e < n)
     {
-        atomicAdd(err, 1);
-        return;
+        const double* p = item_params(it, e);
+        int c = item_chain(it, e);
+        int g = B.c_group[c];
+        int W = B.g_W[g];
+        double f0 = p[0] / B.T;
+        double costh = p[1], amp = exp(p[3]);
+        double dfdt = p[7] / (B.T * B.T);
+        int BW = 2 * ucb_bandwidth(B.L, B.fstar, f0, dfdt, costh, amp, B.T, W);
+        int qmin = (int)(f0 * B.T) - BW / 2;
+        bw_out[e] = BW;
+        imin_out[e] = qmin - B.g_qmin[g];
+        k = 31 - __clz(BW) - 5;  // log2(BW) - 5
+        if (BW > B.bwcap || k < 0 || k >= kNClass || (BW & (BW - 1))) k = kNClass;
+        slot = atomicAdd(&hist[k], 1);
     }
-    int pos = atomicAdd(&cls_count[k], 1);
-    cls_list[k * cls_stride + pos] = e;
+    __syncthreads();
+    if (threadIdx.x < kNClass && hist[threadIdx.x]) base[threadIdx.x] = atomicAdd(&cls_count[threadIdx.x], hist[threadIdx.x]);
+    if (threadIdx.x == kNClass && hist[kNClass]) atomicAdd(err, hist[kNClass]);
+    __syncthreads();
+    if (k >= 0 && k < kNClass) cls_list[k * cls_stride + base[k] + slot] = e;
+}
+
+// Work distribution of the class kernels.  Persistent: a grid sized to the
+// resident capacity loops over the class list.  Otherwise: one workgroup
+// per proposal slot (grid = proposal count), workgroups beyond the class
+// count exit at once; the kernel body then has no loop, so nothing is
+// hoisted across templates and the register footprint stays that of one
+// template.
+#ifndef GH_PERSISTENT
+#define GH_PERSISTENT 0
+#endif
+#if GH_PERSISTENT
+#define GH_ITEM_LOOP for (int w = blockIdx.x; w < cnt; w += gridDim.x)
+#else
+#define GH_ITEM_LOOP if (const int w = blockIdx.x; w < cnt)
+#endif
+
+// The persistent loops rebuild a template per iteration from tables that do
+// not change between iterations (FFT twiddles, spacecraft positions).  Left
+// alone, the compiler hoists those loads out of the loop and keeps every
+// twiddle of every FFT stage live in registers (~100 extra VGPRs, halving
+// occupancy).  Passing the table pointers through an empty asm each
+// iteration makes the loads loop-variant again.
+template <typename T>
+__device__ inline const T* opaque(const T* p)
+{
+    asm volatile("" : "+s"(p));
+    return p;
 }
 
 // symmetric 3x3 weighted bilinear form of complex 3-vectors:
@@ -134,17 +171,15 @@ __global__ __launch_bounds__(NTHR) void eval_kernel(const int* __restrict__ list
     using S = FdShape<N, NTHR>;
     __shared__ double2 lds[S::LDS];
     const int cnt = *count;
-    for (int w = blockIdx.x; w < cnt; w += gridDim.x)
+    GH_ITEM_LOOP
     {
         const int e = list[w];
         const int c = item_chain(it, e), s = item_src(it, e);
         const int g = B.c_group[c];
         const int W = B.g_W[g];
-        UcbConst k;
-        ucb_setup(item_params(it, e), 8, B.T, B.t0, B.L, B.fstar, k);
-        make_uniform(k);
+        const UcbConst& k = ucb_setup_shared(item_params(it, e), 8, B.T, B.t0, B.L, B.fstar);
         double2 dl[S::PER][6];
-        fd_link_spectra<N, NTHR>(k, lds, B.pos, kPosN / N, B.tw, kTwN / N, dl);
+        fd_link_spectra<N, NTHR>(k, lds, opaque(B.pos), kPosN / N, opaque(B.tw), kTwN / N, dl);
 
         const int so = c * kMaxSrc + s;
         const int imin_n = p_imin[e];
@@ -217,14 +252,12 @@ __global__ __launch_bounds__(NTHR) void tmpl_kernel(const int* __restrict__ list
     using S = FdShape<N, NTHR>;
     __shared__ double2 lds[S::LDS];
     const int cnt = *count;
-    for (int w = blockIdx.x; w < cnt; w += gridDim.x)
+    GH_ITEM_LOOP
     {
         const int e = list[w];
-        UcbConst k;
-        ucb_setup(item_params(it, e), 8, B.T, B.t0, B.L, B.fstar, k);
-        make_uniform(k);
+        const UcbConst& k = ucb_setup_shared(item_params(it, e), 8, B.T, B.t0, B.L, B.fstar);
         double2 dl[S::PER][6];
-        fd_link_spectra<N, NTHR>(k, lds, B.pos, kPosN / N, B.tw, kTwN / N, dl);
+        fd_link_spectra<N, NTHR>(k, lds, opaque(B.pos), kPosN / N, opaque(B.tw), kTwN / N, dl);
         double2* o = dst + (size_t)(dslot ? dslot[e] : e) * 3 * B.bwcap;
 #pragma unroll
         for (int kk = 0; kk < S::PER; kk++)
@@ -440,7 +473,7 @@ static int launch_items(glass_hip_fd_batch* P, int n, const Items& it, int mode,
     {
         const int* list = P->cls_list + (size_t)k * P->cls_cap;
         const int* cnt = P->cls + k;
-        int g = (mode == 0 ? P->grid[k] : P->tgrid[k]);
+        int g = GH_PERSISTENT ? (mode == 0 ? P->grid[k] : P->tgrid[k]) : n;
         if (g > n) g = n;
         const bool pr = P->prof && mode == 0;
         if (pr) GH_CHECK(hipEventRecord(P->ev[k][0], s));

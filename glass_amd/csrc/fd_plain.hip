@@ -18,9 +18,7 @@ __global__ __launch_bounds__(NTHR) void fd_waveform_kernel(const double* __restr
 {
     using S = FdShape<N, NTHR>;
     __shared__ double2 lds[S::LDS];
-    UcbConst c;
-    ucb_setup(params + (size_t)blockIdx.x * np, np, T, t0, L, fstar, c);
-    make_uniform(c);
+    const UcbConst& c = ucb_setup_shared(params + (size_t)blockIdx.x * np, np, T, t0, L, fstar);
     double2 d[S::PER][6];
     fd_link_spectra<N, NTHR>(c, lds, pos, posstride, tw, twstride, d);
     double2* o = out + (size_t)blockIdx.x * 5 * N;

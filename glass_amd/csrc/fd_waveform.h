@@ -46,6 +46,19 @@ __device__ inline void make_uniform(UcbConst& c)
     c.np = __builtin_amdgcn_readfirstlane(c.np);
 }
 
+// Per-template constants computed once per workgroup (by thread 0) into
+// LDS; every thread reads them from there.  Synchronises the workgroup on
+// entry (readers of the previous template are done) and on exit.
+__device__ inline const UcbConst& ucb_setup_shared(const double* p, int np, double T, double t0, double L,
+                                                   double fstar)
+{
+    __shared__ UcbConst sc;
+    __syncthreads();
+    if (threadIdx.x == 0) ucb_setup(p, np, T, t0, L, fstar, sc);
+    __syncthreads();
+    return sc;
+}
+
 // Slow samples of one phase into LDS: sample n = P*j + p at LDS slot j.
 template <int N, int NTHR, int P>
 __device__ inline void fd_samples_to_lds(const UcbConst& c, double2* lds, const double* __restrict__ pos,
@@ -145,12 +158,14 @@ template <int N>
 __device__ inline void fd_tdi(const UcbConst& c, int m, int fmt, const double2* d, double2* out)
 {
     const double2 d12 = d[0], d21 = d[1], d31 = d[2], d13 = d[3], d23 = d[4], d32 = d[5];
-    const double f = ((double)(c.q + m - N / 2)) / c.T;
+    // f = (q + m - N/2)/T (glass_lisa.c:664); amplitude path, so the
+    // reciprocal is folded in
+    const double f = ((double)(c.q + m - N / 2)) * c.invT;
     const double fonfs = f * c.invfstar;
     double s1, c1;
     sincos_red(fonfs, &s1, &c1);
     const double c2 = 2. * c1 * c1 - 1.0, s2 = 2. * s1 * c1;
-    const double sqT = sqrt(c.T);
+    const double sqT = c.sqT;
     // link combinations a = d_ij - d_ik, b = d_ji - d_ki per channel
     const double2 aX = make_double2(d12.x - d13.x, d12.y - d13.y), bX = make_double2(d21.x - d31.x, d21.y - d31.y);
     const double2 aY = make_double2(d23.x - d21.x, d23.y - d21.y), bY = make_double2(d32.x - d12.x, d32.y - d12.y);

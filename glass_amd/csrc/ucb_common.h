@@ -43,6 +43,7 @@ struct UcbConst
     double k[3];
     double ep[6], ec[6];  // symmetric tensors: xx, yy, zz, xy, xz, yz
     double invfstar, fstar, invL;
+    double sqT, invT;  // sqrt(T), 1/T (TDI prefactor and bin frequency)
     long q;
     int np;
 };
@@ -93,6 +94,8 @@ __host__ __device__ inline void ucb_setup(const double* p, int np, double T, dou
     c.fstar = fstar;
     c.invfstar = 1. / fstar;
     c.invL = 1. / L;
+    c.sqT = sqrt(T);
+    c.invT = 1. / T;
 }
 
 // r^T E r for a symmetric 3x3 tensor stored as xx,yy,zz,xy,xz,yz
