@@ -422,7 +422,7 @@ static int dalloc(glass_hip_fd_batch* P, T** p, size_t n)
     return GLASS_HIP_OK;
 }
 
-#define GH_NTHR_(N) ((N) < 64 ? 64 : ((N) > 1024 ? 1024 : (N)))
+#define GH_NTHR_(N) (::glass::fd_threads(N))
 #define GH_NTHR(N) GH_NTHR_(N)
 
 // Persistent grid: workgroups resident per CU as bounded by LDS (160 KB)

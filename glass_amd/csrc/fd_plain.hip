@@ -39,7 +39,7 @@ static void launch_fd(int ntmpl, const double* params, int np, double T, double 
                       double fstar, const double* pos, int posstride, const double2* tw, int fmt,
                       double2* out, hipStream_t s)
 {
-    constexpr int NTHR = N < 64 ? 64 : (N > 1024 ? 1024 : N);
+    constexpr int NTHR = fd_threads(N);
     fd_waveform_kernel<N, NTHR><<<ntmpl, NTHR, 0, s>>>(params, np, T, t0, L, fstar, pos, posstride, tw,
                                                         kTwN / N, fmt, out);
 }

@@ -7,22 +7,39 @@
 //  * N <= 1024: NTHR = max(N, 64) threads, one slow sample each, written
 //    straight into LDS (6 links x N complex = 96 N bytes <= 96 KB), six
 //    FFTs, then every thread reads its bin's six link values.
-//  * N = P*1024 (P = 2, 4): decimation in time.  The samples n = P*t + p of
-//    each phase p form a 1024-point series; its six FFTs run in LDS in turn
-//    and each thread folds X_{t+1024 r} += exp(-2 pi i p (t+1024 r)/N) S_p[t]
-//    into registers, so LDS stays at 96 KB and no sample is held in
-//    registers across passes.
+//  * N = P*NS (N >= 2048, NS = 1024): decimation in time.  The samples
+//    n = P*j + p of each phase p form an NS-point series; its six FFTs run
+//    in LDS in turn and each thread folds X_{j+NS r} += w_N^{p j} w_P^{p r}
+//    S_p[j] into registers for its sub-bins j, so LDS stays at 6*NS*16
+//    bytes and no sample is held in registers across passes.
 #pragma once
 #include "fft_lds.h"
 #include "ucb_common.h"
 
+#ifndef GH_NTHR_1024
+#define GH_NTHR_1024 512
+#endif
+#ifndef GH_DECIM_NS
+#define GH_DECIM_NS 1024
+#endif
+#ifndef GH_NTHR_DECIM
+#define GH_NTHR_DECIM 512
+#endif
+
 namespace glass {
+
+// Workgroup size for a template of N bins: one sample per thread up to 512,
+// two at N = 1024 (keeps the 256-VGPR budget of 8-wave workgroups), and
+// 1024 threads for the decimated N >= 2048 path.
+constexpr int fd_threads(int N) { return N < 64 ? 64 : (N <= 512 ? N : (N == 1024 ? GH_NTHR_1024 : GH_NTHR_DECIM)); }
 
 template <int N, int NTHR>
 struct FdShape
 {
     static constexpr int PER = (N + NTHR - 1) / NTHR;  // bins per thread
-    static constexpr int NS = N <= 1024 ? N : 1024;     // FFT length in LDS
+    // FFT length in LDS: the whole template up to 1024 bins, else the
+    // sub-series length of the decimated path
+    static constexpr int NS = N <= 1024 ? N : GH_DECIM_NS;
     static constexpr int LDS = 6 * NS;                  // double2 elements
 };
 
@@ -107,46 +124,55 @@ __device__ inline void fd_link_spectra(const UcbConst& c, double2* lds, const do
     }
     else
     {
-        static_assert(NTHR == 1024 && (N == 2048 || N == 4096), "decimated path: N = 2048 or 4096");
-        constexpr int P = N / 1024;
+        // decimation in time: P phases of NS-point sub-series
+        constexpr int NS = S::NS, P = N / NS, U = NS / NTHR;
+        static_assert(NS % NTHR == 0 && N % NS == 0, "decimated shape");
         const int t = threadIdx.x;
-        double2 X[P][6];
+        double2 X[U][P][6];
 #pragma unroll
-        for (int r = 0; r < P; r++)
+        for (int u = 0; u < U; u++)
 #pragma unroll
-            for (int l = 0; l < 6; l++) X[r][l] = make_double2(0.0, 0.0);
+            for (int r = 0; r < P; r++)
+#pragma unroll
+                for (int l = 0; l < 6; l++) X[u][r][l] = make_double2(0.0, 0.0);
 #pragma unroll 1
         for (int p = 0; p < P; p++)
         {
             fd_samples_to_lds<N, NTHR, P>(c, lds, pos, posstride, p);
-            lds_fft<1024, 6, NTHR>(lds, tw, twstride * P);
-            // w_N^{p t} and the radix-P rotations w_P^{p r}
-            const double2 wt = tw[(p * t) * twstride];
+            lds_fft<NS, 6, NTHR>(lds, tw, twstride * P);
 #pragma unroll
-            for (int l = 0; l < 6; l++)
+            for (int u = 0; u < U; u++)
             {
-                double2 v = cmul(lds[l * 1024 + t], wt);
+                const int j = t + u * NTHR;
+                // w_N^{p j}; the radix-P rotations are w_P^{p r}
+                const double2 wj = tw[(p * j) * twstride];
 #pragma unroll
-                for (int r = 0; r < P; r++)
+                for (int l = 0; l < 6; l++)
                 {
-                    // exp(-2 pi i p r / P) = tw[p*r*(kTw/P)], kTw = N*twstride
-                    double2 w = tw[((p * r) % P) * (N / P) * twstride];
-                    double2 u = cmul(v, w);
-                    X[r][l].x += u.x;
-                    X[r][l].y += u.y;
+                    const double2 v = cmul(lds[l * NS + j], wj);
+#pragma unroll
+                    for (int r = 0; r < P; r++)
+                    {
+                        const double2 w = tw[((p * r) % P) * NS * twstride];
+                        const double2 q = cmul(v, w);
+                        X[u][r][l].x += q.x;
+                        X[u][r][l].y += q.y;
+                    }
                 }
             }
             __syncthreads();
         }
-        // FFT index s = t + 1024 r  <->  output bin m = (s + N/2) mod N,
-        // which is bin k = (r + P/2) mod P of this thread
+        // FFT index s = j + NS r, j = t + u NTHR, is output bin
+        // m = (s + N/2) mod N = t + k NTHR with k as below
 #pragma unroll
-        for (int r = 0; r < P; r++)
-        {
-            const int k = (r + P / 2) % P;
+        for (int u = 0; u < U; u++)
 #pragma unroll
-            for (int l = 0; l < 6; l++) d[k][l] = make_double2(X[r][l].x * inv2N, X[r][l].y * inv2N);
-        }
+            for (int r = 0; r < P; r++)
+            {
+                const int k = (u + U * r + N / (2 * NTHR)) % (N / NTHR);
+#pragma unroll
+                for (int l = 0; l < 6; l++) d[k][l] = make_double2(X[u][r][l].x * inv2N, X[u][r][l].y * inv2N);
+            }
     }
 }
 
