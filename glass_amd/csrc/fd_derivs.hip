@@ -1,0 +1,257 @@
+// fd_derivs.hip -- exact gradient and Fisher matrix of one galactic binary's
+// likelihood contribution, for many sources at once (no reference
+// counterpart: the reference's ucb_fisher, ucb/src/glass_ucb_waveform.c:
+// 364-522, uses forward differences of 8 extra waveforms).
+//
+// For a source with template h(theta) over a residual R that excludes it,
+// logL = const + L(theta),  L = (R|h) - (h|h)/2,  so
+//   dL/dtheta_k = (R - h | dh/dtheta_k),   Gamma_kl = (dh_k | dh_l),
+// with ( | ) the 3-channel noise-weighted product of gaussian_log_likelihood
+// (utils/src/glass_model.c:758-767).  One workgroup per source: the six
+// series sets of fd_tangent.h go through LDS in as few passes as 96 KB
+// allows, each pass FFT'd and TDI'd into per-bin registers; then every
+// thread accumulates L, the 8 gradient entries and the 36 distinct Fisher
+// entries of its bins and the workgroup reduces the 45 sums.
+#include <cmath>
+
+#include "../../include/glass_hip_batch.h"
+#include "fd_tangent.h"
+#include "reduce.h"
+#include "runtime.h"
+
+namespace glass {
+
+int parse_format(const char* format);
+
+constexpr int kNSum = 45;  // L, grad[8], Fisher upper triangle[36]
+
+// B(u,v) = 2 sum_ab Re(u_a conj v_b) w_ab with w = XX,YY,ZZ,XY,XZ,YZ
+__device__ inline double bf3(const double2* u, const double2* v, const double* w)
+{
+    auto re = [](double2 a, double2 b) { return a.x * b.x + a.y * b.y; };
+    return 2.0 * (w[0] * re(u[0], v[0]) + w[1] * re(u[1], v[1]) + w[2] * re(u[2], v[2]) +
+                  w[3] * (re(u[0], v[1]) + re(u[1], v[0])) + w[4] * (re(u[0], v[2]) + re(u[2], v[0])) +
+                  w[5] * (re(u[1], v[2]) + re(u[2], v[1])));
+}
+
+template <int NTHR>
+__device__ inline void block_sum_vec(double (&v)[kNSum], double* out)
+{
+    constexpr int NW = (NTHR + 63) / 64;
+    __shared__ double part[NW][kNSum];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < kNSum; k++)
+    {
+        double x = wave_sum(v[k]);
+        if (lane == 0) part[w][k] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < kNSum)
+    {
+        double s = 0.0;
+        for (int i = 0; i < NW; i++) s += part[i][threadIdx.x];
+        out[threadIdx.x] = s;
+    }
+    __syncthreads();
+}
+
+struct DerivArgs
+{
+    const double* params;
+    const int* imin;
+    int W;
+    const double2* resid;
+    long long rstride;  // doubles
+    const double* invC;
+    long long istride;  // doubles
+    int mode;           // 0: window-aligned weights; 1: template-local (reference)
+    double T, t0, L, fstar;
+    const double* pos;
+    const double2* tw;
+    int fmt;
+    double* outL;
+    double* grad;
+    double* fisher;
+};
+
+// One pass of series sets [S0, S0+NS): samples -> LDS -> FFT -> TDI into H.
+template <int N, int NTHR, int S0, int NS>
+__device__ inline void deriv_pass(const UcbConst& c, const UcbDiff& dd, double2* lds, const DerivArgs& A,
+                                  double2 (&H)[FdShape<N, NTHR>::PER][6][3])
+{
+    using S = FdShape<N, NTHR>;
+    for (int j = threadIdx.x; j < N; j += NTHR)
+    {
+        const double t = ucb_sample_time(c.T, c.t0, j, N);
+        ucb_sample_sets<S0, NS>(c, dd, t, A.pos + (size_t)j * (4096 / N) * 9, lds, N, j);
+    }
+    __syncthreads();
+    lds_fft<N, 6 * NS, NTHR>(lds, A.tw, kTwN / N);
+    const double inv2N = 1. / (double)(2 * N);
+#pragma unroll
+    for (int k = 0; k < S::PER; k++)
+    {
+        const int m = threadIdx.x + k * NTHR;
+        if (m >= N) continue;
+        const int s = (m + N / 2) & (N - 1);
+#pragma unroll
+        for (int q = 0; q < NS; q++)
+        {
+            double2 d6[6], o5[5];
+#pragma unroll
+            for (int l = 0; l < 6; l++)
+            {
+                double2 v = lds[(q * 6 + l) * N + s];
+                d6[l] = make_double2(v.x * inv2N, v.y * inv2N);
+            }
+            fd_tdi<N>(c, m, A.fmt, d6, o5);
+#pragma unroll
+            for (int ch = 0; ch < 3; ch++) H[k][S0 + q][ch] = o5[ch];
+        }
+    }
+    __syncthreads();
+}
+
+template <int N, int NTHR, int S0>
+__device__ inline void deriv_passes(const UcbConst& c, const UcbDiff& dd, double2* lds, const DerivArgs& A,
+                                    double2 (&H)[FdShape<N, NTHR>::PER][6][3])
+{
+    constexpr int CAP = (98304 / (96 * N)) < 1 ? 1 : (98304 / (96 * N));
+    constexpr int NS = (6 - S0) < CAP ? (6 - S0) : CAP;
+    deriv_pass<N, NTHR, S0, NS>(c, dd, lds, A, H);
+    if constexpr (S0 + NS < 6) deriv_passes<N, NTHR, S0 + NS>(c, dd, lds, A, H);
+}
+
+template <int N, int NTHR>
+__global__ __launch_bounds__(NTHR) void deriv_kernel(DerivArgs A)
+{
+    using S = FdShape<N, NTHR>;
+    constexpr int CAP = (98304 / (96 * N)) < 1 ? 1 : (98304 / (96 * N));
+    constexpr int NSMAX = CAP < 6 ? CAP : 6;
+    __shared__ double2 lds[6 * NSMAX * N];
+    __shared__ UcbDiff sdd;
+    __shared__ double red[kNSum];
+    const int e = blockIdx.x;
+    const double* p = A.params + (size_t)e * 8;
+    const UcbConst& c = ucb_setup_shared(p, 8, A.T, A.t0, A.L, A.fstar);
+    if (threadIdx.x == 0) ucb_diff_setup(p, A.T, sdd);
+    __syncthreads();
+    const UcbDiff& dd = sdd;
+    double2 H[S::PER][6][3];
+    deriv_passes<N, NTHR, 0>(c, dd, lds, A, H);
+
+    const double2* R = A.resid + (size_t)e * (A.rstride / 2);
+    const double* iC = A.invC + (size_t)e * A.istride;
+    const int W = A.W, imin = A.imin[e];
+    const double2 DP = make_double2(c.DPr, c.DPi), DC = make_double2(c.DCr, c.DCi);
+    const double2 DP4 = make_double2(dd.DPr4, dd.DPi4), DC4 = make_double2(dd.DCr4, dd.DCi4);
+    const double2 DP5 = make_double2(dd.DPr5, dd.DPi5), DC5 = make_double2(dd.DCr5, dd.DCi5);
+    double acc[kNSum];
+#pragma unroll
+    for (int k = 0; k < kNSum; k++) acc[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < S::PER; k++)
+    {
+        const int m = threadIdx.x + k * NTHR;
+        if (m >= N) continue;
+        const int j = imin + m;
+        const int wi = A.mode == 1 ? m : j;  // weight index
+        if (wi < 0 || wi >= W) continue;
+        double w[6];
+#pragma unroll
+        for (int q = 0; q < 6; q++) w[q] = iC[(size_t)q * W + wi];
+        double2 h[3], dh[8][3], r[3];
+#pragma unroll
+        for (int ch = 0; ch < 3; ch++)
+        {
+            const double2 hp = H[k][0][ch], hc = H[k][1][ch];
+            h[ch] = make_double2(cmulc(DP, hp).x + cmulc(DC, hc).x, cmulc(DP, hp).y + cmulc(DC, hc).y);
+            dh[0][ch] = H[k][2][ch];
+            dh[1][ch] = H[k][3][ch];
+            dh[2][ch] = H[k][4][ch];
+            dh[3][ch] = h[ch];                                  // d/d ln A
+            dh[4][ch] = make_double2(cmulc(DP4, hp).x + cmulc(DC4, hc).x, cmulc(DP4, hp).y + cmulc(DC4, hc).y);
+            dh[5][ch] = make_double2(cmulc(DP5, hp).x + cmulc(DC5, hc).x, cmulc(DP5, hp).y + cmulc(DC5, hc).y);
+            dh[6][ch] = make_double2(h[ch].y, -h[ch].x);        // d/d phi0 = -i h
+            dh[7][ch] = H[k][5][ch];
+            const bool inwin = j >= 0 && j < W;
+            r[ch] = (A.resid && inwin) ? R[(size_t)ch * W + j] : make_double2(0.0, 0.0);
+        }
+        // likelihood terms only where the template lands in the window
+        if (A.mode == 0 || (j >= 0 && j < W))
+        {
+            double wj[6];
+            if (A.mode == 1)
+            {
+                for (int q = 0; q < 6; q++) wj[q] = (j >= 0 && j < W) ? iC[(size_t)q * W + j] : 0.0;
+            }
+            const double* ww = A.mode == 1 ? wj : w;
+            acc[0] += bf3(r, h, ww) - 0.5 * bf3(h, h, ww);
+            double2 rh[3];
+#pragma unroll
+            for (int ch = 0; ch < 3; ch++) rh[ch] = make_double2(r[ch].x - h[ch].x, r[ch].y - h[ch].y);
+#pragma unroll
+            for (int a = 0; a < 8; a++) acc[1 + a] += bf3(rh, dh[a], ww);
+        }
+        int idx = 9;
+#pragma unroll
+        for (int a = 0; a < 8; a++)
+#pragma unroll
+            for (int b = a; b < 8; b++) acc[idx++] += bf3(dh[a], dh[b], w);
+    }
+    block_sum_vec<NTHR>(acc, red);
+    if (threadIdx.x < 64)
+    {
+        const int a = threadIdx.x / 8, b = threadIdx.x % 8;
+        const int lo = a < b ? a : b, hi = a < b ? b : a;
+        const int idx = 9 + lo * 8 - lo * (lo - 1) / 2 + (hi - lo);
+        A.fisher[(size_t)e * 64 + threadIdx.x] = red[idx];
+        if (threadIdx.x < 8) A.grad[(size_t)e * 8 + threadIdx.x] = red[1 + threadIdx.x];
+        if (threadIdx.x == 0) A.outL[e] = red[0];
+    }
+}
+
+}  // namespace glass
+
+using namespace glass;
+
+extern "C" int glass_hip_fd_source_derivs(const glass_hip_orbit* orbit, const char* format, double T, double t0,
+                                          int n, const double* d_params, int BW, const int* d_imin, int W,
+                                          const double* d_resid, long long resid_stride, const double* d_invC,
+                                          long long invC_stride, int mode, double* d_logL, double* d_grad,
+                                          double* d_fisher, void* stream)
+{
+    int fmt = parse_format(format);
+    if (fmt < 0)
+    {
+        set_error("Unsupported data format %s", format ? format : "(null)");
+        return GLASS_HIP_EINVAL;
+    }
+    if (!orbit || n < 0 || !d_params || !d_imin || W < 1 || !d_invC || !d_logL || !d_grad || !d_fisher ||
+        (mode != 0 && mode != 1) || resid_stride < 0 || invC_stride < 0)
+    {
+        set_error("fd_source_derivs: bad arguments");
+        return GLASS_HIP_EINVAL;
+    }
+    if (n == 0) return GLASS_HIP_OK;
+    const double* pos;
+    const double2* tw;
+    int rc;
+    if ((rc = positions(*orbit, T, t0, 4096, &pos)) || (rc = twiddles(&tw))) return rc;
+    DerivArgs A{d_params, d_imin, W, (const double2*)d_resid, resid_stride, d_invC, invC_stride, mode, T, t0,
+                orbit->L, orbit->fstar, pos, tw, fmt, d_logL, d_grad, d_fisher};
+    hipStream_t s = (hipStream_t)stream;
+    switch (BW)
+    {
+#define GH_D(NN)                                                                   \
+    case NN:                                                                       \
+        deriv_kernel<NN, fd_threads(NN)><<<n, fd_threads(NN), 0, s>>>(A);          \
+        break;
+        GH_D(32) GH_D(64) GH_D(128) GH_D(256) GH_D(512) GH_D(1024)
+#undef GH_D
+        default: set_error("fd_source_derivs: BW=%d unsupported (power of two 32..1024)", BW); return GLASS_HIP_EINVAL;
+    }
+    GH_CHECK(hipGetLastError());
+    return GLASS_HIP_OK;
+}

@@ -109,6 +109,19 @@ double glass_hip_gaussian_log_likelihood(const double *dX, const double *dY, con
                                          const double *mE, const double *invC, int NFFT,
                                          int Nchannel);
 
+/* Replaces void ucb_fisher(struct Orbit *orbit, struct Data *data,
+ *   struct Source *source, struct Noise *noise)
+ * (ucb/src/glass_ucb_waveform.h:150, ucb/src/glass_ucb_waveform.c:364-522)
+ * up to the eigensolve: the forward-difference Fisher matrix with the
+ * reference's conventions (eps = 1e-6 on the parameter array, cosine
+ * clipping, per-step ucb_alignment, Gram over the last perturbed BW with
+ * invC read from window bin 0, NaN -> 10), 3 channels.  params[8];
+ * invC 9 planes of NFFT; fisher[64] row-major.  The reference then calls
+ * matrix_eigenstuff (LAPACKE_dgeev) on the host. */
+int glass_hip_ucb_fisher(const glass_hip_orbit *orbit, const char *format, double T, double t0,
+                         int NFFT, int data_qmin, const double *params, const double *invC,
+                         double *fisher);
+
 /* Replaces void invert_noise_covariance_matrix(struct Noise *noise)
  * (utils/src/glass_math.c:240-289), Nchannel 3: C, invC 9 planes of N,
  * logdetC N (may be NULL). */

@@ -103,6 +103,27 @@ int glass_hip_fd_batch_profile_read(glass_hip_fd_batch *plan, double *ms, int *l
  * NULL, their current source parameters (nchain*8*8). Synchronous. */
 int glass_hip_fd_batch_state(glass_hip_fd_batch *plan, double *logL, double *src_params);
 
+/* Exact derivatives (no reference counterpart; replaces the forward
+ * differences of ucb_fisher, ucb/src/glass_ucb_waveform.c:364-522).
+ * For n sources with parameters d_params[n*8] and templates of BW bins
+ * (q and BW held fixed) placed at window offsets d_imin[n] of windows of W
+ * bins: with R the residual WITHOUT the source (3 x W complex at
+ * d_resid + e*resid_stride doubles; NULL = zero) and invC (6 planes x W
+ * at d_invC + e*invC_stride doubles; stride 0 = one window for all),
+ *   d_logL[e]   = L = (R|h) - (h|h)/2   (logL minus a theta-free constant)
+ *   d_grad[e*8] = dL/dtheta = (R - h | dh/dtheta)
+ *   d_fisher[e*64] = (dh/dtheta_k | dh/dtheta_l)
+ * in the parameter-array units of map_ucb_params_to_array
+ * (ucb/src/glass_ucb_model.c:40-53).  mode 0 weights bin m of the
+ * template with invC at window bin imin+m; mode 1 reproduces the
+ * reference's Fisher alignment (invC at window bin m).  BW: 32..1024. */
+int glass_hip_fd_source_derivs(const glass_hip_orbit *orbit, const char *format, double T,
+                               double t0, int n, const double *d_params, int BW,
+                               const int *d_imin, int W, const double *d_resid,
+                               long long resid_stride, const double *d_invC,
+                               long long invC_stride, int mode, double *d_logL, double *d_grad,
+                               double *d_fisher, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,174 @@
+"""GPU Fisher / gradient checks.
+
+* glass_hip_ucb_fisher (reference-compatible forward differences) against
+  the oracle's restatement of ucb_fisher.  The reference differences two
+  templates 1e-6 apart, which turns their ~1e-15 rounding into ~1e-9
+  relative noise on the matrix; parity is asserted at 1e-6.
+* glass_hip_fd_source_derivs (exact forward-mode derivatives) against
+  central finite differences of the oracle's L(theta) = (R|h) - (h|h)/2 and
+  of its templates, with q and BW held fixed, and (mode 1) against the
+  reference's own finite-difference Fisher.
+* the torch.autograd binding.
+"""
+import numpy as np
+import pytest
+
+from glass_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+T4 = synth.T_4YR
+
+
+def bform(u, v, invC):
+    """2 sum_ab Re(u_a conj v_b) invC_ab summed over bins; u,v [3,W]."""
+    s = 0.0
+    for a in range(3):
+        for b in range(3):
+            s += 2.0 * np.sum((u[a] * np.conj(v[b])).real * invC[a, b])
+    return s
+
+
+def make_sources(orc, n, seed, BW=256, W=512, T=T4):
+    rng = np.random.default_rng(seed)
+    P = synth.draw_params(rng, n, T)
+    # keep f0*T away from an integer so the carrier bin q survives the steps
+    P[:, 0] = np.floor(P[:, 0]) + rng.uniform(0.25, 0.75, n)
+    P[:, 3] = np.log(10.0 ** rng.uniform(-22.0, -21.0, n))  # loud enough to matter
+    srcs = []
+    for p in P:
+        q = int(p[0] / T * T)
+        qmin = q - W // 2
+        C = synth.xyz_noise_cov(T, qmin / T, W)
+        invC = orc.invert_noise(C)[0]
+        R = synth.noise_draw(rng, C)
+        imin = (q - BW // 2) - qmin
+        srcs.append(dict(p=p, qmin=qmin, invC=invC, R=R, imin=imin))
+    return srcs
+
+
+def oracle_h(orc, p, T, BW, W, imin):
+    h = orc.ucb_waveform(p, T, BW)[:3]
+    win = np.zeros((3, W), complex)
+    synth.scatter(win, h, imin)
+    return win
+
+
+def oracle_L(orc, p, s, T, BW, W):
+    h = oracle_h(orc, p, T, BW, W, s["imin"])
+    return bform(s["R"], h, s["invC"]) - 0.5 * bform(h, h, s["invC"])
+
+
+def run_derivs(gl, srcs, T, BW, W, mode=0, with_resid=True):
+    import torch
+    from glass_amd.batch import pack_invC
+    from glass_amd.derivs import source_derivs
+    dev = torch.device("cuda")
+    params = torch.tensor(np.stack([s["p"] for s in srcs]), device=dev)
+    imin = torch.tensor([s["imin"] for s in srcs], dtype=torch.int32, device=dev)
+    invC = torch.tensor(np.stack([pack_invC(s["invC"]) for s in srcs]), device=dev)
+    R = torch.tensor(np.stack([s["R"] for s in srcs]), device=dev) if with_resid else None
+    L, g, F = source_derivs(gl.analytic_orbit(), T, params, BW, imin, invC, R, mode=mode)
+    return L.cpu().numpy(), g.cpu().numpy(), F.cpu().numpy()
+
+
+STEPS = np.array([1e-4, 1e-6, 1e-6, 1e-6, 1e-6, 1e-6, 1e-6, 1e-5])
+
+
+def test_value_and_gradient_vs_central_fd(gl, orc):
+    BW, W = 256, 512
+    srcs = make_sources(orc, 4, 11, BW, W)
+    L, g, F = run_derivs(gl, srcs, T4, BW, W)
+    for e, s in enumerate(srcs):
+        Lref = oracle_L(orc, s["p"], s, T4, BW, W)
+        assert abs(L[e] - Lref) <= 1e-10 * max(abs(Lref), 1.0), (e, L[e], Lref)
+        gfd = np.zeros(8)
+        for k in range(8):
+            dp = np.zeros(8)
+            dp[k] = STEPS[k]
+            gfd[k] = (oracle_L(orc, s["p"] + dp, s, T4, BW, W) - oracle_L(orc, s["p"] - dp, s, T4, BW, W)) / (2 * STEPS[k])
+        # compare in the Fisher metric: dimensionless error per direction
+        sig = np.sqrt(np.diag(F[e]))
+        err = np.abs(g[e] - gfd) / sig
+        print(e, "grad err / sigma^-1:", err.max())
+        assert err.max() < 1e-4, (e, g[e], gfd)
+
+
+def test_fisher_vs_central_fd_templates(gl, orc):
+    BW, W = 256, 512
+    srcs = make_sources(orc, 3, 12, BW, W)
+    _, _, F = run_derivs(gl, srcs, T4, BW, W, with_resid=False)
+    for e, s in enumerate(srcs):
+        dh = []
+        for k in range(8):
+            dp = np.zeros(8)
+            dp[k] = STEPS[k]
+            hp = oracle_h(orc, s["p"] + dp, T4, BW, W, s["imin"])
+            hm = oracle_h(orc, s["p"] - dp, T4, BW, W, s["imin"])
+            dh.append((hp - hm) / (2 * STEPS[k]))
+        Ffd = np.array([[bform(dh[a], dh[b], s["invC"]) for b in range(8)] for a in range(8)])
+        d = np.sqrt(np.outer(np.diag(Ffd), np.diag(Ffd)))
+        err = np.max(np.abs(F[e] - Ffd) / d)
+        print(e, "fisher err (normalised):", err)
+        assert err < 1e-5
+        assert np.allclose(F[e], F[e].T, rtol=0, atol=1e-12 * np.abs(F[e]).max())
+        assert np.linalg.eigvalsh(F[e]).min() > -1e-9 * np.abs(F[e]).max()
+
+
+def test_mode1_matches_reference_fd_fisher(gl, orc):
+    """Mode 1 uses the reference's invC alignment; its exact Fisher must
+    agree with the reference's forward-difference one up to the forward
+    difference's truncation (~1e-5 relative)."""
+    W = 512
+    T = synth.YEAR_T
+    rng = np.random.default_rng(13)
+    P = synth.draw_params(rng, 3, T)
+    P[:, 0] = np.floor(P[:, 0]) + 0.5
+    for p in P:
+        q = int(p[0] / T * T)
+        qmin = q - W // 2
+        C = synth.xyz_noise_cov(T, qmin / T, W)
+        invC = orc.invert_noise(C)[0]
+        BW, _, imin = orc.ucb_alignment(p, T, W, qmin)
+        Fref = orc.ucb_fisher_fd(p, T, W, qmin, invC)
+        s = dict(p=p, qmin=qmin, invC=invC, R=np.zeros((3, W), complex), imin=imin)
+        _, _, F = run_derivs(gl, [s], T, BW, W, mode=1, with_resid=False)
+        d = np.sqrt(np.outer(np.diag(Fref), np.diag(Fref)))
+        err = np.max(np.abs(F[0] - Fref) / d)
+        print("mode-1 vs reference FD fisher:", err)
+        assert err < 1e-4
+
+
+def test_fd_fisher_dropin_matches_oracle(gl, orc):
+    from glass_amd.derivs import ucb_fisher
+    W = 256
+    cfg = synth.config0(seed=5, n=6, W=W)
+    o = gl.analytic_orbit()
+    for i in range(6):
+        p = cfg["params"][i]
+        qmin = int(cfg["qmin"][i])
+        invC = orc.invert_noise(cfg["C"][i])[0]
+        Fref = orc.ucb_fisher_fd(p, cfg["T"], W, qmin, invC)
+        F, w, v = ucb_fisher(o, "sangria", cfg["T"], 0.0, W, qmin, p, invC)
+        rel = np.linalg.norm(F - Fref) / np.linalg.norm(Fref)
+        print(i, "FD fisher rel (Frobenius):", rel)
+        assert rel < 1e-6
+        assert np.allclose(v @ np.diag(w) @ v.T, F, rtol=1e-10, atol=1e-10 * np.abs(F).max())
+
+
+def test_autograd_binding(gl, orc):
+    import torch
+    from glass_amd.batch import pack_invC
+    from glass_amd.derivs import UcbSourceLogL, source_derivs
+    BW, W = 128, 256
+    srcs = make_sources(orc, 5, 14, BW, W, T=synth.YEAR_T)
+    dev = torch.device("cuda")
+    params = torch.tensor(np.stack([s["p"] for s in srcs]), device=dev, requires_grad=True)
+    imin = torch.tensor([s["imin"] for s in srcs], dtype=torch.int32, device=dev)
+    invC = torch.tensor(np.stack([pack_invC(s["invC"]) for s in srcs]), device=dev)
+    R = torch.tensor(np.stack([s["R"] for s in srcs]), device=dev)
+    L = UcbSourceLogL().apply(params, gl.analytic_orbit(), synth.YEAR_T, BW, imin, invC, R)
+    wts = torch.arange(1, 6, dtype=torch.float64, device=dev)
+    (L * wts).sum().backward()
+    _, g, _ = source_derivs(gl.analytic_orbit(), synth.YEAR_T, params.detach(), BW, imin, invC, R)
+    assert torch.allclose(params.grad, wts[:, None] * g, rtol=0, atol=0)
