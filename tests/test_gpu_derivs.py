@@ -72,7 +72,15 @@ def run_derivs(gl, srcs, T, BW, W, mode=0, with_resid=True):
     return L.cpu().numpy(), g.cpu().numpy(), F.cpu().numpy()
 
 
-STEPS = np.array([1e-4, 1e-6, 1e-6, 1e-6, 1e-6, 1e-6, 1e-6, 1e-5])
+STEPS = np.array([1e-3, 1e-5, 1e-5, 1e-5, 1e-5, 1e-5, 1e-5, 1e-4])
+
+
+def stencil(f, x, k, h=None):
+    """Five-point central derivative along parameter k (error O(h^4))."""
+    h = STEPS[k] if h is None else h
+    e = np.zeros(8)
+    e[k] = h
+    return (-f(x + 2 * e) + 8 * f(x + e) - 8 * f(x - e) + f(x - 2 * e)) / (12 * h)
 
 
 def test_value_and_gradient_vs_central_fd(gl, orc):
@@ -82,16 +90,17 @@ def test_value_and_gradient_vs_central_fd(gl, orc):
     for e, s in enumerate(srcs):
         Lref = oracle_L(orc, s["p"], s, T4, BW, W)
         assert abs(L[e] - Lref) <= 1e-10 * max(abs(Lref), 1.0), (e, L[e], Lref)
-        gfd = np.zeros(8)
-        for k in range(8):
-            dp = np.zeros(8)
-            dp[k] = STEPS[k]
-            gfd[k] = (oracle_L(orc, s["p"] + dp, s, T4, BW, W) - oracle_L(orc, s["p"] - dp, s, T4, BW, W)) / (2 * STEPS[k])
-        # compare in the Fisher metric: dimensionless error per direction
+        # Finite differences of the waveform are limited by its own FP64
+        # phase rounding: arg2 ~ 1e7 rad carries ~1e-10 rad of noise that is
+        # not smooth in theta, so a step moving the phase by ~1e-3 rad leaves
+        # ~1e-6 of noise in the Fisher metric.  Steps STEPS (five-point
+        # stencil, truncation O(h^4)) sit near that floor.
         sig = np.sqrt(np.diag(F[e]))
+        gfd = np.array([stencil(lambda x: oracle_L(orc, x, s, T4, BW, W), s["p"], k) for k in range(8)])
+        # compare in the Fisher metric: dimensionless error per direction
         err = np.abs(g[e] - gfd) / sig
-        print(e, "grad err / sigma^-1:", err.max())
-        assert err.max() < 1e-4, (e, g[e], gfd)
+        print(e, "grad err / sigma^-1:", err)
+        assert err.max() < 2e-5, (e, g[e], gfd)
 
 
 def test_fisher_vs_central_fd_templates(gl, orc):
@@ -99,18 +108,13 @@ def test_fisher_vs_central_fd_templates(gl, orc):
     srcs = make_sources(orc, 3, 12, BW, W)
     _, _, F = run_derivs(gl, srcs, T4, BW, W, with_resid=False)
     for e, s in enumerate(srcs):
-        dh = []
-        for k in range(8):
-            dp = np.zeros(8)
-            dp[k] = STEPS[k]
-            hp = oracle_h(orc, s["p"] + dp, T4, BW, W, s["imin"])
-            hm = oracle_h(orc, s["p"] - dp, T4, BW, W, s["imin"])
-            dh.append((hp - hm) / (2 * STEPS[k]))
+        dh = [stencil(lambda x: oracle_h(orc, x, T4, BW, W, s["imin"]), s["p"], k) for k in range(8)]
         Ffd = np.array([[bform(dh[a], dh[b], s["invC"]) for b in range(8)] for a in range(8)])
         d = np.sqrt(np.outer(np.diag(Ffd), np.diag(Ffd)))
         err = np.max(np.abs(F[e] - Ffd) / d)
         print(e, "fisher err (normalised):", err)
-        assert err < 1e-5
+        # same finite-difference floor as the gradient test
+        assert err < 2e-5
         assert np.allclose(F[e], F[e].T, rtol=0, atol=1e-12 * np.abs(F[e]).max())
         assert np.linalg.eigvalsh(F[e]).min() > -1e-9 * np.abs(F[e]).max()
 
