@@ -88,6 +88,64 @@ def plain_rate(gl, synth, seconds=2.0):
     return n / (time.perf_counter() - t0)
 
 
+def fisher_leg(gl, synth, n=8192, BW=256, W=512, steps=5, seed=3):
+    """Exact-gradient Fisher evaluations (adjoint gradients case): n sources,
+    BW 256, Tobs 4 yr, each with its own 512-bin window (noise residual and
+    invC); one launch returns L, dL/dtheta and the 8x8 Fisher matrix per
+    source.  Compared with the plain per-call forward-difference ucb_fisher
+    drop-in (9 waveforms + Gram per call)."""
+    import torch
+    from glass_amd.batch import pack_invC
+    from glass_amd.derivs import source_derivs, ucb_fisher
+    T = synth.T_4YR
+    rng = np.random.default_rng(seed)
+    P = synth.draw_params(rng, n, T)
+    q = P[:, 0].astype(np.int64)
+    qmin = q - W // 2
+    imin = (q - BW // 2) - qmin
+    # windows share the PSD shape up to the band: one covariance per source,
+    # vectorised
+    f = qmin[:, None] / T + np.arange(W)[None, :] / T
+    Spm, Sop = synth.noise_levels_sangria(f)
+    x = f / synth.FSTAR
+    tr = np.sin(x) ** 2
+    diag = 16.0 * tr * (2.0 * (1.0 + np.cos(x) ** 2) * Spm + Sop) / 4.0
+    cross = -8.0 * tr * np.cos(x) * (4.0 * Spm + Sop) / 4.0
+    dev = torch.device("cuda")
+    # inverse of [[d,c,c],[c,d,c],[c,c,d]]: diag (d+c)/((d-c)(d+2c)), off -c/(...)
+    den = (diag - cross) * (diag + 2 * cross)
+    idg, ioff = (diag + cross) / den, -cross / den
+    invC = torch.tensor(np.stack([idg, idg, idg, ioff, ioff, ioff], axis=1), device=dev)
+    sd = np.sqrt(np.maximum(diag - cross, 0) / 2)
+    R = torch.tensor((rng.standard_normal((n, 3, W)) + 1j * rng.standard_normal((n, 3, W))) * sd[:, None, :],
+                     device=dev)
+    params = torch.tensor(P, device=dev)
+    im = torch.tensor(imin, dtype=torch.int32, device=dev)
+    o = gl.analytic_orbit()
+    for _ in range(2):
+        source_derivs(o, T, params, BW, im, invC, R)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        source_derivs(o, T, params, BW, im, invC, R)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    # plain: the per-call forward-difference drop-in on host arrays
+    invC33 = np.zeros((3, 3, W))
+    nplain, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < 2.0:
+        i = nplain % 64
+        for a in range(3):
+            for b in range(3):
+                invC33[a, b] = idg[i] if a == b else ioff[i]
+        ucb_fisher(o, "sangria", T, 0.0, W, int(qmin[i]), P[i], invC33, eigen=False)
+        nplain += 1
+    plain = nplain / (time.perf_counter() - t0)
+    return dict(value=n / (ms * 1e-3), ms=ms, plain=plain, n=n)
+
+
 def oracle_leg(cfg, logL_gpu, seconds, nthreads, seed):
     """CPU baseline (the oracle restatement of the reference, OpenMP over
     proposals) on a bounded sample of the same workload, and the parity of
@@ -147,6 +205,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-fisher", action="store_true")
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="also run a 256 MiB device copy, the reference point that scales the "
                          "FETCH_SIZE/WRITE_SIZE counters of a profiled run")
@@ -230,6 +289,19 @@ def main():
         plain = plain_rate(gl, synth)
         log(f"plain per-call drop-ins: {plain:.1f} evals/s")
 
+    fisher = None
+    if not args.no_fisher:
+        fl = fisher_leg(gl, synth)
+        fisher = {"value": round(fl["value"], 1), "unit": "exact-gradient Fisher evals/s",
+                  "ms_per_launch": round(fl["ms"], 4),
+                  "plain_fd_fisher": round(fl["plain"], 1),
+                  "vs_plain": round(fl["value"] / fl["plain"], 1),
+                  "config": "8192 sources, BW 256, Tobs 4 yr, own 512-bin window each; one launch -> logL, "
+                            "dlogL/dtheta[8], Fisher[8x8] per source; plain = per-call forward-difference "
+                            "ucb_fisher drop-in"}
+        log(f"exact Fisher: {fl['value']:.4e} evals/s ({fl['ms']:.3f} ms / {fl['n']}), "
+            f"plain FD ucb_fisher {fl['plain']:.1f}/s")
+
     cpu = None
     parity = None
     if not args.no_cpu_baseline:
@@ -258,6 +330,7 @@ def main():
                                              "method": "per-call ucb_waveform + gaussian_log_likelihood "
                                                        "drop-ins, host memory, BW 128, W 256"},
         "batched_vs_plain": None if plain is None else round(value / plain, 1),
+        "fisher": fisher,
         "parity": parity,
         "roofline": roofline,
         "cpu_baseline": cpu,
