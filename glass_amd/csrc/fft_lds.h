@@ -3,14 +3,17 @@
 // NL independent length-N forward transforms stored link-major in LDS
 // (buf[l*N + n]), unnormalised, exp(-2 pi i n k / N) convention, as
 // kiss_fft forward (external/kissfft/kiss_fft.c:337-369).  Stockham
-// autosort formulation, radix 4 with one radix-2 stage when log2 N is odd.
+// autosort formulation: one radix-3 stage when 3 divides N (N = 3*2^a, the
+// WDM sizes), one radix-2 stage when the power-of-two part has an odd
+// exponent, radix 4 for the rest.
 // Each stage reads every butterfly's inputs into registers, synchronises,
 // then writes the outputs into the same buffer, so a single LDS buffer
 // suffices (96*N bytes for the six link series of a template).
 //
 // Twiddles come from a table tw[k] = exp(-2 pi i k / NTW) of NTW entries
-// (NTW a power of two >= N) built once per device with the phase formed as
-// -2*pi*k/NTW, the kiss_fft twiddle formula.
+// (NTW a multiple of N: 4096 for N = 2^a, 12288 for N = 3*2^a) built once
+// per device with the phase formed as -2*pi*k/NTW, the kiss_fft twiddle
+// formula.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -56,7 +59,19 @@ __device__ inline void stockham_stage(double2* buf, int Ns, const double2* __res
 #pragma unroll
             for (int m = 1; m < R; m++) v[p][m] = cmul(v[p][m], tw[m * step]);
             double2 y[R];
-            if (R == 4)
+            if constexpr (R == 3)
+            {
+                // w = exp(-2 pi i/3): y1 = a0 - s/2 - i(sqrt3/2) d, y2 = conj rotation
+                const double h3 = 0.86602540378443864676;
+                double2 a0 = v[p][0], a1 = v[p][1], a2 = v[p][2];
+                double2 s12 = make_double2(a1.x + a2.x, a1.y + a2.y);
+                double2 d12 = make_double2(a1.x - a2.x, a1.y - a2.y);
+                double2 t = make_double2(a0.x - 0.5 * s12.x, a0.y - 0.5 * s12.y);
+                y[0] = make_double2(a0.x + s12.x, a0.y + s12.y);
+                y[1] = make_double2(t.x + h3 * d12.y, t.y - h3 * d12.x);
+                y[2] = make_double2(t.x - h3 * d12.y, t.y + h3 * d12.x);
+            }
+            else if constexpr (R == 4)
             {
                 double2 a0 = v[p][0], a1 = v[p][1], a2 = v[p][2], a3 = v[p][3];
                 double2 s02 = make_double2(a0.x + a2.x, a0.y + a2.y);
@@ -87,9 +102,16 @@ __device__ inline void stockham_stage(double2* buf, int Ns, const double2* __res
 template <int N, int NL, int NTHR>
 __device__ inline void lds_fft(double2* buf, const double2* __restrict__ tw, int twstride)
 {
-    static_assert((N & (N - 1)) == 0 && N >= 4, "power of two");
-    constexpr int LOG2 = __builtin_ctz(N);
+    constexpr bool R3 = N % 3 == 0;
+    constexpr int P2 = R3 ? N / 3 : N;
+    static_assert((P2 & (P2 - 1)) == 0 && N >= 4, "N = 2^a or 3*2^a");
+    constexpr int LOG2 = __builtin_ctz(P2);
     int Ns = 1;
+    if (R3)
+    {
+        stockham_stage<N, NL, 3, NTHR>(buf, Ns, tw, twstride);
+        Ns *= 3;
+    }
     if (LOG2 & 1)
     {
         stockham_stage<N, NL, 2, NTHR>(buf, Ns, tw, twstride);

@@ -11,6 +11,7 @@
 
 #include "reduce.h"
 #include "runtime.h"
+#include "../../include/glass_hip_wdm.h"
 
 namespace glass {
 
@@ -380,4 +381,124 @@ extern "C" int glass_hip_invert_noise_covariance(const double* C, double* invC, 
     memcpy(invC, cx->hbuf.p, 9 * pb);
     if (logdetC) memcpy(logdetC, (double*)cx->hbuf.p + 9 * (size_t)N, pb);
     return GLASS_HIP_OK;
+}
+
+// ------------------------------------------------ wavelet domain -------
+
+namespace glass {
+
+// sum_k r^T C r over n pixels: r 3 planes, C 6 planes (XX,YY,ZZ,XY,XZ,YZ);
+// per pixel XX + 2 XY as snr_wavelet / gaussian_log_likelhood_wavelet
+// group it (utils/src/glass_math.c:428-438, utils/src/glass_model.c:839-853)
+__global__ __launch_bounds__(kRedThr) void wquad_kernel(const double* __restrict__ r, const double* __restrict__ C,
+                                                        int n, int chunk, double* part)
+{
+    const int b0 = blockIdx.x * chunk, b1 = min(n, b0 + chunk);
+    double acc = 0.0;
+    for (int k = b0 + threadIdx.x; k < b1; k += kRedThr)
+    {
+        const double x = r[k], y = r[n + k], z = r[2 * n + k];
+        const double xx = x * x * C[k] + y * y * C[n + k] + z * z * C[2 * n + k];
+        const double xy = x * y * C[3 * n + k] + x * z * C[4 * n + k] + y * z * C[5 * n + k];
+        acc += xx + 2.0 * xy;
+    }
+    acc = block_sum<kRedThr>(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// Stages n pixels of 9 planes (r: 3, C: 6) already laid out in the pinned
+// buffer, reduces, returns the sum.
+static double wquad_staged(ThreadCtx* cx, int n)
+{
+    const size_t planes = sizeof(double) * 9 * (size_t)n;
+    if (hipMemcpyAsync(cx->dbuf.p, cx->hbuf.p, planes, hipMemcpyHostToDevice, cx->stream) != hipSuccess)
+    {
+        set_error("wavelet staging copy failed");
+        return NAN;
+    }
+    int chunk;
+    if (cx->dpart.ensure(sizeof(double) * kMaxPart)) return NAN;
+    const int nb = grid_for(n, &chunk);
+    const double* d = (const double*)cx->dbuf.p;
+    double* dres = (double*)((char*)cx->dbuf.p + ((planes + 255) & ~(size_t)255));
+    wquad_kernel<<<nb, kRedThr, 0, cx->stream>>>(d, d + 3 * (size_t)n, n, chunk, (double*)cx->dpart.p);
+    sum_partials<<<1, kRedThr, 0, cx->stream>>>((double*)cx->dpart.p, nb, 1.0, dres);
+    if (hipGetLastError() != hipSuccess)
+    {
+        set_error("wavelet reduction launch failed");
+        return NAN;
+    }
+    return fetch_result(cx, dres);
+}
+
+static const int kPlane6[6] = {0, 4, 8, 1, 2, 5};  // XX YY ZZ XY XZ YZ in a row-major 3x3
+
+}  // namespace glass
+
+extern "C" double glass_hip_snr_wavelet(const double* X, const double* Y, const double* Z, const double* invC, int N,
+                                        const int* list, int Nlist)
+{
+    if (N < 0 || Nlist < 0 || (Nlist > 0 && (!X || !Y || !Z || !invC || !list)))
+    {
+        set_error("snr_wavelet: bad arguments");
+        return NAN;
+    }
+    for (int t = 0; t < Nlist; t++)
+        if (list[t] < 0 || list[t] >= N)
+        {
+            set_error("snr_wavelet: list[%d]=%d outside [0,%d)", t, list[t], N);
+            return NAN;
+        }
+    if (Nlist == 0) return 0.0;
+    ThreadCtx* cx;
+    if (ctx_get(&cx)) return NAN;
+    const size_t bytes = sizeof(double) * 9 * (size_t)Nlist + 512;
+    if (cx->dbuf.ensure(bytes) || cx->hbuf.ensure(bytes)) return NAN;
+    // the listed pixels only: 3 template planes and 6 invC planes
+    double* h = (double*)cx->hbuf.p;
+    const double* ch[3] = {X, Y, Z};
+    for (int t = 0; t < Nlist; t++)
+    {
+        const int k = list[t];
+        for (int a = 0; a < 3; a++) h[a * (size_t)Nlist + t] = ch[a][k];
+        for (int p = 0; p < 6; p++) h[(3 + p) * (size_t)Nlist + t] = invC[kPlane6[p] * (size_t)N + k];
+    }
+    const double s2 = wquad_staged(cx, Nlist);
+    return sqrt(s2);
+}
+
+extern "C" double glass_hip_gaussian_log_likelihood_wavelet(const glass_hip_wdm* w, const double* dX,
+                                                            const double* dY, const double* dZ, const double* mX,
+                                                            const double* mY, const double* mZ, const int* mlist,
+                                                            int Nlist, const double* invC, int N)
+{
+    if (!w || N <= 0 || Nlist < 0 || !dX || !dY || !dZ || !invC || (Nlist > 0 && (!mX || !mY || !mZ || !mlist)) ||
+        w->NT <= 0)
+    {
+        set_error("gaussian_log_likelihood_wavelet: bad arguments");
+        return NAN;
+    }
+    ThreadCtx* cx;
+    if (ctx_get(&cx)) return NAN;
+    const size_t bytes = sizeof(double) * 9 * (size_t)N + 512;
+    if (cx->dbuf.ensure(bytes) || cx->hbuf.ensure(bytes)) return NAN;
+    // residual with the reference's list rule, formed in the staging buffer
+    double* r = (double*)cx->hbuf.p;
+    memcpy(r, dX, sizeof(double) * N);
+    memcpy(r + N, dY, sizeof(double) * N);
+    memcpy(r + 2 * (size_t)N, dZ, sizeof(double) * N);
+    for (int t = 0; t < Nlist; t++)
+    {
+        const int k = mlist[t];
+        if (k >= 0 && k < N)
+        {
+            const int i = (k + w->kmin) % w->NT;
+            r[k] -= mX[k];
+            r[N + k] -= mY[k];
+            r[2 * (size_t)N + k] -= mZ[k];
+            if (i < w->imin || i > w->imax) r[k] = r[N + k] = r[2 * (size_t)N + k] = 0.0;
+        }
+    }
+    for (int p = 0; p < 6; p++) memcpy(r + (3 + p) * (size_t)N, invC + kPlane6[p] * (size_t)N, sizeof(double) * N);
+    return -0.5 * wquad_staged(cx, N);
 }

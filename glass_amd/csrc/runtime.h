@@ -60,6 +60,9 @@ int ctx_get(ThreadCtx** out);
 // FFT twiddles tw[k] = exp(-2 pi i k / kTwN), k < kTwN, on the current device.
 constexpr int kTwN = 4096;
 int twiddles(const double2** out);
+// Same for the sizes N = 3*2^a: tw3[k] = exp(-2 pi i k / kTw3N).
+constexpr int kTw3N = 12288;
+int twiddles3(const double2** out);
 
 // Spacecraft positions of the analytic orbit at the npos sample times
 // t0 + T*n/npos (npos a power of two), 9 doubles per sample

@@ -92,36 +92,39 @@ int ctx_get(ThreadCtx** out)
 }
 
 static std::mutex g_tab_mu;
-static std::map<int, double2*> g_tw;
+static std::map<int, double2*> g_tw, g_tw3;
 typedef std::tuple<int, double, double, double, double, double, int> PosKey;
 static std::map<PosKey, double*> g_pos;
 
-int twiddles(const double2** out)
+static int twiddle_table(std::map<int, double2*>& cache, int ntw, const double2** out)
 {
     int dev = 0;
     GH_CHECK(hipGetDevice(&dev));
     std::lock_guard<std::mutex> lk(g_tab_mu);
-    auto it = g_tw.find(dev);
-    if (it != g_tw.end())
+    auto it = cache.find(dev);
+    if (it != cache.end())
     {
         *out = it->second;
         return GLASS_HIP_OK;
     }
     // kiss_fft twiddle formula (external/kissfft/kiss_fft.c:356-363)
-    std::vector<double2> h(kTwN);
+    std::vector<double2> h(ntw);
     const double pi = 3.141592653589793238462643383279502884197169399375105820974944;
-    for (int k = 0; k < kTwN; k++)
+    for (int k = 0; k < ntw; k++)
     {
-        double ph = -2 * pi * k / kTwN;
+        double ph = -2 * pi * k / ntw;
         h[k] = make_double2(cos(ph), sin(ph));
     }
     double2* d = nullptr;
-    GH_CHECK(hipMalloc(&d, sizeof(double2) * kTwN));
-    GH_CHECK(hipMemcpy(d, h.data(), sizeof(double2) * kTwN, hipMemcpyHostToDevice));
-    g_tw[dev] = d;
+    GH_CHECK(hipMalloc(&d, sizeof(double2) * ntw));
+    GH_CHECK(hipMemcpy(d, h.data(), sizeof(double2) * ntw, hipMemcpyHostToDevice));
+    cache[dev] = d;
     *out = d;
     return GLASS_HIP_OK;
 }
+
+int twiddles(const double2** out) { return twiddle_table(g_tw, kTwN, out); }
+int twiddles3(const double2** out) { return twiddle_table(g_tw3, kTw3N, out); }
 
 void analytic_positions(const glass_hip_orbit& o, double t, double* p)
 {

@@ -1,0 +1,1136 @@
+// wdm_ucb.hip -- galactic-binary templates in the WDM wavelet domain on
+// gfx950: ucb_waveform_wavelet (ucb/src/glass_ucb_waveform.c:1061-1190),
+// wavelet_nwip-based inner products, snr_wavelet and the wavelet
+// likelihood, as drop-ins and as a batched device API.
+//
+// Layout of the computation (one template per workgroup, two kernels):
+//  K1 resp_kernel   : on the Norb orbit knots -- barycentre amplitude and
+//                     phase splines, phase at spacecraft 1, the 24 delayed
+//                     one-arm terms of X,Y,Z (LISA_spline_response), their
+//                     amplitude/phase (extract_amplitude_and_phase,
+//                     unwrap_phase) and the six TDI splines.  The serial
+//                     parts (tridiagonal sweeps, sign-flip scan, unwrap)
+//                     run one spline / channel per lane, side by side.
+//  K2 layer_kernel  : the heterodyned series at NT*(L+1) samples (LDS),
+//                     a packed real FFT (X+iY in one complex FFT, Z as
+//                     half-length), per layer the Meyer-windowed band and
+//                     an inverse FFT of NT, the pixel list, then the fused
+//                     inner products against data and invC.
+// Everything that depends only on the orbit and the time grids (knots,
+// spline x-terms, spacecraft geometry at the knots, spacecraft 1 at the
+// downsampled times, interval indices) is tabulated once on the host with
+// the reference's operation order and lives on the device.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "../../include/glass_hip_wdm.h"
+#include "fft_lds.h"
+#include "reduce.h"
+#include "runtime.h"
+#include "wdm_common.h"
+
+namespace glass {
+namespace wdm {
+
+// ------------------------------------------------------------ TDI table --
+
+__constant__ TdiTerm c_tdi[8] = {
+    {0, 2, 1, +1, 0, 1, -1, 1, 2, 2.0, 2.0},
+    {1, 2, 1, -1, 0, 2, +1, 1, 2, 1.0, 2.0},
+    {2, 1, 2, +1, 1, 1, -1, 0, 1, 1.0, 2.0},
+    {0, 1, 0, -1, 0, 1, +1, 0, 2, 2.0, 0.0},
+    {0, 2, 0, +1, 1, 2, -1, 1, 1, 2.0, 0.0},
+    {2, 1, 0, -1, 1, 1, +1, 0, 1, 1.0, 0.0},
+    {1, 2, 0, +1, 0, 2, -1, 1, 2, 1.0, 0.0},
+    {0, 0, 0, -1, 0, 2, +1, 1, 1, 0.0, 0.0},
+};
+
+// ----------------------------------------------------- host: splines ----
+
+// Natural cubic spline coefficients c = y0|y1|y2|y3 (N each), the
+// recurrence of spline_coefficients (utils/src/glass_math.c:100-175).
+static void spline_host(int N, const double* x, const double* y, double* c)
+{
+#pragma clang fp contract(off)
+    std::vector<double> dx(N, 0.0), alpha(N, 0.0), l(N, 0.0), z(N, 0.0), mu(N, 0.0);
+    double *y0 = c, *y1 = c + N, *y2 = c + 2 * N, *y3 = c + 3 * N;
+    dx[0] = x[1] - x[0];
+    for (int i = 1; i < N - 1; i++)
+    {
+        dx[i] = x[i + 1] - x[i];
+        alpha[i] = 3.0 * ((y[i + 1] - y[i]) / dx[i] - (y[i] - y[i - 1]) / dx[i - 1]);
+    }
+    l[0] = l[N - 1] = 1.0;
+    for (int i = 1; i < N - 1; i++)
+    {
+        l[i] = 2.0 * (x[i + 1] - x[i - 1]) - dx[i - 1] * mu[i - 1];
+        mu[i] = dx[i] / l[i];
+        z[i] = (alpha[i] - dx[i - 1] * z[i - 1]) / l[i];
+    }
+    y0[N - 1] = y[N - 1];
+    y1[N - 1] = y2[N - 1] = y3[N - 1] = 0.0;
+    for (int i = N - 2; i >= 0; i--)
+    {
+        y0[i] = y[i];
+        y2[i] = z[i] - mu[i] * y2[i + 1];
+        y1[i] = (y0[i + 1] - y0[i]) / dx[i] - dx[i] * (y2[i + 1] + 2.0 * y2[i]) / 3.0;
+        y3[i] = (y2[i + 1] - y2[i]) / (3.0 * dx[i]);
+    }
+}
+
+// x-only terms of the recurrence: dx | l | mu (N each)
+static void spline_prep_host(int N, const double* x, double* p)
+{
+#pragma clang fp contract(off)
+    double *dx = p, *l = p + N, *mu = p + 2 * N;
+    for (int i = 0; i < 3 * N; i++) p[i] = 0.0;
+    dx[0] = x[1] - x[0];
+    for (int i = 1; i < N - 1; i++) dx[i] = x[i + 1] - x[i];
+    l[0] = l[N - 1] = 1.0;
+    for (int i = 1; i < N - 1; i++)
+    {
+        l[i] = 2.0 * (x[i + 1] - x[i - 1]) - dx[i - 1] * mu[i - 1];
+        mu[i] = dx[i] / l[i];
+    }
+}
+
+// ------------------------------------------------------ host: geometry --
+
+static double incomplete_beta(double a, double b, double x)
+{
+#pragma clang fp contract(off)
+    // incomplete_beta_function (utils/src/glass_math.c:1214-1242): the
+    // Gauss series 2F1(a, 1-b; a+1; x) to |term| <= 1e-8
+    double aa = a, bb = 1 - b, cc = a + 1;
+    double term = aa * bb * x / cc;
+    double value = 1.0 + term;
+    int n = 1;
+    while (fabs(term) > 1e-8)
+    {
+        aa++, bb++, cc++, n++;
+        term *= aa * bb * x / cc / n;
+        value += term;
+    }
+    double B = exp(lgamma(a) + lgamma(b) - lgamma(a + b));
+    return pow(x, a) * value / B / a;
+}
+
+// build_wdm_filter_freq (utils/src/glass_wavelet.c:305-349), forward
+void wdm_window(int Nf, int Nt, double A, double* phif)
+{
+#pragma clang fp contract(off)
+    const double DeltaOmega = M_PI / Nf;
+    const double domega = 2 * DeltaOmega / (double)Nt;
+    for (int i = 0; i < Nt / 2 + 1; i++)
+    {
+        double omega = domega * i;
+        double B = DeltaOmega - 2 * A;
+        double insDOM = 1.0 / sqrt(DeltaOmega);
+        double z = 0.0;
+        if (fabs(omega) >= A && fabs(omega) < A + B)
+        {
+            double x = (fabs(omega) - A) / B;
+            double y = incomplete_beta(6.0, 6.0, x);
+            z = insDOM * cos(y * M_PI_2);
+        }
+        if (fabs(omega) < A) z = insDOM;
+        phif[i] = z;
+    }
+    double norm = 0.0;
+    for (int i = 0; i < Nt / 2 + 1; i++)
+    {
+        if (i == 0) norm += phif[i] * phif[i];
+        else norm += 2 * phif[i] * phif[i];
+    }
+    norm *= domega * M_1_PI;
+    norm = sqrt(norm);
+    norm *= Nf / 2.0;
+    for (int i = 0; i < Nt / 2 + 1; i++) phif[i] /= norm;
+}
+
+// Device tables of one (wdm grid, orbit knots, Tobs, t0) combination.
+struct Geo
+{
+    int Norb, NT, NF, kmin, kmax, imin, imax;
+    double Tobs, t0, dt, df, A, ot0, ods;
+    const double* ot;      // Norb orbit knots
+    const double* oc;      // 9 x 4 x Norb position splines (row 3*sc+coord)
+    const double* prep_o;  // 3 x Norb  dx | l | mu on the orbit knots
+    const double* tssb;    // Norb response times t0 + i Tobs/(Norb-1)
+    const double* prep_s;  // 3 x Norb  on the response times
+    const double* resp;    // Norb x 24: x[3][3]/c, n[3][3], L[3] at tssb
+    const double* ds[kMaxLayers + 1];   // per L: N_ds x 5: t, r0x, r0y, r0z, dx
+    const int* dsi[kMaxLayers + 1];     // per L: response-time interval
+    const double* win[kMaxLayers + 1];  // per L: window of L+1 layers
+    const double2* tw;     // 4096-entry twiddles
+    const double2* tw3;    // 12288-entry twiddles
+};
+
+struct GeoHost
+{
+    Geo g;
+    std::vector<void*> bufs;
+    ~GeoHost()
+    {
+        for (void* p : bufs) (void)hipFree(p);
+    }
+};
+
+template <typename T>
+static int upload(GeoHost& G, const std::vector<T>& v, const T** out)
+{
+    void* d = nullptr;
+    GH_CHECK(hipMalloc(&d, sizeof(T) * std::max<size_t>(v.size(), 1)));
+    G.bufs.push_back(d);
+    GH_CHECK(hipMemcpy(d, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+    *out = (const T*)d;
+    return GLASS_HIP_OK;
+}
+
+static double spline_host_at(const std::vector<double>& x, const double* c, int N, double t)
+{
+#pragma clang fp contract(off)
+    int n = knot_index(x.data(), N, t);
+    double dx = t - x[n];
+    return c[n] + c[N + n] * dx + c[2 * N + n] * dx * dx + c[3 * N + n] * dx * dx * dx;
+}
+
+// Fills the tables.  pos: 9 x Norb positions (row 3*sc + coord).
+static int geo_build(GeoHost& G, const glass_hip_wdm& w, int Norb, const double* ot_in,
+                     const double* pos, double Tobs, double t0)
+{
+#pragma clang fp contract(off)
+    Geo& g = G.g;
+    const int N = Norb;
+    g.Norb = N;
+    g.NT = w.NT;
+    g.NF = w.NF;
+    g.kmin = w.kmin;
+    g.kmax = w.kmax;
+    g.imin = w.imin;
+    g.imax = w.imax;
+    g.Tobs = Tobs;
+    g.t0 = t0;
+    g.dt = w.dt;
+    g.df = w.df;
+    g.A = w.A;
+    std::vector<double> ot(ot_in, ot_in + N);
+    g.ot0 = ot[0];
+    g.ods = ot[1] - ot[0];
+    std::vector<double> oc(9 * 4 * N), prep_o(3 * N), tssb(N), prep_s(3 * N), resp(24 * N, 0.0);
+    for (int s = 0; s < 9; s++) spline_host(N, ot.data(), pos + s * N, oc.data() + 4 * s * N);
+    spline_prep_host(N, ot.data(), prep_o.data());
+    const double dts = Tobs / (double)(N - 1);
+    for (int i = 0; i < N; i++) tssb[i] = t0 + i * dts;
+    spline_prep_host(N, tssb.data(), prep_s.data());
+    if (tssb[0] < ot[0] || tssb[N - 1] > ot[N - 1])
+    {
+        set_error("wavelet waveform: response times [%g, %g] outside the orbit knots [%g, %g]",
+                  tssb[0], tssb[N - 1], ot[0], ot[N - 1]);
+        return GLASS_HIP_EINVAL;
+    }
+    // spacecraft geometry at the response times (LISA_spline_response,
+    // utils/src/glass_lisa.c:976-1016)
+    for (int m = 0; m < N; m++)
+    {
+        double x[3][3], n[3][3], L[3];
+        for (int i = 0; i < 3; i++)
+            for (int d = 0; d < 3; d++)
+                x[i][d] = spline_host_at(ot, oc.data() + 4 * (3 * i + d) * N, N, tssb[m]) / CLIGHT;
+        for (int i = 0; i < 3; i++)
+        {
+            n[0][i] = x[1][i] - x[2][i];
+            n[1][i] = x[2][i] - x[0][i];
+            n[2][i] = x[0][i] - x[1][i];
+        }
+        for (int i = 0; i < 3; i++)
+        {
+            L[i] = 0.0;
+            for (int j = 0; j < 3; j++) L[i] += n[i][j] * n[i][j];
+            L[i] = sqrt(L[i]);
+        }
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) n[i][j] /= L[i];
+        double* r = resp.data() + 24 * m;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++)
+            {
+                r[3 * i + j] = x[i][j];
+                r[9 + 3 * i + j] = n[i][j];
+            }
+        for (int i = 0; i < 3; i++) r[18 + i] = L[i];
+    }
+    int rc;
+    if ((rc = upload(G, ot, &g.ot)) || (rc = upload(G, oc, &g.oc)) || (rc = upload(G, prep_o, &g.prep_o)) ||
+        (rc = upload(G, tssb, &g.tssb)) || (rc = upload(G, prep_s, &g.prep_s)) ||
+        (rc = upload(G, resp, &g.resp)))
+        return rc;
+    // per layer count L: the downsampled times, spacecraft 1 there, the
+    // response-time interval and offset of each sample, and the window
+    for (int L = 1; L <= kMaxLayers; L++)
+    {
+        const int Nds = w.NT * (L + 1);
+        const double dt_ds = w.dt / (double)(L + 1);
+        std::vector<double> ds(5 * (size_t)Nds);
+        std::vector<int> dsi(Nds);
+        for (int i = 0; i < Nds; i++)
+        {
+            double t = t0 + i * dt_ds;
+            if (t < tssb[0] || t > tssb[N - 1] || t < ot[0] || t > ot[N - 1])
+            {
+                set_error("wavelet waveform: sample time %g of the %d-layer grid (NT*dt = %g) lies "
+                          "outside [t0, t0+Tobs] = [%g, %g]",
+                          t, L, w.NT * w.dt, tssb[0], tssb[N - 1]);
+                return GLASS_HIP_EINVAL;
+            }
+            double* e = ds.data() + 5 * (size_t)i;
+            e[0] = t;
+            for (int d = 0; d < 3; d++) e[1 + d] = spline_host_at(ot, oc.data() + 4 * d * N, N, t) / CLIGHT;
+            int k = knot_index(tssb.data(), N, t);
+            dsi[i] = k;
+            e[4] = t - tssb[k];
+        }
+        std::vector<double> win(w.NT / 2 + 1);
+        wdm_window(L + 1, w.NT, w.A, win.data());
+        if ((rc = upload(G, ds, &g.ds[L])) || (rc = upload(G, dsi, &g.dsi[L])) ||
+            (rc = upload(G, win, &g.win[L])))
+            return rc;
+    }
+    g.ds[0] = nullptr;
+    g.dsi[0] = nullptr;
+    g.win[0] = nullptr;
+    if ((rc = twiddles(&g.tw)) || (rc = twiddles3(&g.tw3))) return rc;
+    return GLASS_HIP_OK;
+}
+
+// Validates the grid for the template kernels.
+static int check_grid(const glass_hip_wdm* w, int Norb)
+{
+    if (!w) return set_error("NULL wdm"), GLASS_HIP_EINVAL;
+    if (w->NT != 256 && w->NT != 512)
+    {
+        set_error("wavelet templates: NT = %d unsupported (this build computes templates for NT 256 or "
+                  "512; larger NT needs a template FFT beyond one workgroup's LDS)",
+                  w->NT);
+        return GLASS_HIP_EINVAL;
+    }
+    if (w->NF < 4 || w->dt <= 0 || w->df <= 0)
+        return set_error("wavelet templates: bad grid NF=%d dt=%g df=%g", w->NF, w->dt, w->df), GLASS_HIP_EINVAL;
+    if (Norb < 4 || Norb > kMaxKnots)
+        return set_error("wavelet templates: %d orbit knots (supported 4..%d)", Norb, kMaxKnots),
+               GLASS_HIP_EINVAL;
+    return GLASS_HIP_OK;
+}
+
+// ------------------------------------------------------- K1: response ---
+
+// Per-template scratch (global): header, barycentre phase spline (4N),
+// TDI splines (6 x 4N: amp X, phase X, amp Y, phase Y, amp Z, phase Z).
+__host__ __device__ inline size_t scratch_doubles(int N) { return 64 + 28 * (size_t)N; }
+
+constexpr int kRespThreads = 128;
+
+// LDS of K1 in doubles: ssb 8N | M,Mf 6N | As 3N | flip,pjump 6N | phase_sc N |
+// z 6N | y2 6N
+__host__ __device__ inline size_t resp_lds_bytes(int N) { return sizeof(double) * 36 * (size_t)N + sizeof(Hdr); }
+
+// One lane's tridiagonal sweep of the natural spline through y (values at
+// the knots described by prep): writes z and y2 (N each).
+__device__ inline void spline_sweep(int N, const double* __restrict__ prep, const double* y, double* z, double* y2)
+{
+#pragma clang fp contract(off)
+    const double *dx = prep, *l = prep + N, *mu = prep + 2 * N;
+    z[0] = 0.0;
+    double zp = 0.0;
+    for (int i = 1; i < N - 1; i++)
+    {
+        const double alpha = 3.0 * ((y[i + 1] - y[i]) / dx[i] - (y[i] - y[i - 1]) / dx[i - 1]);
+        zp = (alpha - dx[i - 1] * zp) / l[i];
+        z[i] = zp;
+    }
+    z[N - 1] = 0.0;
+    double yn = 0.0;
+    y2[N - 1] = 0.0;
+    for (int i = N - 2; i >= 0; i--)
+    {
+        yn = z[i] - mu[i] * yn;
+        y2[i] = yn;
+    }
+}
+
+// y1, y3 of knot i from y0 and y2
+__device__ inline void spline_finish(int N, int i, const double* __restrict__ prep, const double* y0,
+                                     const double* y2, double* y1o, double* y3o)
+{
+#pragma clang fp contract(off)
+    if (i == N - 1)
+    {
+        *y1o = 0.0;
+        *y3o = 0.0;
+        return;
+    }
+    const double dx = prep[i];
+    *y1o = (y0[i + 1] - y0[i]) / dx - dx * (y2[i + 1] + 2.0 * y2[i]) / 3.0;
+    *y3o = (y2[i + 1] - y2[i]) / (3.0 * dx);
+}
+
+__global__ void __launch_bounds__(kRespThreads) resp_kernel(Geo g, int n, const double* __restrict__ params,
+                                                            double* __restrict__ scratch)
+{
+#pragma clang fp contract(off)
+    const int e = blockIdx.x;
+    if (e >= n) return;
+    extern __shared__ double lds[];
+    const int N = g.Norb;
+    const int tid = threadIdx.x;
+    double* ssb = lds;               // amp y0..y3 | phase y0..y3
+    double* Mx = ssb + 8 * N;        // M X,Y,Z | Mf X,Y,Z
+    double* As = Mx + 6 * N;         // |M| X,Y,Z
+    double* fl = As + 3 * N;         // flip X,Y,Z | pjump X,Y,Z
+    double* phsc = fl + 6 * N;       // phase at spacecraft 1
+    double* zz = phsc + N;           // 6N sweep scratch
+    double* y2s = zz + 6 * N;        // 6N second-derivative coefficients
+    Hdr* H = (Hdr*)(y2s + 6 * N);
+    double* scr = scratch + scratch_doubles(N) * (size_t)e;
+
+    if (tid == 0)
+    {
+        make_hdr(params + 8 * (size_t)e, g.Tobs, g.df, g.A, H);
+        *(Hdr*)scr = *H;
+    }
+    __syncthreads();
+    const Hdr h = *H;
+    // barycentre amplitude and phase at the orbit knots
+    for (int i = tid; i < N; i += kRespThreads)
+    {
+        const double t = g.ot[i];
+        ssb[i] = ucb_amplitude(t, h.A0, h.f0, h.fdot);
+        ssb[4 * N + i] = ucb_phase(t, h.f0, h.phi0, h.fdot);
+    }
+    __syncthreads();
+    if (tid < 2) spline_sweep(N, g.prep_o, ssb + 4 * N * tid, zz + N * tid, ssb + 4 * N * tid + 2 * N);
+    __syncthreads();
+    for (int i = tid; i < 2 * N; i += kRespThreads)
+    {
+        const int s = i / N, k = i % N;
+        double* c = ssb + 4 * N * s;
+        spline_finish(N, k, g.prep_o, c, c + 2 * N, c + N + k, c + 3 * N + k);
+    }
+    __syncthreads();
+    const double* cph = ssb + 4 * N;
+    // export the barycentre phase spline for the layer kernel
+    for (int i = tid; i < 4 * N; i += kRespThreads) scr[64 + i] = cph[i];
+    // phase at spacecraft 1 (LISA_spacecraft_to_barycenter_time, flag -1)
+    for (int m = tid; m < N; m += kRespThreads)
+    {
+        const double* r = g.resp + 24 * m;
+        double kdotr = 0.0;
+        for (int i = 0; i < 3; i++) kdotr += h.k[i] * r[i];
+        const double tsc = g.tssb[m] - kdotr;
+        const int j = even_index(tsc, g.ot0, g.ods, N);
+        phsc[m] = spline_at(cph, N, j, tsc - g.ot[j]);
+    }
+    // the X, Y, Z responses: items (sample m, channel ch)
+    for (int it = tid; it < 3 * N; it += kRespThreads)
+    {
+        const int m = it / 3, ch = it % 3;
+        const double* r = g.resp + 24 * m;
+        const double* x = r;          // x[i][j] at 3i+j
+        const double* nn = r + 9;
+        const double* L = r + 18;
+        double kr[3], kn[3], App[3], Apm[3], Acp[3], Acm[3];
+        for (int i = 0; i < 3; i++)
+        {
+            kr[i] = 0.0;
+            for (int j = 0; j < 3; j++) kr[i] += h.k[j] * x[3 * i + j];
+        }
+        for (int i = 0; i < 3; i++)
+        {
+            kn[i] = 0.0;
+            for (int j = 0; j < 3; j++) kn[i] += h.k[j] * nn[3 * i + j];
+        }
+        for (int i = 0; i < 3; i++)
+        {
+            double dp = 0.0, dc = 0.0;
+            for (int j = 0; j < 3; j++)
+                for (int l = 0; l < 3; l++)
+                {
+                    dp += (nn[3 * i + j] * nn[3 * i + l]) * h.ep[3 * j + l];
+                    dc += (nn[3 * i + j] * nn[3 * i + l]) * h.ec[3 * j + l];
+                }
+            App[i] = dp / (1.0 + kn[i]);
+            Apm[i] = dp / (1.0 - kn[i]);
+            Acp[i] = dc / (1.0 + kn[i]);
+            Acm[i] = dc / (1.0 - kn[i]);
+        }
+        const int vtx[3] = {ch, (ch + 1) % 3, (ch + 2) % 3};
+        const double t = g.tssb[m];
+        double M = 0.0, Mf = 0.0;
+#pragma unroll 1
+        for (int q = 0; q < 8; q++)
+        {
+            const TdiTerm T = c_tdi[q];
+            const double te = t - kr[vtx[T.v]] - T.m1 * L[vtx[T.i1]] - T.m2 * L[vtx[T.i2]];
+            const int j = even_index(te, g.ot0, g.ods, N);
+            const double dx = te - g.ot[j];
+            const double a = spline_at(ssb, N, j, dx);
+            const double ph = spline_at(cph, N, j, dx);
+            // correctly rounded: the channel sums cancel by (f/f*)^2, which
+            // would magnify one-ulp differences of sin/cos
+            double sp, cp;
+            sincos_cr(ph, &sp, &cp);
+            const double hp = a * (h.Ap * h.c2p * cp + h.Ac * h.s2p * sp);
+            const double hc = a * (h.Ac * h.c2p * sp - h.Ap * h.s2p * cp);
+            const double hpf = a * (-h.Ap * h.c2p * sp + h.Ac * h.s2p * cp);
+            const double hcf = a * (h.Ac * h.c2p * cp + h.Ap * h.s2p * sp);
+            const int j1 = vtx[T.j1], j2 = vtx[T.j2];
+            const double P1 = T.pm1 ? App[j1] : Apm[j1], Q1 = T.pm1 ? Acp[j1] : Acm[j1];
+            const double P2 = T.pm2 ? App[j2] : Apm[j2], Q2 = T.pm2 ? Acp[j2] : Acm[j2];
+            double v;
+            v = hp * P1 + hc * Q1;
+            M = T.s1 > 0 ? M + v : M - v;
+            v = hp * P2 + hc * Q2;
+            M = T.s2 > 0 ? M + v : M - v;
+            v = hpf * P1 + hcf * Q1;
+            Mf = T.s1 > 0 ? Mf + v : Mf - v;
+            v = hpf * P2 + hcf * Q2;
+            Mf = T.s2 > 0 ? Mf + v : Mf - v;
+        }
+        Mx[ch * N + m] = M;
+        Mx[(3 + ch) * N + m] = Mf;
+        As[ch * N + m] = sqrt(M * M + Mf * Mf);
+    }
+    __syncthreads();
+    // sign flips of the amplitude (extract_amplitude_and_phase,
+    // utils/src/glass_math.c:1244-1296), one channel per lane
+    if (tid < 3)
+    {
+        const double* A = As + tid * N;
+        double* flip = fl + tid * N;
+        double* pj = fl + (3 + tid) * N;
+        flip[0] = 1.0;
+        pj[0] = 0.0;
+        int i = 1;
+        do
+        {
+            flip[i] = flip[i - 1];
+            pj[i] = pj[i - 1];
+            if (A[i] < A[i - 1] && A[i] < A[i + 1])
+            {
+                const double d1 = A[i + 1] + A[i - 1] - 2.0 * A[i];
+                const double d2 = -A[i + 1] + A[i - 1] - 2.0 * A[i];
+                const double d3 = -A[i + 1] + A[i - 1] + 2.0 * A[i];
+                if (fabs(d2 / d1) < 0.1)
+                {
+                    flip[i + 1] = -1.0 * flip[i];
+                    pj[i + 1] = pj[i] + M_PI;
+                    i++;
+                }
+                if (fabs(d3 / d1) < 0.1)
+                {
+                    flip[i] = -1.0 * flip[i - 1];
+                    pj[i] = pj[i - 1] + M_PI;
+                }
+            }
+            i++;
+        } while (i < N - 1);
+        flip[N - 1] = flip[N - 2];
+        pj[N - 1] = pj[N - 2];
+    }
+    __syncthreads();
+    // signed amplitude and phase relative to the spacecraft-1 carrier
+    for (int it = tid; it < 3 * N; it += kRespThreads)
+    {
+        const int ch = it / N, i = it % N;
+        const double M = Mx[ch * N + i], Mf = Mx[(3 + ch) * N + i];
+        const double v = remainder(phsc[i], PI2);
+        Mx[ch * N + i] = fl[ch * N + i] * As[ch * N + i];
+        Mx[(3 + ch) * N + i] = -atan2_cr(Mf, M) + fl[(3 + ch) * N + i] - v;
+    }
+    __syncthreads();
+    // unwrap_phase (utils/src/glass_math.c:1034-1052)
+    if (tid < 3)
+    {
+        double* ph = Mx + (3 + tid) * N;
+        double v = ph[0];
+        for (int i = 0; i < N; i++)
+        {
+            double u = ph[i];
+            const double q = rint(fabs(u - v) / PI2);
+            if (q > 0.0)
+            {
+                if (v > u) u += q * PI2;
+                else u -= q * PI2;
+            }
+            v = u;
+            ph[i] = u;
+        }
+    }
+    __syncthreads();
+    // six TDI splines on the response times: spline s = 2*ch + (0 amp, 1 phase)
+    if (tid < 6)
+    {
+        const int ch = tid >> 1, ph = tid & 1;
+        spline_sweep(N, g.prep_s, Mx + (3 * ph + ch) * N, zz + tid * N, y2s + tid * N);
+    }
+    __syncthreads();
+    for (int it = tid; it < 6 * N; it += kRespThreads)
+    {
+        const int s = it / N, i = it % N;
+        const int ch = s >> 1, ph = s & 1;
+        const double* y0 = Mx + (3 * ph + ch) * N;
+        const double* y2 = y2s + s * N;
+        double y1, y3;
+        spline_finish(N, i, g.prep_s, y0, y2, &y1, &y3);
+        double* c = scr + 64 + 4 * N + 4 * (size_t)N * s;
+        c[i] = y0[i];
+        c[N + i] = y1;
+        c[2 * N + i] = y2[i];
+        c[3 * N + i] = y3;
+    }
+}
+
+// ---------------------------------------------------------- K2: layers --
+
+constexpr int kLayerThreads = 256;
+
+struct LayerArgs
+{
+    const double* data;   // 3 planes of npix (NULL: no inner products)
+    const double* invC;   // 6 planes XX,YY,ZZ,XY,XZ,YZ of npix
+    int npix;
+    int* nwave;           // [n]
+    int* list;            // [n x 3 NT] (compact pixel list)
+    double* values;       // [n x 3 x 3 NT]
+    double* dlogL;        // [n] (with data)
+    double* snr;          // [n] (with data)
+};
+
+// X, Y (packed in U) or Z (half-length V) spectrum at bin k, 0 < k < Nds/2
+template <int NDS>
+__device__ inline double2 spectrum(int ch, int k, const double2* U, const double2* V, const double2* tw,
+                                   int twstride)
+{
+    if (ch < 2)
+    {
+        const double2 a = U[k], b = U[(NDS - k) % NDS];
+        // X = (U[k] + conj U[N-k]) / 2 ; Y = (U[k] - conj U[N-k]) / (2i)
+        if (ch == 0) return make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
+        return make_double2(0.5 * (a.y + b.y), -0.5 * (a.x - b.x));
+    }
+    constexpr int M = NDS / 2;
+    const double2 a = V[k % M], b = V[(M - k % M) % M];
+    const double2 E = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
+    const double2 O = make_double2(0.5 * (a.y + b.y), -0.5 * (a.x - b.x));
+    const double2 w = tw[k * twstride];  // exp(-2 pi i k / NDS)
+    return make_double2(E.x + (w.x * O.x - w.y * O.y), E.y + (w.x * O.y + w.y * O.x));
+}
+
+template <int NT, int L>
+__device__ void layer_body(const Geo& g, const Hdr& h, int e, const LayerArgs& a, double2* U, double2* V,
+                           double2* B, int* cnt, const double* __restrict__ scr)
+{
+    constexpr int NDS = NT * (L + 1);
+    constexpr int M = NDS / 2;
+    const int tid = threadIdx.x;
+    const int N = g.Norb;
+    const double* cph = scr + 64;
+    const double* ctdi = scr + 64 + 4 * N;
+    const double* ds = g.ds[L];
+    const int* dsi = g.dsi[L];
+    {
+#pragma clang fp contract(off)
+        const double w0 = PI2 * h.fhet;
+        for (int i = tid; i < NDS; i += kLayerThreads)
+        {
+            const double* e5 = ds + 5 * (size_t)i;
+            const double t = e5[0];
+            double kdotr = 0.0;
+            for (int d = 0; d < 3; d++) kdotr += h.k[d] * e5[1 + d];
+            const double tsc = t - kdotr;
+            const int j = even_index(tsc, g.ot0, g.ods, N);
+            const double phase_ds = spline_at(cph, N, j, tsc - g.ot[j]);
+            const double phase_het = w0 * t;
+            const int si = dsi[i];
+            const double sdx = e5[4];
+            double x[3];
+            for (int ch = 0; ch < 3; ch++)
+            {
+                const double* ca = ctdi + 4 * (size_t)N * (2 * ch);
+                const double* cp = ctdi + 4 * (size_t)N * (2 * ch + 1);
+                const double Amp = spline_at(ca, N, si, sdx);
+                double Phase = spline_at(cp, N, si, sdx);
+                Phase += phase_ds;
+                Phase -= phase_het;
+                double s, c;
+                sincos_red(Phase, &s, &c);
+                x[ch] = Amp * c;
+            }
+            U[i] = make_double2(x[0], x[1]);
+            if (i & 1) V[i >> 1].y = x[2];
+            else V[i >> 1].x = x[2];
+        }
+    }
+    __syncthreads();
+    // forward FFTs: U (NDS) and V (NDS/2)
+    const bool three = (NDS % 3) == 0;
+    const double2* twU = three ? g.tw3 : g.tw;
+    const int tsU = (three ? kTw3N : kTwN) / NDS;
+    lds_fft<NDS, 1, kLayerThreads>(U, twU, tsU);
+    lds_fft<M, 1, kLayerThreads>(V, twU, tsU * 2);
+    // pixel list (ucb/src/glass_ucb_waveform.c:1163-1178): candidates in
+    // (time i, layer j) order, kept when kmin <= k < kmax
+    int* lst = a.list + (size_t)e * 3 * NT;
+    {
+        constexpr int NC = NT * L;
+        constexpr int PER = (NC + kLayerThreads - 1) / kLayerThreads;
+        const int c0 = tid * PER;
+        int mine = 0;
+        for (int c = c0; c < c0 + PER && c < NC; c++)
+        {
+            const int k = c / L + (h.jmin + c % L) * NT;
+            mine += (k >= g.kmin && k < g.kmax);
+        }
+        cnt[tid] = mine;
+        __syncthreads();
+        if (tid == 0)
+        {
+            int s = 0;
+            for (int t = 0; t < kLayerThreads; t++)
+            {
+                const int v = cnt[t];
+                cnt[t] = s;
+                s += v;
+            }
+            a.nwave[e] = s;
+            cnt[kLayerThreads] = s;
+        }
+        __syncthreads();
+        int pos = cnt[tid];
+        for (int c = c0; c < c0 + PER && c < NC; c++)
+        {
+            const int k = c / L + (h.jmin + c % L) * NT;
+            if (k >= g.kmin && k < g.kmax) lst[pos++] = k - g.kmin;
+        }
+    }
+    __syncthreads();
+    const int Nw = cnt[kLayerThreads];
+    const double* win = g.win[L];
+    const double norm = sqrt((double)g.NF) / (L + 1.0);
+    double* vals = a.values + (size_t)e * 9 * NT;
+    for (int ch = 0; ch < 3; ch++)
+    {
+        // windowed bands of the L layers, conjugated so the forward FFT
+        // gives the conjugate of the inverse transform
+        for (int it = tid; it < L * NT; it += kLayerThreads)
+        {
+            const int off = it / NT, jj = it % NT;
+            const int j = jj - NT / 2;
+            const int m = h.jmin + off;
+            const int center = (off + 1) * NT / 2;
+            const int idx = center + j;
+            double2 v = make_double2(0.0, 0.0);
+            if (jj > 0 && idx >= 0 && idx <= NDS / 2)
+            {
+                double wt = win[j < 0 ? -j : j];
+                if (j == 0 && (m == 0 || m == g.NF)) wt /= 2.0;
+                const double2 S = spectrum<NDS>(ch, idx, U, V, twU, tsU);
+                v = make_double2(wt * S.x, -(wt * S.y));
+            }
+            B[off * NT + jj] = v;
+        }
+        __syncthreads();
+        lds_fft<NT, L, kLayerThreads>(B, g.tw, kTwN / NT);
+        for (int it = tid; it < L * NT; it += kLayerThreads)
+        {
+            const int n = it / L, off = it % L;
+            const int m = h.jmin + off;
+            const double2 r = B[off * NT + n];
+            const double re = r.x / NT, im = -r.y / NT;
+            double val;
+            if (m == 0 || m == g.NF) val = (n % 2 == 0) ? re * M_SQRT2 * norm : 0.0;
+            else
+            {
+                const int sgn = (m % 2 == 0) ? 1 : -1;
+                val = ((n + m) % 2 == 0) ? re * norm : sgn * im * norm;
+            }
+            const int pos = n * L + off;
+            if (pos < Nw) vals[ch * 3 * NT + pos] = val;
+        }
+        __syncthreads();
+    }
+    if (!a.data) return;
+    __threadfence_block();
+    __syncthreads();
+    // wavelet_nwip terms over the pixel list: snr^2 = sum h^T C h; the
+    // logL change of adding the template to a zero model with the
+    // reference's edge rule (gaussian_log_likelhood_wavelet,
+    // utils/src/glass_model.c:805-855)
+    double hh = 0.0, dl = 0.0;
+    for (int t = tid; t < Nw; t += kLayerThreads)
+    {
+        const int k = lst[t];
+        const double hv[3] = {vals[t], vals[3 * NT + t], vals[6 * NT + t]};
+        double C[3][3];
+        C[0][0] = a.invC[k];
+        C[1][1] = a.invC[(size_t)a.npix + k];
+        C[2][2] = a.invC[2 * (size_t)a.npix + k];
+        C[0][1] = C[1][0] = a.invC[3 * (size_t)a.npix + k];
+        C[0][2] = C[2][0] = a.invC[4 * (size_t)a.npix + k];
+        C[1][2] = C[2][1] = a.invC[5 * (size_t)a.npix + k];
+        const double d[3] = {a.data[k], a.data[(size_t)a.npix + k], a.data[2 * (size_t)a.npix + k]};
+        double th = 0.0, tdh = 0.0, tdd = 0.0;
+        for (int p = 0; p < 3; p++)
+            for (int q = 0; q < 3; q++)
+            {
+                th += hv[p] * hv[q] * C[p][q];
+                tdh += d[p] * hv[q] * C[p][q];
+                tdd += d[p] * d[q] * C[p][q];
+            }
+        hh += th;
+        const int i = (k + g.kmin) % g.NT;
+        if (k >= 0 && k < a.npix) dl += (i < g.imin || i > g.imax) ? 0.5 * tdd : tdh - 0.5 * th;
+    }
+    hh = block_sum<kLayerThreads>(hh);
+    dl = block_sum<kLayerThreads>(dl);
+    if (tid == 0)
+    {
+        a.snr[e] = sqrt(hh);
+        a.dlogL[e] = dl;
+    }
+}
+
+template <int NT>
+__global__ void __launch_bounds__(kLayerThreads) layer_kernel(Geo g, int n, LayerArgs a,
+                                                              const double* __restrict__ scratch)
+{
+    const int e = blockIdx.x;
+    if (e >= n) return;
+    extern __shared__ double2 lds2[];
+    const double* scr = scratch + scratch_doubles(g.Norb) * (size_t)e;
+    __shared__ Hdr H;
+    if (threadIdx.x == 0) H = *(const Hdr*)scr;
+    __syncthreads();
+    const Hdr h = H;
+    // LDS: U (4 NT) | V (2 NT) | B (3 NT); list counts in static LDS
+    __shared__ int cnt[kLayerThreads + 1];
+    double2* U = lds2;
+    double2* V = U + 4 * NT;
+    double2* B = V + 2 * NT;
+    if (h.L == 1) layer_body<NT, 1>(g, h, e, a, U, V, B, cnt, scr);
+    else if (h.L == 2) layer_body<NT, 2>(g, h, e, a, U, V, B, cnt, scr);
+    else layer_body<NT, 3>(g, h, e, a, U, V, B, cnt, scr);
+}
+
+template <int NT>
+constexpr size_t layer_lds_bytes()
+{
+    return sizeof(double2) * 9 * NT;
+}
+
+// ----------------------------------------------------------- launching --
+
+static int launch_templates(const GeoHost& G, int n, const double* d_params, double* d_scratch,
+                            const LayerArgs& a, hipStream_t s)
+{
+    const Geo& g = G.g;
+    if (n <= 0) return GLASS_HIP_OK;
+    hipLaunchKernelGGL(resp_kernel, dim3(n), dim3(kRespThreads), resp_lds_bytes(g.Norb), s, g, n, d_params,
+                       d_scratch);
+    GH_CHECK(hipGetLastError());
+    if (g.NT == 512)
+        hipLaunchKernelGGL(layer_kernel<512>, dim3(n), dim3(kLayerThreads), layer_lds_bytes<512>(), s, g, n, a,
+                           d_scratch);
+    else
+        hipLaunchKernelGGL(layer_kernel<256>, dim3(n), dim3(kLayerThreads), layer_lds_bytes<256>(), s, g, n, a,
+                           d_scratch);
+    GH_CHECK(hipGetLastError());
+    return GLASS_HIP_OK;
+}
+
+static int set_lds_limits()
+{
+    static std::once_flag once;
+    static int rc = GLASS_HIP_OK;
+    std::call_once(once, [] {
+        if (hipFuncSetAttribute((const void*)resp_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)resp_lds_bytes(kMaxKnots)) != hipSuccess ||
+            hipFuncSetAttribute((const void*)layer_kernel<512>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)layer_lds_bytes<512>()) != hipSuccess)
+        {
+            set_error("hipFuncSetAttribute (dynamic LDS) failed");
+            rc = GLASS_HIP_ERUNTIME;
+        }
+    });
+    return rc;
+}
+
+// -------------------------------------------------- geometry cache ------
+
+struct GeoKey
+{
+    std::vector<double> blob;
+    bool operator<(const GeoKey& o) const { return blob < o.blob; }
+};
+
+static std::mutex g_geo_mu;
+static std::map<std::pair<int, GeoKey>, GeoHost*> g_geo;
+
+static int geo_get(const glass_hip_wdm* w, int Norb, const double* ot, const double* ox, const double* oy,
+                   const double* oz, double Tobs, double t0, const GeoHost** out)
+{
+    int rc = check_grid(w, Norb);
+    if (rc) return rc;
+    int dev = 0;
+    GH_CHECK(hipGetDevice(&dev));
+    GeoKey key;
+    key.blob = {(double)w->NF, (double)w->NT, w->dt, w->df, w->A, (double)w->kmin, (double)w->kmax,
+                (double)w->imin, (double)w->imax, Tobs, t0, (double)Norb};
+    key.blob.insert(key.blob.end(), ot, ot + Norb);
+    // pos rows: spacecraft i coordinate d at 3i+d
+    std::vector<double> pos(9 * (size_t)Norb);
+    for (int i = 0; i < 3; i++)
+    {
+        std::memcpy(pos.data() + (3 * i + 0) * (size_t)Norb, ox + i * (size_t)Norb, sizeof(double) * Norb);
+        std::memcpy(pos.data() + (3 * i + 1) * (size_t)Norb, oy + i * (size_t)Norb, sizeof(double) * Norb);
+        std::memcpy(pos.data() + (3 * i + 2) * (size_t)Norb, oz + i * (size_t)Norb, sizeof(double) * Norb);
+    }
+    key.blob.insert(key.blob.end(), pos.begin(), pos.end());
+    std::lock_guard<std::mutex> lk(g_geo_mu);
+    auto it = g_geo.find({dev, key});
+    if (it != g_geo.end())
+    {
+        *out = it->second;
+        return GLASS_HIP_OK;
+    }
+    if ((rc = set_lds_limits())) return rc;
+    GeoHost* G = new GeoHost();
+    if ((rc = geo_build(*G, *w, Norb, ot, pos.data(), Tobs, t0)))
+    {
+        delete G;
+        return rc;
+    }
+    g_geo[{dev, key}] = G;
+    *out = G;
+    return GLASS_HIP_OK;
+}
+
+}  // namespace wdm
+}  // namespace glass
+
+using namespace glass;
+using namespace glass::wdm;
+
+// ------------------------------------------------------------- drop-ins --
+
+extern "C" int glass_hip_interpolated_analytic_orbit(const glass_hip_orbit* orbit, double Tobs, double t0, int* Norb,
+                                                     double* t, double* x, double* y, double* z)
+{
+#pragma clang fp contract(off)
+    if (!orbit || !Norb || Tobs <= 0) return set_error("interpolated_analytic_orbit: bad arguments"), GLASS_HIP_EINVAL;
+    // initialize_interpolated_analytic_orbits (utils/src/glass_lisa.c:271-322)
+    const double YEAR = 3.15581497632e7;
+    int N = (int)(200 * Tobs / YEAR);
+    if (N < 20) N = 20;
+    *Norb = N;
+    if (!t) return GLASS_HIP_OK;
+    if (!x || !y || !z) return set_error("interpolated_analytic_orbit: NULL position array"), GLASS_HIP_EINVAL;
+    const double buffer = Tobs / (double)(N - 1);
+    const double cadence = (Tobs + 2.0 * buffer) / (double)(N - 1);
+    for (int n = 0; n < N; n++)
+    {
+        const double tn = t0 + n * cadence - buffer;
+        double p9[9];
+        analytic_positions(*orbit, tn, p9);
+        t[n] = tn;
+        for (int i = 0; i < 3; i++)
+        {
+            x[i * (size_t)N + n] = p9[i];
+            y[i * (size_t)N + n] = p9[3 + i];
+            z[i * (size_t)N + n] = p9[6 + i];
+        }
+    }
+    return GLASS_HIP_OK;
+}
+
+extern "C" int glass_hip_ucb_waveform_wavelet(const glass_hip_wdm* w, int Norb, const double* orbit_t,
+                                              const double* orbit_x, const double* orbit_y,
+                                              const double* orbit_z, double Tobs, double t0,
+                                              const double* params, int* wavelet_list, int* Nwavelet, double* X,
+                                              double* Y, double* Z)
+{
+    if (!orbit_t || !orbit_x || !orbit_y || !orbit_z || !params || !wavelet_list || !Nwavelet || !X || !Y || !Z)
+        return set_error("ucb_waveform_wavelet: NULL argument"), GLASS_HIP_EINVAL;
+    ThreadCtx* c;
+    int rc = ctx_get(&c);
+    if (rc) return rc;
+    const GeoHost* G;
+    if ((rc = geo_get(w, Norb, orbit_t, orbit_x, orbit_y, orbit_z, Tobs, t0, &G))) return rc;
+    const int NT = w->NT;
+    // device: params 8 | scratch | list 3NT ints | values 9NT | nwave
+    const size_t nscr = scratch_doubles(Norb);
+    const size_t bytes = sizeof(double) * (8 + nscr + 9 * NT) + sizeof(int) * (3 * NT + 4);
+    if ((rc = c->dbuf.ensure(bytes)) || (rc = c->hbuf.ensure(bytes))) return rc;
+    double* dp = (double*)c->dbuf.p;
+    double* dscr = dp + 8;
+    double* dval = dscr + nscr;
+    int* dlist = (int*)(dval + 9 * NT);
+    int* dnw = dlist + 3 * NT;
+    double* hp = (double*)c->hbuf.p;
+    std::memcpy(hp, params, sizeof(double) * 8);
+    GH_CHECK(hipMemcpyAsync(dp, hp, sizeof(double) * 8, hipMemcpyHostToDevice, c->stream));
+    LayerArgs a{};
+    a.nwave = dnw;
+    a.list = dlist;
+    a.values = dval;
+    if ((rc = launch_templates(*G, 1, dp, dscr, a, c->stream))) return rc;
+    GH_CHECK(hipMemcpyAsync(hp, dval, sizeof(double) * 9 * NT + sizeof(int) * (3 * NT + 1), hipMemcpyDeviceToHost,
+                            c->stream));
+    GH_CHECK(hipStreamSynchronize(c->stream));
+    const double* hv = hp;
+    const int* hl = (const int*)(hp + 9 * NT);
+    const int nw = hl[3 * NT];
+    *Nwavelet = nw;
+    // insert the pixels at their list positions (the caller pre-zeroes, as
+    // the reference's callers do)
+    for (int t = 0; t < nw; t++)
+    {
+        wavelet_list[t] = hl[t];
+        X[hl[t]] = hv[t];
+        Y[hl[t]] = hv[3 * NT + t];
+        Z[hl[t]] = hv[6 * NT + t];
+    }
+    return GLASS_HIP_OK;
+}
+
+// ---------------------------------------------------------------- batch --
+
+struct glass_hip_wdm_batch
+{
+    int device = 0;
+    const GeoHost* G = nullptr;
+    int npix = 0, max_templates = 0;
+    double* data = nullptr;   // 3 x npix
+    double* invC = nullptr;   // 6 x npix
+    double* scratch = nullptr;
+    int* list = nullptr;
+    double* values = nullptr;
+    int* nwave = nullptr;
+};
+
+extern "C" int glass_hip_wdm_batch_create(glass_hip_wdm_batch** plan, const glass_hip_wdm* w, int Norb,
+                                          const double* orbit_t, const double* orbit_x, const double* orbit_y,
+                                          const double* orbit_z, double Tobs, double t0, const double* data,
+                                          const double* invC, int npix, int max_templates)
+{
+    if (!plan || !orbit_t || !orbit_x || !orbit_y || !orbit_z || !data || !invC)
+        return set_error("wdm_batch_create: NULL argument"), GLASS_HIP_EINVAL;
+    if (npix <= 0 || max_templates <= 0)
+        return set_error("wdm_batch_create: npix=%d max_templates=%d", npix, max_templates), GLASS_HIP_EINVAL;
+    ThreadCtx* c;
+    int rc = ctx_get(&c);
+    if (rc) return rc;
+    auto* P = new glass_hip_wdm_batch();
+    P->device = c->device;
+    auto fail = [&](int r) {
+        glass_hip_wdm_batch_destroy(P);
+        return r;
+    };
+    if ((rc = geo_get(w, Norb, orbit_t, orbit_x, orbit_y, orbit_z, Tobs, t0, &P->G))) return fail(rc);
+    if (w->kmax - w->kmin != npix)
+    {
+        set_error("wdm_batch_create: npix = %d but kmax - kmin = %d", npix, w->kmax - w->kmin);
+        return fail(GLASS_HIP_EINVAL);
+    }
+    P->npix = npix;
+    P->max_templates = max_templates;
+    const int NT = w->NT;
+    if (hipMalloc(&P->data, sizeof(double) * 3 * (size_t)npix) != hipSuccess ||
+        hipMalloc(&P->invC, sizeof(double) * 6 * (size_t)npix) != hipSuccess ||
+        hipMalloc(&P->scratch, sizeof(double) * scratch_doubles(Norb) * (size_t)max_templates) != hipSuccess ||
+        hipMalloc(&P->list, sizeof(int) * 3 * NT * (size_t)max_templates) != hipSuccess ||
+        hipMalloc(&P->values, sizeof(double) * 9 * NT * (size_t)max_templates) != hipSuccess ||
+        hipMalloc(&P->nwave, sizeof(int) * (size_t)max_templates) != hipSuccess)
+    {
+        set_error("wdm_batch_create: device allocation failed");
+        return fail(GLASS_HIP_ERUNTIME);
+    }
+    if (hipMemcpy(P->data, data, sizeof(double) * 3 * (size_t)npix, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(P->invC, invC, sizeof(double) * 6 * (size_t)npix, hipMemcpyHostToDevice) != hipSuccess)
+    {
+        set_error("wdm_batch_create: upload failed");
+        return fail(GLASS_HIP_ERUNTIME);
+    }
+    *plan = P;
+    return GLASS_HIP_OK;
+}
+
+extern "C" int glass_hip_wdm_batch_destroy(glass_hip_wdm_batch* P)
+{
+    if (!P) return GLASS_HIP_OK;
+    (void)hipSetDevice(P->device);
+    for (void* p : {(void*)P->data, (void*)P->invC, (void*)P->scratch, (void*)P->list, (void*)P->values,
+                    (void*)P->nwave})
+        if (p) (void)hipFree(p);
+    delete P;
+    return GLASS_HIP_OK;
+}
+
+extern "C" int glass_hip_wdm_batch_buffers(glass_hip_wdm_batch* P, int** d_nwave, int** d_list,
+                                           double** d_values)
+{
+    if (!P) return set_error("wdm_batch_buffers: NULL plan"), GLASS_HIP_EINVAL;
+    if (d_nwave) *d_nwave = P->nwave;
+    if (d_list) *d_list = P->list;
+    if (d_values) *d_values = P->values;
+    return GLASS_HIP_OK;
+}
+
+extern "C" int glass_hip_wdm_batch_eval(glass_hip_wdm_batch* P, int n, const double* d_params, double* d_dlogL,
+                                        double* d_snr, void* stream)
+{
+    if (!P || !d_params || !d_dlogL || !d_snr) return set_error("wdm_batch_eval: NULL argument"), GLASS_HIP_EINVAL;
+    if (n < 0 || n > P->max_templates)
+        return set_error("wdm_batch_eval: n = %d outside 0..%d", n, P->max_templates), GLASS_HIP_EINVAL;
+    GH_CHECK(hipSetDevice(P->device));
+    LayerArgs a;
+    a.data = P->data;
+    a.invC = P->invC;
+    a.npix = P->npix;
+    a.nwave = P->nwave;
+    a.list = P->list;
+    a.values = P->values;
+    a.dlogL = d_dlogL;
+    a.snr = d_snr;
+    return launch_templates(*P->G, n, d_params, P->scratch, a, (hipStream_t)stream);
+}
+
+extern "C" int glass_hip_wdm_batch_templates(glass_hip_wdm_batch* P, int n, int* nwave, int* list, double* values)
+{
+    if (!P || !nwave || !list || !values) return set_error("wdm_batch_templates: NULL argument"), GLASS_HIP_EINVAL;
+    if (n < 0 || n > P->max_templates)
+        return set_error("wdm_batch_templates: n = %d outside 0..%d", n, P->max_templates), GLASS_HIP_EINVAL;
+    GH_CHECK(hipSetDevice(P->device));
+    GH_CHECK(hipDeviceSynchronize());
+    const size_t NT = P->G->g.NT;
+    GH_CHECK(hipMemcpy(nwave, P->nwave, sizeof(int) * n, hipMemcpyDeviceToHost));
+    GH_CHECK(hipMemcpy(list, P->list, sizeof(int) * 3 * NT * n, hipMemcpyDeviceToHost));
+    GH_CHECK(hipMemcpy(values, P->values, sizeof(double) * 9 * NT * n, hipMemcpyDeviceToHost));
+    return GLASS_HIP_OK;
+}
+
+extern "C" int glass_hip_wdm_batch_scratch(glass_hip_wdm_batch* P, int e, double* out, int nout)
+{
+    if (!P || !out) return set_error("wdm_batch_scratch: NULL argument"), GLASS_HIP_EINVAL;
+    const size_t ns = scratch_doubles(P->G->g.Norb);
+    if (e < 0 || e >= P->max_templates || nout < (int)ns)
+        return set_error("wdm_batch_scratch: template %d / buffer %d (need %zu)", e, nout, ns), GLASS_HIP_EINVAL;
+    GH_CHECK(hipSetDevice(P->device));
+    GH_CHECK(hipDeviceSynchronize());
+    GH_CHECK(hipMemcpy(out, P->scratch + ns * (size_t)e, sizeof(double) * ns, hipMemcpyDeviceToHost));
+    return GLASS_HIP_OK;
+}

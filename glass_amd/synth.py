@@ -168,3 +168,57 @@ def config1(seed=1, ngroup=256, ntemp=16, nprop_per_chain=80, T=T_2YR, f_lo=0.5e
     return dict(T=T, group_qmin=gqmin.astype(np.int32), group_W=Wg, data=data, C=invC33,
                 chain_group=chain_group, chain_nsrc=chain_nsrc, src_params=src_params,
                 p_chain=p_chain, p_src=p_src, p_params=p_params, injected=inj, seed=seed)
+
+
+def wdm_pixel_cov(T, NF, NT):
+    """3x3 pixel covariance per WDM layer m = 0..NF, [NF+1, 3, 3], of
+    stationary noise with the analytic XYZ spectrum: the layer-centre
+    approximation of stationary_dft_psd_to_wdm_psd (utils/src/glass_wavelet.c:
+    902-921: DFT variance / ND at bin m NT/2), with the DFT variance
+    4 ND^2 C(f)/T of this module's Fourier convention (C = E|d_k|^2)."""
+    ND = NF * NT
+    f = np.arange(NF + 1) * (NT / 2) / T
+    f[0] = f[1]                          # the spectrum diverges at DC
+    Cl = np.empty((NF + 1, 3, 3))
+    for m in range(NF + 1):
+        Cm = xyz_noise_cov(T, f[m], 1)[:, :, 0]
+        Cl[m] = 4.0 * ND * Cm / T
+    return Cl
+
+
+def config2(seed=7, n=4096, NF=4096, NT=512, T=YEAR_T, inject=8):
+    """Wavelet-domain case: an NT x NF grid over Tobs (15 s cadence at 1 yr:
+    NT 512, NF 4096, dt = T/NT), n templates with config-0 distributions,
+    stationary Gaussian pixel noise with per-layer 3x3 covariance, and the
+    first `inject` templates' parameters (for the caller to inject).
+    Returns the pixel data of the data range [kmin, kmax) = layers 1..NF-2
+    as [3, npix], invC as 9 row-major planes [9, npix] and 6 planes
+    (XX, YY, ZZ, XY, XZ, YZ) [6, npix]."""
+    rng = np.random.default_rng(seed)
+    dt = T / NT
+    P = draw_params(rng, n, T)
+    Cl = wdm_pixel_cov(T, NF, NT)
+    kmin, kmax = NT, NT * (NF - 1)
+    layers = np.arange(kmin, kmax) // NT
+    Lc = np.linalg.cholesky(Cl)                       # [NF+1, 3, 3]
+    z = rng.standard_normal((kmax - kmin, 3))
+    data = np.einsum("kij,kj->ik", Lc[layers], z)     # [3, npix]
+    iC = np.linalg.inv(Cl)                            # [NF+1, 3, 3]
+    invC9 = np.ascontiguousarray(iC[layers].reshape(-1, 9).T)
+    invC6 = np.ascontiguousarray(invC9[[0, 4, 8, 1, 2, 5]])
+    return dict(params=P, T=T, NF=NF, NT=NT, dt=dt, kmin=kmin, kmax=kmax, data=np.ascontiguousarray(data),
+                invC9=invC9, invC6=invC6, inject=P[:inject].copy(), seed=seed)
+
+
+def wdm_random_noise(NF, NT, kmin, kmax, seed):
+    """Per-layer random SPD pixel covariance (condition ~10) on the pixel
+    range [kmin, kmax): (data [3, npix], invC [9, npix]).  With it the
+    full-grid wavelet logL is accurate to rounding, unlike the XYZ spectrum
+    near DC whose covariance is nearly singular."""
+    rng = np.random.default_rng(seed)
+    A = rng.standard_normal((NF + 1, 3, 3))
+    Cl = np.einsum("mij,mkj->mik", A, A) + 3 * np.eye(3)
+    layers = np.arange(kmin, kmax) // NT
+    d = np.einsum("kij,kj->ik", np.linalg.cholesky(Cl)[layers], rng.standard_normal((len(layers), 3)))
+    invC9 = np.ascontiguousarray(np.linalg.inv(Cl)[layers].reshape(-1, 9).T)
+    return np.ascontiguousarray(d), invC9

@@ -132,6 +132,109 @@ void orc_ucb_fisher_fd(const orc_orbit *orbit, const char *format, double T, dou
                        int NFFT, int data_qmin, const double *params,
                        const double *invC, double *fisher);
 
+/* polarization tensors and propagation vector, 1-indexed
+ * (utils/src/glass_lisa.c:1566-1596) */
+void orc_polarization_tensor(double costh, double phi, double ep[4][4], double ec[4][4], double k[4]);
+
+/* ------------------------------------------------------------- WDM ----- */
+/* The fields of struct Wavelets (utils/src/glass_wavelet.h:25-67) that the
+ * wavelet path reads, filled by hand for an NF x NT grid of pixel duration
+ * dt: df = 1/(2 dt), A = 0 (setup_wdm_basis, utils/src/glass_wavelet.c:39-49),
+ * imin/imax = 100 / NT-100 (WAVELET_EDGE_BUFFER), kmin = NT (pixel (0,1)),
+ * kmax = NT*(NF-1) (pixel (0,NF-1)); window = build_wdm_filter_freq(.., NF,
+ * NT, A, forward) in caller memory of NT/2+1 doubles. */
+typedef struct
+{
+    int NF, NT;
+    double dt, df, A;
+    int kmin, kmax, imin, imax;
+    double *window;
+} orc_wdm;
+
+void   orc_wdm_init(orc_wdm *w, int NF, int NT, double dt, double *window);
+/* utils/src/glass_math.c:1214-1242 */
+double orc_incomplete_beta(double a, double b, double x);
+/* utils/src/glass_wavelet.c:327-349 */
+void   orc_wdm_filter(double *phif, int Nf, int Nt, double A, int forward);
+/* utils/src/glass_wavelet.c:410-460: freq ND/2+1 complex bins -> ND pixels
+ * (pixel (n, m) at n + NT m) */
+void   orc_wavelet_transform_freq(const orc_wdm *w, const double *freq, double *wdm);
+/* utils/src/glass_wavelet.c:293-303 (in place, ND = NF*NT real samples) */
+void   orc_wavelet_transform_timefreq(const orc_wdm *w, double *timedata);
+/* utils/src/glass_wavelet.c:351-405: ND pixels -> ND/2+1 complex bins */
+void   orc_wavelet_transform_inverse_freq(const orc_wdm *w, const double *wdm, double *freq);
+/* utils/src/glass_wavelet.c:544-611 (in place, NT*(Nlayers+1) samples) */
+void   orc_wavelet_transform_timefreq_by_layers(const orc_wdm *w, int jmin, int Nlayers,
+                                                const double *phif, double *timedata);
+/* utils/src/glass_wavelet.c:865-900: NF+1 layer variances from a one-sided
+ * DFT PSD of ND/2+1 bins */
+void   orc_wdm_layer_var(const orc_wdm *w, const double *dft_psd, double *layer_var);
+
+/* natural cubic spline (utils/src/glass_math.c:100-190): c = y0|y1|y2|y3,
+ * N doubles each; lookup even_sampled_search (even != 0) or binary_search
+ * (utils/src/glass_math.c:393-467) */
+void   orc_spline_coefficients(int N, const double *x, const double *y, double *c);
+double orc_spline_eval(int N, const double *x, const double *c, int even, double xv);
+int    orc_binary_search(const double *a, int lo, int hi, double x);
+int    orc_even_search(const double *a, int lo, int hi, double x);
+
+/* initialize_interpolated_analytic_orbits (utils/src/glass_lisa.c:271-322):
+ * N = orc_iorbit_size(Tobs) knots t[N], positions pos[9][N] (spacecraft i,
+ * coordinate d at row 3i+d) and their spline coefficients coef[9][4][N]. */
+int    orc_iorbit_size(double Tobs);
+void   orc_iorbit_fill(const orc_orbit *o, double Tobs, double t0, int N, double *t, double *pos,
+                       double *coef);
+
+/* utils/src/glass_math.c:1244-1308 and :1034-1052 */
+void   orc_extract_amplitude_and_phase(int Ns, double *As, double *Dphi, const double *M,
+                                       const double *Mf, const double *phiR);
+void   orc_unwrap_phase(int N, double *phase);
+
+/* 0 (default): the template path calls the platform libm's sin/cos, as the
+ * reference does; 1: correctly rounded sin/cos (see orc_wdm.c). */
+void   orc_set_cr_trig(int on);
+
+/* ucb/src/glass_ucb_waveform.c:1003-1037 */
+void   orc_ucb_wavelet_layers(const orc_wdm *w, double Tobs, const double *params, int *jstart,
+                              int *jwidth);
+/* The interpolated TDI series A(t) cos(Phi(t) - 2 pi fhet t) of
+ * ucb_waveform_wavelet at N samples t0 + i dt_s (out: 3 x N); see orc_wdm.c */
+void   orc_ucb_wavelet_series(int Norb, const double *ot, const double *oc, double Tobs, double t0,
+                              const double *params, int N, double dt_s, double fhet, double *out);
+/* The response stage of ucb_waveform_wavelet alone: barycentre phase at the
+ * knots, phase at spacecraft 1, signed TDI amplitude and unwrapped phase
+ * (3 x Norb each) at the response times. */
+void   orc_ucb_wavelet_response(int Norb, const double *ot, const double *oc, double Tobs, double t0,
+                                const double *params, double *phase_ssb, double *phase_sc,
+                                double *amp3, double *ph3);
+/* ucb_waveform_wavelet (ucb/src/glass_ucb_waveform.c:1061-1190) with the
+ * orbit given as its knots/coefficients.  The compact form writes the
+ * pixel values in list order (wX[n] belongs to pixel list[n]); the drop-in
+ * form scatters them into X,Y,Z at list[n] as the reference does.  Both
+ * return the number of layers. */
+int    orc_ucb_waveform_wavelet_compact(const orc_wdm *w, int Norb, const double *ot,
+                                        const double *oc, double Tobs, double t0,
+                                        const double *params, int *list, int *Nwavelet,
+                                        double *wX, double *wY, double *wZ);
+int    orc_ucb_waveform_wavelet(const orc_wdm *w, int Norb, const double *ot, const double *oc,
+                                double Tobs, double t0, const double *params, int *list,
+                                int *Nwavelet, double *X, double *Y, double *Z);
+/* utils/src/glass_math.c:428-438; invC 9 planes of Npix */
+double orc_snr_wavelet(const double *X, const double *Y, const double *Z, const double *invC,
+                       int Npix, const int *list, int Nlist);
+/* utils/src/glass_model.c:805-855; data/model 3 planes of Npix, invC 9 */
+double orc_gaussian_log_likelihood_wavelet(const orc_wdm *w, const double *dX, const double *dY,
+                                           const double *dZ, const double *mX, const double *mY,
+                                           const double *mZ, const int *mlist, int Nlist,
+                                           const double *invC, int Npix);
+/* CPU baseline of the wavelet workload: per template the compact waveform,
+ * snr_wavelet and the logL change the template makes (see orc_wdm.c);
+ * data 3 planes and invC 9 planes of Npix.  Returns wall seconds. */
+double orc_wdm_eval_batch(const orc_wdm *w, int Norb, const double *ot, const double *oc,
+                          double Tobs, double t0, int n, const double *params, const double *data,
+                          const double *invC, int Npix, double *dlogL, double *snr, int *nwave,
+                          int nthreads);
+
 /* ------------------------------------------------------ CPU baseline ---- */
 /* One MCMC-step likelihood evaluation the way the reference's sampler
  * performs it (ucb/src/glass_ucb_sampler.c:139-155 -> generate_ucb_model,

@@ -92,6 +92,7 @@ def lib():
                                      ctypes.POINTER(Chain), ctypes.c_int, _ip, _ip, _dp, _dp,
                                      ctypes.c_int]
         L.orc_eval_batch.restype = ctypes.c_double
+        _wdm_signatures(L)
         _lib = L
     return _lib
 
@@ -268,3 +269,251 @@ def ref_kissfft():
         L.kiss_fftri.argtypes = [ctypes.c_void_p, _dp, _dp]
         _ref = L
     return _ref
+
+
+# ----------------------------------------------------------------- WDM -----
+
+class WdmStruct(ctypes.Structure):
+    _fields_ = [("NF", ctypes.c_int), ("NT", ctypes.c_int), ("dt", ctypes.c_double),
+                ("df", ctypes.c_double), ("A", ctypes.c_double), ("kmin", ctypes.c_int),
+                ("kmax", ctypes.c_int), ("imin", ctypes.c_int), ("imax", ctypes.c_int),
+                ("window", _dp)]
+
+
+def _wdm_signatures(L):
+    c_int, c_d = ctypes.c_int, ctypes.c_double
+    W = ctypes.POINTER(WdmStruct)
+    L.orc_wdm_init.argtypes = [W, c_int, c_int, c_d, _dp]
+    L.orc_incomplete_beta.argtypes = [c_d, c_d, c_d]
+    L.orc_incomplete_beta.restype = c_d
+    L.orc_wdm_filter.argtypes = [_dp, c_int, c_int, c_d, c_int]
+    L.orc_wavelet_transform_freq.argtypes = [W, _dp, _dp]
+    L.orc_wavelet_transform_timefreq.argtypes = [W, _dp]
+    L.orc_wavelet_transform_inverse_freq.argtypes = [W, _dp, _dp]
+    L.orc_wavelet_transform_timefreq_by_layers.argtypes = [W, c_int, c_int, _dp, _dp]
+    L.orc_wdm_layer_var.argtypes = [W, _dp, _dp]
+    L.orc_spline_coefficients.argtypes = [c_int, _dp, _dp, _dp]
+    L.orc_spline_eval.argtypes = [c_int, _dp, _dp, c_int, c_d]
+    L.orc_spline_eval.restype = c_d
+    L.orc_binary_search.argtypes = [_dp, c_int, c_int, c_d]
+    L.orc_binary_search.restype = c_int
+    L.orc_even_search.argtypes = [_dp, c_int, c_int, c_d]
+    L.orc_even_search.restype = c_int
+    L.orc_iorbit_size.argtypes = [c_d]
+    L.orc_iorbit_size.restype = c_int
+    L.orc_iorbit_fill.argtypes = [ctypes.POINTER(Orbit), c_d, c_d, c_int, _dp, _dp, _dp]
+    L.orc_extract_amplitude_and_phase.argtypes = [c_int, _dp, _dp, _dp, _dp, _dp]
+    L.orc_unwrap_phase.argtypes = [c_int, _dp]
+    L.orc_ucb_wavelet_layers.argtypes = [W, c_d, _dp, _ip, _ip]
+    L.orc_ucb_waveform_wavelet_compact.argtypes = [W, c_int, _dp, _dp, c_d, c_d, _dp, _ip, _ip,
+                                                   _dp, _dp, _dp]
+    L.orc_ucb_waveform_wavelet_compact.restype = c_int
+    L.orc_ucb_wavelet_series.argtypes = [c_int, _dp, _dp, c_d, c_d, _dp, c_int, c_d, c_d, _dp]
+    L.orc_set_cr_trig.argtypes = [c_int]
+    L.orc_ucb_wavelet_response.argtypes = [c_int, _dp, _dp, c_d, c_d, _dp, _dp, _dp, _dp, _dp]
+    L.orc_snr_wavelet.argtypes = [_dp, _dp, _dp, _dp, c_int, _ip, c_int]
+    L.orc_snr_wavelet.restype = c_d
+    L.orc_gaussian_log_likelihood_wavelet.argtypes = [W] + [_dp] * 6 + [_ip, c_int, _dp, c_int]
+    L.orc_gaussian_log_likelihood_wavelet.restype = c_d
+    L.orc_wdm_eval_batch.argtypes = [W, c_int, _dp, _dp, c_d, c_d, c_int, _dp, _dp, _dp, c_int,
+                                     _dp, _dp, _ip, c_int]
+    L.orc_wdm_eval_batch.restype = c_d
+
+
+class Wdm:
+    """Hand-filled WDM geometry (NF layers x NT time pixels of duration dt)
+    with its forward-normalised Meyer window."""
+
+    def __init__(self, NF, NT, dt):
+        self.window = np.zeros(NT // 2 + 1)
+        self.s = WdmStruct()
+        lib().orc_wdm_init(ctypes.byref(self.s), NF, NT, float(dt), _d(self.window))
+        for f, _ in WdmStruct._fields_:
+            if f != "window":
+                setattr(self, f, getattr(self.s, f))
+
+    @property
+    def npix(self):
+        return self.kmax - self.kmin
+
+    @property
+    def ref(self):
+        return ctypes.byref(self.s)
+
+
+def incomplete_beta(a, b, x):
+    return lib().orc_incomplete_beta(a, b, x)
+
+
+def wdm_filter(Nf, Nt, A=0.0, forward=True):
+    phif = np.zeros(Nt // 2 + 1)
+    lib().orc_wdm_filter(_d(phif), Nf, Nt, A, int(forward))
+    return phif
+
+
+def wavelet_transform_timefreq(w, x):
+    out = np.array(x, dtype=np.float64, copy=True)
+    lib().orc_wavelet_transform_timefreq(w.ref, _d(out))
+    return out
+
+
+def wavelet_transform_freq(w, freq):
+    freq = np.ascontiguousarray(freq, dtype=np.float64)
+    out = np.zeros(w.NF * w.NT)
+    lib().orc_wavelet_transform_freq(w.ref, _d(freq), _d(out))
+    return out
+
+
+def wavelet_transform_inverse_freq(w, pix):
+    pix = np.ascontiguousarray(pix, dtype=np.float64)
+    out = np.zeros(w.NF * w.NT + 2)
+    lib().orc_wavelet_transform_inverse_freq(w.ref, _d(pix), _d(out))
+    return out
+
+
+def wavelet_transform_by_layers(w, jmin, nlayers, phif, x):
+    out = np.array(x, dtype=np.float64, copy=True)
+    phif = np.ascontiguousarray(phif, dtype=np.float64)
+    lib().orc_wavelet_transform_timefreq_by_layers(w.ref, jmin, nlayers, _d(phif), _d(out))
+    return out
+
+
+def wdm_layer_var(w, dft_psd):
+    psd = np.ascontiguousarray(dft_psd, dtype=np.float64)
+    out = np.zeros(w.NF + 1)
+    lib().orc_wdm_layer_var(w.ref, _d(psd), _d(out))
+    return out
+
+
+def spline_coefficients(x, y):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    c = np.zeros(4 * len(x))
+    lib().orc_spline_coefficients(len(x), _d(x), _d(y), _d(c))
+    return c
+
+
+def spline_eval(x, c, xv, even=False):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    c = np.ascontiguousarray(c, dtype=np.float64)
+    return np.array([lib().orc_spline_eval(len(x), _d(x), _d(c), int(even), float(v))
+                     for v in np.atleast_1d(xv)])
+
+
+def binary_search(a, x):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return lib().orc_binary_search(_d(a), 0, len(a), float(x))
+
+
+def iorbit(Tobs, t0=0.0, orbit=None):
+    """Knots, positions [9,N] and spline coefficients [9,4N] of the
+    interpolated analytic orbit."""
+    orbit = orbit or analytic_orbit()
+    N = lib().orc_iorbit_size(float(Tobs))
+    t = np.zeros(N)
+    pos = np.zeros((9, N))
+    coef = np.zeros((9, 4 * N))
+    lib().orc_iorbit_fill(ctypes.byref(orbit), float(Tobs), float(t0), N, _d(t), _d(pos), _d(coef))
+    return t, pos, coef
+
+
+def extract_amplitude_and_phase(M, Mf, phiR):
+    M, Mf, phiR = (np.ascontiguousarray(v, dtype=np.float64) for v in (M, Mf, phiR))
+    As = np.zeros(len(M))
+    Dphi = np.zeros(len(M))
+    lib().orc_extract_amplitude_and_phase(len(M), _d(As), _d(Dphi), _d(M), _d(Mf), _d(phiR))
+    return As, Dphi
+
+
+def unwrap_phase(ph):
+    out = np.array(ph, dtype=np.float64, copy=True)
+    lib().orc_unwrap_phase(len(out), _d(out))
+    return out
+
+
+def ucb_wavelet_layers(w, Tobs, params):
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    j0, nl = ctypes.c_int(), ctypes.c_int()
+    lib().orc_ucb_wavelet_layers(w.ref, float(Tobs), _d(p), ctypes.byref(j0), ctypes.byref(nl))
+    return j0.value, nl.value
+
+
+class cr_trig:
+    """Context manager: the wavelet template path of the oracle uses
+    correctly rounded sin/cos inside the block (glibc's otherwise)."""
+
+    def __enter__(self):
+        lib().orc_set_cr_trig(1)
+        return self
+
+    def __exit__(self, *a):
+        lib().orc_set_cr_trig(0)
+
+
+def ucb_waveform_wavelet(w, orb, Tobs, params, t0=0.0):
+    """Compact ucb_waveform_wavelet: (list [Nw] int32, values [3, Nw])."""
+    t, _, coef = orb
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    cap = 4 * w.NT
+    lst = np.zeros(cap, dtype=np.int32)
+    vals = np.zeros((3, cap))
+    nw = ctypes.c_int()
+    lib().orc_ucb_waveform_wavelet_compact(w.ref, len(t), _d(t), _d(coef), float(Tobs), float(t0),
+                                           _d(p), _i(lst), ctypes.byref(nw), _d(vals[0]),
+                                           _d(vals[1]), _d(vals[2]))
+    n = nw.value
+    return lst[:n].copy(), vals[:, :n].copy()
+
+
+def ucb_wavelet_series(orb, Tobs, params, N, dt_s, fhet, t0=0.0):
+    """Interpolated TDI series [3, N] at t0 + i dt_s, heterodyned by fhet."""
+    t, _, coef = orb
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    out = np.zeros((3, N))
+    lib().orc_ucb_wavelet_series(len(t), _d(t), _d(coef), float(Tobs), float(t0), _d(p), int(N),
+                                 float(dt_s), float(fhet), _d(out))
+    return out
+
+
+def ucb_wavelet_response(orb, Tobs, params, t0=0.0):
+    """(phase_ssb [N], phase_sc [N], amp [3,N], phase [3,N]) of the response stage."""
+    t, _, coef = orb
+    N = len(t)
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    ps, pc, a3, p3 = np.zeros(N), np.zeros(N), np.zeros((3, N)), np.zeros((3, N))
+    lib().orc_ucb_wavelet_response(N, _d(t), _d(coef), float(Tobs), float(t0), _d(p), _d(ps), _d(pc), _d(a3),
+                                   _d(p3))
+    return ps, pc, a3, p3
+
+
+def snr_wavelet(X, Y, Z, invC9, lst):
+    X, Y, Z = (np.ascontiguousarray(v, dtype=np.float64) for v in (X, Y, Z))
+    ic = np.ascontiguousarray(invC9, dtype=np.float64)
+    lst = np.ascontiguousarray(lst, dtype=np.int32)
+    return lib().orc_snr_wavelet(_d(X), _d(Y), _d(Z), _d(ic), len(X), _i(lst), len(lst))
+
+
+def gaussian_log_likelihood_wavelet(w, data, model, mlist, invC9):
+    """data, model: [3, Npix]; invC9: [9, Npix] (row-major 3x3 planes)."""
+    d = np.ascontiguousarray(data, dtype=np.float64)
+    m = np.ascontiguousarray(model, dtype=np.float64)
+    ic = np.ascontiguousarray(invC9, dtype=np.float64)
+    ml = np.ascontiguousarray(mlist, dtype=np.int32)
+    npix = d.shape[1]
+    return lib().orc_gaussian_log_likelihood_wavelet(w.ref, _d(d[0]), _d(d[1]), _d(d[2]), _d(m[0]),
+                                                     _d(m[1]), _d(m[2]), _i(ml), len(ml), _d(ic),
+                                                     npix)
+
+
+def wdm_eval_batch(w, orb, Tobs, params, data, invC9, t0=0.0, nthreads=0):
+    t, _, coef = orb
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    d = np.ascontiguousarray(data, dtype=np.float64)
+    ic = np.ascontiguousarray(invC9, dtype=np.float64)
+    n = p.shape[0]
+    dl = np.zeros(n)
+    sn = np.zeros(n)
+    nw = np.zeros(n, dtype=np.int32)
+    secs = lib().orc_wdm_eval_batch(w.ref, len(t), _d(t), _d(coef), float(Tobs), float(t0), n, _d(p),
+                                    _d(d), _d(ic), d.shape[1], _d(dl), _d(sn), _i(nw), nthreads)
+    return dl, sn, nw, secs

@@ -66,7 +66,7 @@ void orc_analytic_orbits(const orc_orbit *o, double t, double *x, double *y, dou
     }
 }
 
-static void polarization_tensor(double costh, double phi, double ep[4][4], double ec[4][4], double k[4])
+void orc_polarization_tensor(double costh, double phi, double ep[4][4], double ec[4][4], double k[4])
 {
     double u[4], v[4];
     double sinth = sqrt(1.0 - costh * costh);
@@ -136,7 +136,7 @@ void orc_ucb_slow_series(const orc_orbit *orbit, double T, double t0, const doub
     double df = O_PI2 * (((double)q) / T);
     double DPr = Aplus * cosps, DPi = -Across * sinps;
     double DCr = -Aplus * sinps, DCi = -Across * cosps;
-    polarization_tensor(costh, phi, ep, ec, k);
+    orc_polarization_tensor(costh, phi, ep, ec, k);
 
     double *s = (double *)calloc((size_t)6 * 2 * BW, sizeof(double));
     for (int n = 1; n <= BW; n++)

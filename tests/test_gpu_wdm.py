@@ -1,0 +1,208 @@
+"""GPU parity of the wavelet-domain path against the oracle.
+
+* transforms (forward, from spectrum, inverse; device API) at grids from
+  64 x 32 up to the 4096 x 512 grid of a year at 15 s, random series and the
+  reference's impulse known answer: normwise 1e-12;
+* interpolated orbit knots: bitwise;
+* ucb_waveform_wavelet drop-in: identical pixel lists; values normwise 1e-12
+  per template against the oracle with correctly rounded sin/cos.  Against
+  the glibc oracle the bound per template is 1e-12 plus that template's own
+  glibc-vs-correctly-rounded difference: glibc misrounds ~0.1% of sin/cos
+  arguments and the TDI sums cancel by (f/f*)^2, so the reference's output
+  itself moves by up to ~3e-11 between libms (1.5% of templates > 1e-12);
+* batched templates: lists/values as above, snr and dlogL relative 1e-10;
+* snr_wavelet: relative 1e-12; gaussian_log_likelhood_wavelet: the
+  reference's formula cancels (XX and 2XY of ~1e14 for the XYZ spectrum near
+  DC), so its own rounding is ~1e-16 * sum|terms|; the bound is 1e-12 of
+  that scale (and 1e-10 relative on a well-conditioned covariance).
+"""
+import numpy as np
+import pytest
+
+from glass_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+T1 = synth.YEAR_T
+
+
+def normwise(a, b):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def W():
+    from glass_amd import wdm
+    return wdm
+
+
+def grids(orc, W, NF, NT, dt=100.0):
+    return orc.Wdm(NF, NT, dt), W.Wdm.grid(NF, NT, dt)
+
+
+@pytest.mark.parametrize("NF,NT", [(64, 32), (96, 64), (1536, 64), (256, 512), (4096, 512)])
+def test_transform_forward_vs_oracle(orc, W, NF, NT):
+    ow, gw = grids(orc, W, NF, NT)
+    x = np.random.default_rng(NF * 7 + NT).standard_normal(NF * NT)
+    ref = orc.wavelet_transform_timefreq(ow, x)
+    got = W.wavelet_transform_timefreq(gw, x)
+    err = normwise(got, ref)
+    print(NF, NT, "forward normwise", err)
+    assert err < 1e-12
+
+
+@pytest.mark.parametrize("NF,NT", [(64, 32), (1536, 64), (4096, 512)])
+def test_transform_freq_and_inverse_vs_oracle(orc, W, NF, NT):
+    ow, gw = grids(orc, W, NF, NT)
+    x = np.random.default_rng(NF + NT).standard_normal(NF * NT)
+    F = orc.fftr(x).view(np.float64)
+    assert normwise(W.wavelet_transform_freq(gw, F), orc.wavelet_transform_freq(ow, F)) < 1e-12
+    pix = orc.wavelet_transform_timefreq(ow, x)
+    got = W.wavelet_transform_inverse_freq(gw, pix)
+    ref = orc.wavelet_transform_inverse_freq(ow, pix)
+    assert normwise(got, ref) < 1e-12
+
+
+@pytest.mark.parametrize("NF,NT", [(1536, 32), (64, 64)])
+def test_transform_impulse_known_answer(W, NF, NT):
+    """fft_wdm_unit_tests.c:547-590 through the GPU transforms."""
+    gw = W.Wdm.grid(NF, NT, 7680.0)
+    x = np.zeros(NF * NT)
+    x[0] = 1.0
+    F = W.wavelet_transform_inverse_freq(gw, W.wavelet_transform_timefreq(gw, x))
+    ref = np.zeros(NF * NT + 2)
+    ref[0::2] = 1.0
+    assert np.all(np.abs(F - ref) <= 1e-10 + 1e-5 * np.abs(ref))
+
+
+def test_wdm_transform_device_api(W):
+    import torch
+    gw = W.Wdm.grid(4096, 512, T1 / 512)
+    x = np.random.default_rng(3).standard_normal(4096 * 512)
+    d = torch.tensor(x, device="cuda")
+    out = W.wdm_transform(gw, d)
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy(), W.wavelet_transform_timefreq(gw, x))
+
+
+def test_transform_bad_grid(W):
+    from glass_amd import GlassHipError
+    with pytest.raises(GlassHipError):
+        W.wavelet_transform_timefreq(W.Wdm.grid(64, 48, 100.0), np.zeros(64 * 48))   # NT not 2^a
+    with pytest.raises(GlassHipError):
+        W.wavelet_transform_timefreq(W.Wdm.grid(80, 32, 100.0), np.zeros(80 * 32))   # NF 5*2^4
+
+
+def test_interpolated_orbit_bitwise(orc, gl, W):
+    t, x, y, z = W.interpolated_analytic_orbit(gl.analytic_orbit(), T1)
+    ot, pos, _ = orc.iorbit(T1)
+    assert np.array_equal(t, ot)
+    for i in range(3):
+        assert np.array_equal(x[i], pos[3 * i]) and np.array_equal(y[i], pos[3 * i + 1])
+        assert np.array_equal(z[i], pos[3 * i + 2])
+
+
+@pytest.fixture(scope="module")
+def g2(orc, gl, W):
+    NF, NT = 4096, 512
+    ow, gw = grids(orc, W, NF, NT, T1 / NT)
+    return ow, gw, orc.iorbit(T1), W.interpolated_analytic_orbit(gl.analytic_orbit(), T1)
+
+
+def oracle_templates(orc, ow, oorb, p):
+    """(list, values with glibc sin/cos, values with correctly rounded sin/cos)"""
+    lst, v = orc.ucb_waveform_wavelet(ow, oorb, T1, p)
+    with orc.cr_trig():
+        l2, vcr = orc.ucb_waveform_wavelet(ow, oorb, T1, p)
+    assert np.array_equal(lst, l2)
+    return lst, v, vcr
+
+
+def test_ucb_waveform_wavelet_dropin(orc, W, g2):
+    ow, gw, oorb, gorb = g2
+    P = synth.draw_params(np.random.default_rng(17), 48, T1)
+    X = np.zeros(gw.npix)
+    Y, Z = np.zeros_like(X), np.zeros_like(X)
+    worst_cr, worst_env = 0.0, 0.0
+    for e, p in enumerate(P):
+        lst, v, vcr = oracle_templates(orc, ow, oorb, p)
+        glst, X, Y, Z = W.ucb_waveform_wavelet(gw, gorb, T1, p, X=X, Y=Y, Z=Z)
+        assert np.array_equal(glst, lst), e
+        got = np.stack([X[glst], Y[glst], Z[glst]])
+        e_cr = normwise(got, vcr)
+        env = normwise(v, vcr)
+        e_gl = normwise(got, v)
+        assert e_gl <= 1e-12 + env, (e, e_gl, env)
+        worst_cr = max(worst_cr, e_cr)
+        worst_env = max(worst_env, env)
+        X[glst] = Y[glst] = Z[glst] = 0.0
+    print("wavelet waveform worst normwise vs CR oracle", worst_cr, "; oracle glibc-vs-CR envelope", worst_env)
+    assert worst_cr < 1e-12
+
+
+def test_wavelet_batch_vs_oracle(orc, W, g2):
+    import torch
+    ow, gw, oorb, gorb = g2
+    cfg = synth.config2(n=256, seed=5)
+    P = cfg["params"]
+    B = W.WdmBatch(gw, gorb, T1, cfg["data"], cfg["invC6"], max_templates=256)
+    dl, sn = B.eval(torch.tensor(P, device="cuda"))
+    dl, sn = dl.cpu().numpy(), sn.cpu().numpy()
+    with orc.cr_trig():
+        ref_dl, ref_sn, ref_nw, _ = orc.wdm_eval_batch(ow, oorb, T1, P, cfg["data"], cfg["invC9"])
+    nw, lst, val = B.templates(256)
+    assert np.array_equal(nw, ref_nw)
+    worst = 0.0
+    for e in range(0, 256, 16):
+        olst, ov, ovcr = oracle_templates(orc, ow, oorb, P[e])
+        assert np.array_equal(lst[e, :nw[e]], olst)
+        worst = max(worst, normwise(val[e, :, :nw[e]], ovcr))
+        assert normwise(val[e, :, :nw[e]], ov) <= 1e-12 + normwise(ov, ovcr)
+    print("batch template worst normwise (CR oracle)", worst)
+    assert worst < 1e-12
+    rel_dl = np.max(np.abs(dl - ref_dl) / np.maximum(np.abs(ref_dl), 1.0))
+    rel_sn = np.max(np.abs(sn - ref_sn) / ref_sn)
+    print("batch dlogL rel", rel_dl, "snr rel", rel_sn)
+    assert rel_dl < 1e-10 and rel_sn < 1e-10
+    B.close()
+
+
+def test_snr_and_logl_wavelet_dropins(orc, W, g2):
+    ow, gw, oorb, gorb = g2
+    cfg = synth.config2(n=4, seed=6)
+    d, ic9 = cfg["data"], cfg["invC9"]
+    lst, v = orc.ucb_waveform_wavelet(ow, oorb, T1, cfg["params"][0])
+    m = np.zeros_like(d)
+    m[:, lst] = v
+    s_ref = orc.snr_wavelet(m[0], m[1], m[2], ic9, lst)
+    s = W.snr_wavelet(m[0], m[1], m[2], ic9, lst)
+    assert abs(s - s_ref) < 1e-12 * s_ref
+    L_ref = orc.gaussian_log_likelihood_wavelet(ow, d, m, lst, ic9)
+    L = W.gaussian_log_likelihood_wavelet(gw, d, m, lst, ic9)
+    # rounding scale of the reference's XX + 2 XY over all pixels
+    scale = sum(np.sum(np.abs(d[a] * d[b] * ic9[3 * a + b])) for a in range(3) for b in range(3))
+    print("logL wavelet", L, L_ref, "diff / scale", abs(L - L_ref) / scale)
+    assert abs(L - L_ref) < 1e-12 * scale
+
+
+def test_logl_wavelet_well_conditioned(orc, W, g2):
+    ow, gw, oorb, gorb = g2
+    d, ic9 = synth.wdm_random_noise(ow.NF, ow.NT, ow.kmin, ow.kmax, 8)
+    p = synth.draw_params(np.random.default_rng(8), 1, T1)[0]
+    lst, v = orc.ucb_waveform_wavelet(ow, oorb, T1, p)
+    m = np.zeros_like(d)
+    m[:, lst] = 1e3 * v
+    L_ref = orc.gaussian_log_likelihood_wavelet(ow, d, m, lst, ic9)
+    L = W.gaussian_log_likelihood_wavelet(gw, d, m, lst, ic9)
+    assert abs(L - L_ref) < 1e-10 * abs(L_ref)
+
+
+def test_wavelet_bad_inputs(gl, W, g2):
+    from glass_amd import GlassHipError
+    ow, gw, oorb, gorb = g2
+    bad = W.Wdm.grid(4096, 1024, T1 / 1024)
+    with pytest.raises(GlassHipError):
+        W.ucb_waveform_wavelet(bad, gorb, T1, synth.draw_params(np.random.default_rng(1), 1, T1)[0])
+    with pytest.raises(GlassHipError):
+        X = np.zeros(8)
+        W.snr_wavelet(X, X, X, np.zeros((9, 8)), [0, 9])
