@@ -33,6 +33,13 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
 
+# HBM bytes per launch of the batched per-class kernels, measured with rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE on this workload and scaled by the same counters on a 256 MiB device copy (fetch x2.00,
+# write x1.00): profiles/eval_kernel_batched_fd_sweep.md.  Keyed by template bins N.
+MEASURED_TRAFFIC = {512: 2.3187e9, 1024: 1.8220e9, 2048: 2.3717e9, 256: 8.1357e8, 4096: 7.1843e8,
+                    128: 2.2562e8, 64: 6.550e7}
+
+
 def wave_flops(N):
     """Algorithmic FP64 work of one template of N bins plus its delta-logL:
     940 N + 30 N log2 N for the slow series, six FFTs and TDI (each sin/cos
@@ -146,6 +153,89 @@ def fisher_leg(gl, synth, n=8192, BW=256, W=512, steps=5, seed=3):
     return dict(value=n / (ms * 1e-3), ms=ms, plain=plain, n=n)
 
 
+def wdm_leg(gl, synth, n=4096, steps=5, seed=7, cpu_seconds=6.0, nthreads=0, parity_n=64, plain=True):
+    """Wavelet-domain case: n templates (config-0 distributions, Tobs 1 yr)
+    on the NT 512 x NF 4096 grid (15 s cadence), stationary XYZ pixel noise;
+    one launch = every template's ucb_waveform_wavelet pixels plus its
+    wavelet_nwip terms against the data (snr and the logL change).  Also one
+    full forward WDM transform of a 2^21-sample noise series, and the
+    per-call drop-in for comparison."""
+    import torch
+    from glass_amd import wdm as W
+    T = synth.YEAR_T
+    cfg = synth.config2(n=n, seed=seed)
+    w = W.Wdm.grid(cfg["NF"], cfg["NT"], cfg["dt"])
+    orb = W.interpolated_analytic_orbit(gl.analytic_orbit(), T)
+    B = W.WdmBatch(w, orb, T, cfg["data"], cfg["invC6"], max_templates=n)
+    P = torch.tensor(cfg["params"], device="cuda")
+    for _ in range(2):
+        B.eval(P)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        dl, sn = B.eval(P)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    dl, sn = dl.cpu().numpy(), sn.cpu().numpy()
+    nw, _, _ = B.templates(n)
+    layers = {int(L): int(np.sum(nw == L * cfg["NT"])) for L in (1, 2, 3)}
+    # full transform of a noise series on the device
+    ND = cfg["NF"] * cfg["NT"]
+    x = torch.randn(ND, dtype=torch.float64, device="cuda")
+    out = torch.empty_like(x)
+    W.wdm_transform(w, x, out)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(steps):
+        W.wdm_transform(w, x, out)
+    e1.record()
+    torch.cuda.synchronize()
+    tms = e0.elapsed_time(e1) / steps
+    # plain: the per-call drop-in (host arrays)
+    rate = None
+    if plain:
+        X = np.zeros(w.npix)
+        Y, Z = np.zeros_like(X), np.zeros_like(X)
+        nplain, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < 2.0:
+            lst, X, Y, Z = W.ucb_waveform_wavelet(w, orb, T, cfg["params"][nplain % n], X=X, Y=Y, Z=Z)
+            X[lst] = Y[lst] = Z[lst] = 0.0
+            nplain += 1
+        rate = nplain / (time.perf_counter() - t0)
+    res = dict(value=n / (ms * 1e-3), ms=ms, n=n, layers=layers, transform_ms=tms, ND=ND, plain=rate)
+    B.close()
+    # parity and CPU baseline with the oracle
+    try:
+        from oracle import orc
+    except Exception as exc:  # noqa: BLE001
+        log(f"wdm oracle unavailable ({exc}); no parity / cpu baseline")
+        return res
+    ow = orc.Wdm(cfg["NF"], cfg["NT"], cfg["dt"])
+    oorb = orc.iorbit(T)
+    idx = np.random.default_rng(seed + 1).choice(n, parity_n, replace=False)
+    with orc.cr_trig():
+        rdl, rsn, _, _ = orc.wdm_eval_batch(ow, oorb, T, cfg["params"][idx], cfg["data"], cfg["invC9"],
+                                            nthreads=nthreads)
+    res["parity"] = {"checked": int(parity_n),
+                     "max_dlogL_rel_err": float(np.max(np.abs(dl[idx] - rdl) / np.maximum(np.abs(rdl), 1.0))),
+                     "max_snr_rel_err": float(np.max(np.abs(sn[idx] - rsn) / rsn)), "gate": 1e-10}
+    xs = x.cpu().numpy()
+    ref = orc.wavelet_transform_timefreq(ow, xs)
+    res["transform_normwise_err"] = float(np.max(np.abs(out.cpu().numpy() - ref)) / np.max(np.abs(ref)))
+    if cpu_seconds > 0:
+        reps, tot, m = 0, 0.0, 0
+        sample = cfg["params"][:256]
+        while tot < cpu_seconds:
+            _, _, _, secs = orc.wdm_eval_batch(ow, oorb, T, sample, cfg["data"], cfg["invC9"], nthreads=nthreads)
+            tot += secs
+            reps += 1
+            m += len(sample)
+        res["cpu"] = dict(rate=m / tot, n=len(sample), reps=reps, secs=tot)
+    return res
+
+
 def oracle_leg(cfg, logL_gpu, seconds, nthreads, seed):
     """CPU baseline (the oracle restatement of the reference, OpenMP over
     proposals) on a bounded sample of the same workload, and the parity of
@@ -206,6 +296,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-fisher", action="store_true")
+    ap.add_argument("--no-wdm", action="store_true")
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="also run a 256 MiB device copy, the reference point that scales the "
                          "FETCH_SIZE/WRITE_SIZE counters of a profiled run")
@@ -272,7 +363,11 @@ def main():
     roofline = {"bound": "mfma", "pipe": "fp64 VALU (no MFMA: nothing on the path is a dense contraction)",
                 "kernel": f"eval_kernel<{Nd},{min(max(Nd, 64), 1024)}>", "achieved": round(ach_tf, 4),
                 "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(ach_tf / FP64_PEAK_TFLOPS, 5),
-                "traffic": None, "avg_launch_ms": round(avg_ms, 4), "evals_per_launch": int(kcount[kd]),
+                "traffic": MEASURED_TRAFFIC.get(Nd),
+                "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes of this workload, calibrated "
+                                  "on a 256 MiB copy; bytes per launch (L2/MALL reuse keeps it below the "
+                                  "algorithmic bytes)",
+                "avg_launch_ms": round(avg_ms, 4), "evals_per_launch": int(kcount[kd]),
                 "hbm_algorithmic_GBs": round(ach_gbs, 2), "hbm_frac": round(ach_gbs / HBM_PEAK_GBS, 5)}
 
     if args.copy_calibrate:
@@ -301,6 +396,32 @@ def main():
                             "ucb_fisher drop-in"}
         log(f"exact Fisher: {fl['value']:.4e} evals/s ({fl['ms']:.3f} ms / {fl['n']}), "
             f"plain FD ucb_fisher {fl['plain']:.1f}/s")
+
+    wdm = None
+    if not args.no_wdm:
+        nthr = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+        wl = wdm_leg(gl, synth, cpu_seconds=0.0 if args.no_cpu_baseline else args.cpu_seconds / 2,
+                     nthreads=nthr, plain=not args.no_plain)
+        byts = 16.0 * wl["ND"]
+        wdm = {"value": round(wl["value"], 1), "unit": "wavelet templates + wavelet_nwip evals/s",
+               "ms_per_launch": round(wl["ms"], 4), "templates": wl["n"], "layers_histogram": wl["layers"],
+               "plain_dropin": None if wl["plain"] is None else round(wl["plain"], 1),
+               "vs_plain": None if wl["plain"] is None else round(wl["value"] / wl["plain"], 1),
+               "transform": {"ND": wl["ND"], "ms": round(wl["transform_ms"], 4),
+                             "algorithmic_GBs": round(byts / (wl["transform_ms"] * 1e-3) / 1e9, 1),
+                             "hbm_frac": round(byts / (wl["transform_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                             "normwise_err_vs_oracle": wl.get("transform_normwise_err")},
+               "parity": wl.get("parity"),
+               "config": "Tobs 1 yr, NT 512 x NF 4096 (15 s cadence), 4096 templates (config-0 parameter "
+                         "distributions), stationary XYZ pixel noise; plain = per-call ucb_waveform_wavelet "
+                         "drop-in on host arrays"}
+        if "cpu" in wl:
+            wdm["cpu_baseline"] = {"value": round(wl["cpu"]["rate"], 1), "unit": "templates/s", "cores": nthr,
+                                   "kind": "port",
+                                   "sample": f"{wl['cpu']['n']} templates x {wl['cpu']['reps']} by the oracle "
+                                             f"restatement, OpenMP over templates, {wl['cpu']['secs']:.1f} s"}
+        log(f"wdm: {wl['value']:.4e} templates/s ({wl['ms']:.3f} ms / {wl['n']}), transform "
+            f"{wl['transform_ms']:.3f} ms, plain {wl['plain']}/s, parity {wl.get('parity')}")
 
     cpu = None
     parity = None
@@ -331,6 +452,7 @@ def main():
                                                        "drop-ins, host memory, BW 128, W 256"},
         "batched_vs_plain": None if plain is None else round(value / plain, 1),
         "fisher": fisher,
+        "wdm": wdm,
         "parity": parity,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -1,0 +1,55 @@
+"""Per-kernel summaries of rocprofv3 CSV output (the tables kept with the
+measurements).
+
+    python -m glass_amd.rocprof_summary counters run_counter_collection.csv
+    python -m glass_amd.rocprof_summary stats run_kernel_stats.csv
+
+counters: per kernel (template arguments kept, grid size in blocks shown so
+batched launches stand apart from per-call ones) the number of dispatches,
+the mean dispatch duration and the mean of every counter per dispatch
+(summed over the counter's instances).  stats: the kernel-trace summary
+(calls, mean and total duration) in a short table.
+"""
+import collections
+import csv
+import re
+import sys
+
+
+def _short(name):
+    m = re.search(r"glass::(?:wdm::)?(\w+(?:<[^()]*?>)?)", name)
+    if m:
+        return m.group(1).replace(" ", "")
+    return name.split("(")[0].replace("void ", "")[:60]
+
+
+def counters(path):
+    agg = collections.defaultdict(lambda: collections.defaultdict(float))
+    disp = collections.defaultdict(set)
+    dur = collections.defaultdict(float)
+    names = set()
+    for r in csv.DictReader(open(path)):
+        wg = max(int(r.get("Workgroup_Size", 1) or 1), 1)
+        key = (_short(r["Kernel_Name"]), int(r["Grid_Size"]) // wg)
+        agg[key][r["Counter_Name"]] += float(r["Counter_Value"])
+        names.add(r["Counter_Name"])
+        if r["Dispatch_Id"] not in disp[key]:
+            disp[key].add(r["Dispatch_Id"])
+            dur[key] += int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+    cols = sorted(names)
+    print("kernel | blocks | dispatches | mean_ms | " + " | ".join(cols))
+    for key in sorted(agg, key=lambda k: -dur[k]):
+        n = len(disp[key])
+        vals = " | ".join(f"{agg[key][c] / n:.4e}" for c in cols)
+        print(f"{key[0]} | {key[1]} | {n} | {dur[key] / n / 1e6:.4f} | {vals}")
+
+
+def stats(path):
+    print("kernel | calls | mean_us | total_ms | percent")
+    for r in csv.DictReader(open(path)):
+        print(f"{_short(r['Name'])} | {r['Calls']} | {float(r['AverageNs']) / 1e3:.2f} | "
+              f"{float(r['TotalDurationNs']) / 1e6:.3f} | {float(r['Percentage']):.2f}")
+
+
+if __name__ == "__main__":
+    {"counters": counters, "stats": stats}[sys.argv[1]](sys.argv[2])

@@ -20,14 +20,23 @@ def fm(tmp_path_factory):
                    'extern "C" void fm_sincos(const double* x, double* s, double* c, int n)'
                    '{ for (int i = 0; i < n; i++) glass::sincos_red(x[i], s + i, c + i); }\n'
                    'extern "C" void fm_sinc(const double* x, double* y, int n)'
-                   '{ for (int i = 0; i < n; i++) y[i] = glass::sinc_ratio(x[i]); }\n')
+                   '{ for (int i = 0; i < n; i++) y[i] = glass::sinc_ratio(x[i]); }\n'
+                   'extern "C" void fm_sincos_cr(const double* x, double* s, double* c, int n)'
+                   '{ for (int i = 0; i < n; i++) glass::sincos_cr(x[i], s + i, c + i); }\n'
+                   'extern "C" void fm_atan2_cr(const double* y, const double* x, double* r, int n)'
+                   '{ for (int i = 0; i < n; i++) r[i] = glass::atan2_cr(y[i], x[i]); }\n'
+                   'extern "C" void fm_exp_cr(const double* x, double* r, int n)'
+                   '{ for (int i = 0; i < n; i++) r[i] = glass::exp_cr(x[i]); }\n')
     so = d / "fm.so"
-    subprocess.run(["g++", "-O2", "-ffp-contract=off", "-shared", "-fPIC", "-I",
+    subprocess.run(["g++", "-O2", "-ffp-contract=off", "-std=c++17", "-shared", "-fPIC", "-I",
                     os.path.join(ROOT, "glass_amd", "csrc"), str(src), "-o", str(so)], check=True)
     L = ctypes.CDLL(str(so))
     dp = ctypes.POINTER(ctypes.c_double)
     L.fm_sincos.argtypes = [dp, dp, dp, ctypes.c_int]
     L.fm_sinc.argtypes = [dp, dp, ctypes.c_int]
+    L.fm_sincos_cr.argtypes = [dp, dp, dp, ctypes.c_int]
+    L.fm_atan2_cr.argtypes = [dp, dp, dp, ctypes.c_int]
+    L.fm_exp_cr.argtypes = [dp, dp, ctypes.c_int]
     return L, dp
 
 
@@ -69,3 +78,52 @@ def test_sinc_ratio(fm):
     L.fm_sinc(x.ctypes.data_as(dp), y.ctypes.data_as(dp), len(x))
     ref = np.where(x == 0, 1.0, np.sin(x) / np.where(x == 0, 1.0, x))
     assert ulps(y, ref, 0.0) <= 3
+
+
+# The correctly rounded variants used on the wavelet template's phase chain:
+# compared with values rounded from 150-bit mpmath evaluations.
+
+def _mp():
+    mp = pytest.importorskip("mpmath")
+    mp.mp.prec = 150
+    return mp
+
+
+@pytest.mark.parametrize("lo,hi", [(-7.0, 7.0), (1e5, 2e6), (-5e8, 5e8)])
+def test_sincos_cr_correctly_rounded(fm, lo, hi):
+    mp = _mp()
+    L, dp = fm
+    x = np.random.default_rng(int(abs(hi))).uniform(lo, hi, 1500)
+    s = np.empty_like(x)
+    c = np.empty_like(x)
+    L.fm_sincos_cr(x.ctypes.data_as(dp), s.ctypes.data_as(dp), c.ctypes.data_as(dp), len(x))
+    assert np.array_equal(s, [float(mp.sin(mp.mpf(v))) for v in x])
+    assert np.array_equal(c, [float(mp.cos(mp.mpf(v))) for v in x])
+
+
+@pytest.mark.parametrize("scale", [1.0, 1e-23, 1e5])
+def test_atan2_cr_correctly_rounded(fm, scale):
+    mp = _mp()
+    L, dp = fm
+    rng = np.random.default_rng(int(scale * 7) % 1000 + 1)
+    y = rng.standard_normal(1500) * scale
+    x = rng.standard_normal(1500) * scale
+    r = np.empty_like(x)
+    L.fm_atan2_cr(y.ctypes.data_as(dp), x.ctypes.data_as(dp), r.ctypes.data_as(dp), len(x))
+    assert np.array_equal(r, [float(mp.atan2(mp.mpf(a), mp.mpf(b))) for a, b in zip(y, x)])
+    z = np.array([0.0, -0.0, 1.0, -1.0])
+    for a in z:
+        for b in z:
+            L.fm_atan2_cr(np.array([a]).ctypes.data_as(dp), np.array([b]).ctypes.data_as(dp),
+                          r.ctypes.data_as(dp), 1)
+            assert r[0] == np.arctan2(a, b) and np.signbit(r[0]) == np.signbit(np.arctan2(a, b))
+
+
+@pytest.mark.parametrize("lo,hi", [(-55.0, -46.0), (-1.0, 1.0), (-700.0, 700.0)])
+def test_exp_cr_correctly_rounded(fm, lo, hi):
+    mp = _mp()
+    L, dp = fm
+    x = np.random.default_rng(int(abs(lo))).uniform(lo, hi, 1500)
+    r = np.empty_like(x)
+    L.fm_exp_cr(x.ctypes.data_as(dp), r.ctypes.data_as(dp), len(x))
+    assert np.array_equal(r, [float(mp.exp(mp.mpf(v))) for v in x])
