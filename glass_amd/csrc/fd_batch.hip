@@ -6,15 +6,18 @@
 // and evaluate the chain's likelihood and the template's snr.  The sweep is
 // a fixed launch sequence with no host synchronisation:
 //   1. zero the BW-class counters (a kernel, not a memset: graph-safe)
-//   2. align_kernel: ucb_alignment per proposal on the device; bucket the
-//      proposal into its BW class (32..1024) with an atomic counter
-//   3. per BW class, a persistent eval_kernel<N>: each workgroup takes
-//      proposals from its class list, builds the template in LDS (slow
-//      series, 6 FFTs, TDI) and reduces (r|dh) - (dh|dh)/2 over the union of
-//      the old and new supports only, reading the chain residual, the
-//      group's invC and the source's stored template with coalesced loads.
+//   2. align_kernel: ucb_alignment per proposal on the device; each block
+//      histograms its proposals into the 8 BW classes (32..4096) in LDS and
+//      reserves its slots with one atomic per class
+//   3. per BW class, an eval_kernel<N> launched over all proposals: block w
+//      takes entry w of its class list (blocks past the class count exit),
+//      builds the template in LDS (slow series, 6 FFTs, TDI) and reduces
+//      (r|dh) - (dh|dh)/2 over the union of the old and new supports only,
+//      reading the chain residual, the group's invC and the source's stored
+//      template with coalesced loads.
 // The full-window likelihood is therefore never re-read per proposal: the
-// HBM traffic per proposal is ~144 bytes per template bin.
+// algorithmic HBM traffic per proposal is ~144 bytes per template bin, and
+// proposals of one chain find their window in L2.
 #include <cmath>
 #include <cstring>
 #include <vector>

@@ -3,7 +3,7 @@
 // the TDI term table, and the per-template constants.
 //
 // Everything on the phase chain is evaluated with contraction off and in the
-// reference's operation order (utils/src/glass_math.c:100-190 splines,
+// reference's operation order (utils/src/glass_math.c:98-190 splines,
 // ucb/src/glass_ucb_waveform.c:331-362 barycentre phase), so a template's
 // phase samples of ~1e6 rad round exactly as on the CPU.
 #pragma once

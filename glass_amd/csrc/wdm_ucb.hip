@@ -1,5 +1,5 @@
 // wdm_ucb.hip -- galactic-binary templates in the WDM wavelet domain on
-// gfx950: ucb_waveform_wavelet (ucb/src/glass_ucb_waveform.c:1061-1190),
+// gfx950: ucb_waveform_wavelet (ucb/src/glass_ucb_waveform.c:1061-1240),
 // wavelet_nwip-based inner products, snr_wavelet and the wavelet
 // likelihood, as drop-ins and as a batched device API.
 //
@@ -52,7 +52,7 @@ __constant__ TdiTerm c_tdi[8] = {
 // ----------------------------------------------------- host: splines ----
 
 // Natural cubic spline coefficients c = y0|y1|y2|y3 (N each), the
-// recurrence of spline_coefficients (utils/src/glass_math.c:100-175).
+// recurrence of spline_coefficients (utils/src/glass_math.c:98-178).
 static void spline_host(int N, const double* x, const double* y, double* c)
 {
 #pragma clang fp contract(off)
@@ -682,7 +682,7 @@ __device__ void layer_body(const Geo& g, const Hdr& h, int e, const LayerArgs& a
     const int tsU = (three ? kTw3N : kTwN) / NDS;
     lds_fft<NDS, 1, kLayerThreads>(U, twU, tsU);
     lds_fft<M, 1, kLayerThreads>(V, twU, tsU * 2);
-    // pixel list (ucb/src/glass_ucb_waveform.c:1163-1178): candidates in
+    // pixel list (ucb/src/glass_ucb_waveform.c:1189-1212): candidates in
     // (time i, layer j) order, kept when kmin <= k < kmax
     int* lst = a.list + (size_t)e * 3 * NT;
     {
@@ -932,7 +932,7 @@ extern "C" int glass_hip_interpolated_analytic_orbit(const glass_hip_orbit* orbi
 {
 #pragma clang fp contract(off)
     if (!orbit || !Norb || Tobs <= 0) return set_error("interpolated_analytic_orbit: bad arguments"), GLASS_HIP_EINVAL;
-    // initialize_interpolated_analytic_orbits (utils/src/glass_lisa.c:271-322)
+    // initialize_interpolated_analytic_orbits (utils/src/glass_lisa.c:271-380)
     const double YEAR = 3.15581497632e7;
     int N = (int)(200 * Tobs / YEAR);
     if (N < 20) N = 20;

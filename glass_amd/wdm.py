@@ -1,7 +1,7 @@
 """Wavelet-domain (WDM) part of the UCB path on the GPU.
 
 Mirrors the reference's C interface (utils/src/glass_wavelet.c,
-ucb/src/glass_ucb_waveform.c:1061-1190, utils/src/glass_math.c:428-438,
+ucb/src/glass_ucb_waveform.c:1061-1240, utils/src/glass_math.c:428-438,
 utils/src/glass_model.c:805-855) over ``include/glass_hip_wdm.h``:
 
 * Wdm: the struct Wavelets fields the path reads (NF, NT, dt, df, A, kmin,

@@ -34,7 +34,7 @@ typedef struct glass_hip_wdm
 } glass_hip_wdm;
 
 /* Replaces void initialize_interpolated_analytic_orbits(struct Orbit *orbit,
- *   double Tobs, double t0) (utils/src/glass_lisa.c:271-322) for the fields
+ *   double Tobs, double t0) (utils/src/glass_lisa.c:271-380) for the fields
  * the wavelet path reads: *Norb = max(20, (int)(200 Tobs / YEAR)) knots
  * t[Norb] and the analytic spacecraft positions x, y, z (each 3 x Norb,
  * row i = spacecraft i).  With t == NULL only *Norb is returned. */
@@ -63,7 +63,7 @@ int glass_hip_wavelet_transform_inverse_freq(const glass_hip_wdm *wdm, const dou
 /* Replaces void ucb_waveform_wavelet(struct Orbit *orbit,
  *   struct Wavelets *wdm, double Tobs, double t0, double *params,
  *   int *wavelet_list, int *Nwavelet, double *X, double *Y, double *Z)
- * (ucb/src/glass_ucb_waveform.h:231, glass_ucb_waveform.c:1061-1190).
+ * (ucb/src/glass_ucb_waveform.h:231, glass_ucb_waveform.c:1061-1240).
  * The orbit is given by its Norb knots orbit_t and the spacecraft
  * positions orbit_x/y/z, each 3 x Norb (row i = spacecraft i; struct Orbit
  * t, x[i], y[i], z[i], utils/src/glass_lisa.h:60-109), as

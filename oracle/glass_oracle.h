@@ -170,7 +170,7 @@ void   orc_wavelet_transform_timefreq_by_layers(const orc_wdm *w, int jmin, int 
  * DFT PSD of ND/2+1 bins */
 void   orc_wdm_layer_var(const orc_wdm *w, const double *dft_psd, double *layer_var);
 
-/* natural cubic spline (utils/src/glass_math.c:100-190): c = y0|y1|y2|y3,
+/* natural cubic spline (utils/src/glass_math.c:98-190): c = y0|y1|y2|y3,
  * N doubles each; lookup even_sampled_search (even != 0) or binary_search
  * (utils/src/glass_math.c:393-467) */
 void   orc_spline_coefficients(int N, const double *x, const double *y, double *c);
@@ -178,7 +178,7 @@ double orc_spline_eval(int N, const double *x, const double *c, int even, double
 int    orc_binary_search(const double *a, int lo, int hi, double x);
 int    orc_even_search(const double *a, int lo, int hi, double x);
 
-/* initialize_interpolated_analytic_orbits (utils/src/glass_lisa.c:271-322):
+/* initialize_interpolated_analytic_orbits (utils/src/glass_lisa.c:271-380):
  * N = orc_iorbit_size(Tobs) knots t[N], positions pos[9][N] (spacecraft i,
  * coordinate d at row 3i+d) and their spline coefficients coef[9][4][N]. */
 int    orc_iorbit_size(double Tobs);
@@ -207,7 +207,7 @@ void   orc_ucb_wavelet_series(int Norb, const double *ot, const double *oc, doub
 void   orc_ucb_wavelet_response(int Norb, const double *ot, const double *oc, double Tobs, double t0,
                                 const double *params, double *phase_ssb, double *phase_sc,
                                 double *amp3, double *ph3);
-/* ucb_waveform_wavelet (ucb/src/glass_ucb_waveform.c:1061-1190) with the
+/* ucb_waveform_wavelet (ucb/src/glass_ucb_waveform.c:1061-1240) with the
  * orbit given as its knots/coefficients.  The compact form writes the
  * pixel values in list order (wX[n] belongs to pixel list[n]); the drop-in
  * form scatters them into X,Y,Z at list[n] as the reference does.  Both

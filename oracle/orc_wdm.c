@@ -13,9 +13,9 @@
  *   wavelet_transform_freq        utils/src/glass_wavelet.c:410-460
  *   wavelet_transform_timefreq_by_layers utils/src/glass_wavelet.c:544-611
  *   spline_coefficients, spline_interpolation, even_sampled_search,
- *   binary_search                 utils/src/glass_math.c:100-190,393-467
+ *   binary_search                 utils/src/glass_math.c:98-190,393-473
  *   initialize_interpolated_analytic_orbits (knots + position splines)
- *                                 utils/src/glass_lisa.c:271-322
+ *                                 utils/src/glass_lisa.c:271-380
  *   LISA_spacecraft_to_barycenter_time utils/src/glass_lisa.c:712-754
  *   hplus_and_hcross, LISA_TDI_spline, LISA_spline_response
  *                                 utils/src/glass_lisa.c:757-1074
@@ -24,7 +24,7 @@
  *   ucb_phase, ucb_amplitude, ucb_barycenter_waveform
  *                                 ucb/src/glass_ucb_waveform.c:331-362
  *   ucb_wavelet_layers, build_interpolated_waveform, ucb_waveform_wavelet
- *                                 ucb/src/glass_ucb_waveform.c:1003-1190
+ *                                 ucb/src/glass_ucb_waveform.c:1003-1240
  *   snr_wavelet                   utils/src/glass_math.c:428-438
  *   gaussian_log_likelhood_wavelet utils/src/glass_model.c:805-855
  *   stationary_dft_psd_to_wdm_layer_var / _psd utils/src/glass_wavelet.c:865-921
@@ -654,13 +654,13 @@ static double ucb_amplitude(double t, const double *p, double T)
 }
 
 /* The interpolated TDI time series of ucb_waveform_wavelet
- * (ucb/src/glass_ucb_waveform.c:1064-1160): SSB phase/amplitude splines on
+ * (ucb/src/glass_ucb_waveform.c:1064-1178): SSB phase/amplitude splines on
  * the orbit knots, spacecraft-1 phase, TDI amplitude/phase on Norb samples
  * and their splines, then N samples t_i = t0 + i*dt_s of
  * A(t) cos(Phi(t) - 2 pi fhet t_i) into out[3][N].  With N = NT(L+1),
  * dt_s = dt/(L+1), fhet = (jmin-1) df this is the heterodyned series the
  * waveform transforms; with N = NF*NT, dt_s = Tobs/N, fhet = 0 it is the
- * full-cadence series of ucb_waveform_wavelet_slow (:1243-1312). */
+ * full-cadence series of ucb_waveform_wavelet_slow (:1243-1356). */
 void orc_ucb_wavelet_series(int Norb, const double *ot, const double *oc, double Tobs, double t0,
                             const double *params, int N_ds, double dt_ds, double fhet, double *wave)
 {
