@@ -39,6 +39,10 @@ FP64_PEAK_TFLOPS = 78.6
 MEASURED_TRAFFIC = {512: 2.3187e9, 1024: 1.8220e9, 2048: 2.3717e9, 256: 8.1357e8, 4096: 7.1843e8,
                     128: 2.2562e8, 64: 6.550e7}
 
+# HBM bytes per launch of logl_chunk_kernel on the inner-product leg (rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE, same calibration): profiles/logl_chunk_kernel_batched_inner_product.md.
+INNER_TRAFFIC = 1.1350e9
+
 
 def wave_flops(N):
     """Algorithmic FP64 work of one template of N bins plus its delta-logL:
@@ -236,6 +240,51 @@ def wdm_leg(gl, synth, n=4096, steps=5, seed=7, cpu_seconds=6.0, nthreads=0, par
     return res
 
 
+def inner_leg(gl, synth, n=4096, steps=20, warmup=3, seed=11, parity_n=64):
+    """Batched full-window likelihood (glass_hip_logl_batch_eval): n
+    independent gaussian_log_likelihood items with the batched case's
+    windows (512-4096 bins, Tobs 2 yr), each with its own data, model and
+    invC, resident in HBM.  144 algorithmic bytes per bin (3 data + 3
+    model channels complex128, 6 invC planes).  Returns the launch time
+    from HIP events on the launch stream and oracle parity on a sample."""
+    import torch
+    from glass_amd.batch import LoglBatch
+    from oracle import orc
+
+    r = synth.logl_batch_inputs(n, seed=seed, device="cuda")
+    W, off = r["W"], r["off"]
+    bins = int(W.astype(np.int64).sum())
+    B = LoglBatch(W, off, off, off)
+    out = torch.empty(n, dtype=torch.float64, device="cuda")
+    for _ in range(warmup):
+        B.eval(r["data"], r["model"], r["invC"], out)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        B.eval(r["data"], r["model"], r["invC"], out)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    got = out.cpu().numpy()
+    rng = np.random.default_rng(seed + 1)
+    worst = 0.0
+    for e in rng.choice(n, min(parity_n, n), replace=False):
+        w, o = int(W[e]), int(off[e])
+        D = np.zeros((5, w), dtype=np.complex128)
+        M = np.zeros((5, w), dtype=np.complex128)
+        D[:3] = r["data"][3 * o:3 * (o + w)].cpu().numpy().reshape(3, w)
+        M[:3] = r["model"][3 * o:3 * (o + w)].cpu().numpy().reshape(3, w)
+        ic = r["invC"][6 * o:6 * (o + w)].cpu().numpy().reshape(6, w)
+        I9 = np.stack([ic[0], ic[3], ic[4], ic[3], ic[1], ic[5], ic[4], ic[5], ic[2]]).reshape(3, 3, w)
+        ref = orc.gaussian_log_likelihood(D, M, I9, 3)
+        worst = max(worst, abs(got[e] - ref) / abs(ref))
+    B.close()
+    del r
+    torch.cuda.empty_cache()
+    return dict(n=n, bins=bins, ms=ms, bytes=144.0 * bins, max_rel=worst, checked=min(parity_n, n))
+
+
 def oracle_leg(cfg, logL_gpu, seconds, nthreads, seed):
     """CPU baseline (the oracle restatement of the reference, OpenMP over
     proposals) on a bounded sample of the same workload, and the parity of
@@ -297,6 +346,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-fisher", action="store_true")
     ap.add_argument("--no-wdm", action="store_true")
+    ap.add_argument("--no-inner", action="store_true")
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="also run a 256 MiB device copy, the reference point that scales the "
                          "FETCH_SIZE/WRITE_SIZE counters of a profiled run")
@@ -423,6 +473,23 @@ def main():
         log(f"wdm: {wl['value']:.4e} templates/s ({wl['ms']:.3f} ms / {wl['n']}), transform "
             f"{wl['transform_ms']:.3f} ms, plain {wl['plain']}/s, parity {wl.get('parity')}")
 
+    inner = None
+    if not args.no_inner:
+        il = inner_leg(gl, synth)
+        gbs = il["bytes"] / (il["ms"] * 1e-3) / 1e9
+        inner = {"value": round(il["n"] / (il["ms"] * 1e-3), 1), "unit": "full-window logL evals/s",
+                 "ms_per_launch": round(il["ms"], 4), "items": il["n"], "bins": il["bins"],
+                 "roofline": {"bound": "hbm", "kernel": "logl_chunk_kernel + logl_finish_kernel",
+                              "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": INNER_TRAFFIC,
+                              "algorithmic_bytes": il["bytes"]},
+                 "parity": {"checked": il["checked"], "max_logL_rel_err": il["max_rel"], "gate": 1e-10,
+                            "pass": bool(il["max_rel"] <= 1e-10)},
+                 "config": "batched inner product: %d independent gaussian_log_likelihood items (3 channels), "
+                           "windows 512-4096 bins at Tobs 2 yr, own data/model/invC each, 144 B/bin" % il["n"]}
+        log(f"inner: {il['ms']:.4f} ms per launch of {il['n']} items / {il['bins']} bins, {gbs:.1f} GB/s, "
+            f"parity {il['max_rel']:.3e}")
+
     cpu = None
     parity = None
     if not args.no_cpu_baseline:
@@ -453,6 +520,7 @@ def main():
         "batched_vs_plain": None if plain is None else round(value / plain, 1),
         "fisher": fisher,
         "wdm": wdm,
+        "inner_product": inner,
         "parity": parity,
         "roofline": roofline,
         "cpu_baseline": cpu,

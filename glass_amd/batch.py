@@ -29,6 +29,9 @@ _EXTRA += [
     ("glass_hip_fd_batch_profile_read", [_vp, _dp, _ip, _ip], _i),
     ("glass_hip_ucb_bandwidth", [_d] * 7 + [_i], _i),
     ("glass_hip_ucb_alignment", [_vp, _d, _i, _i, _dp, _ip, _ip, _ip], _i),
+    ("glass_hip_logl_batch_create", [ctypes.POINTER(_vp), _i, _ip, _vp, _vp, _vp], _i),
+    ("glass_hip_logl_batch_destroy", [_vp], _i),
+    ("glass_hip_logl_batch_eval", [_vp, _vp, _vp, _vp, _vp, _vp], _i),
 ]
 
 
@@ -159,3 +162,48 @@ class FdBatch:
         check(_L().glass_hip_fd_batch_state(self.h, lg.ctypes.data_as(_dp), sp.ctypes.data_as(_dp)),
               "fd_batch_state")
         return lg, sp
+
+
+class LoglBatch:
+    """n independent gaussian_log_likelihood calls (Nchannel 3) in one launch.
+
+    Item e reads a window of W[e] bins: data and model as complex [3, W[e]]
+    at bin offsets data_off[e] / model_off[e] of flat complex128 device
+    buffers (3 W per item), invC as [6, W[e]] planes (XX YY ZZ XY XZ YZ) at
+    bin offset invC_off[e] of a flat float64 device buffer (6 W per item).
+    """
+
+    def __init__(self, W, data_off, model_off, invC_off):
+        W = np.ascontiguousarray(W, dtype=np.int32)
+        offs = [np.ascontiguousarray(o, dtype=np.int64) for o in (data_off, model_off, invC_off)]
+        assert all(len(o) == len(W) for o in offs)
+        self.n = len(W)
+        h = _vp()
+        check(_L().glass_hip_logl_batch_create(ctypes.byref(h), self.n, W.ctypes.data_as(_ip),
+                                               *(o.ctypes.data_as(_vp) for o in offs)), "logl_batch_create")
+        self.h = h
+
+    def eval(self, data, model, invC, out=None):
+        """data, model: complex128 (or float64 re/im) device tensors; invC:
+        float64 device tensor.  Returns logL[n] (device), on the current
+        torch stream."""
+        import torch
+        if out is None:
+            out = torch.empty(self.n, dtype=torch.float64, device=data.device)
+        for t in (data, model, invC, out):
+            assert t.is_cuda and t.is_contiguous()
+        s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        check(_L().glass_hip_logl_batch_eval(self.h, _ptr(data), _ptr(model), _ptr(invC), _ptr(out), s),
+              "logl_batch_eval")
+        return out
+
+    def close(self):
+        if getattr(self, "h", None):
+            _L().glass_hip_logl_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -222,3 +222,68 @@ def wdm_random_noise(NF, NT, kmin, kmax, seed):
     d = np.einsum("kij,kj->ik", np.linalg.cholesky(Cl)[layers], rng.standard_normal((len(layers), 3)))
     invC9 = np.ascontiguousarray(np.linalg.inv(Cl)[layers].reshape(-1, 9).T)
     return np.ascontiguousarray(d), invC9
+
+
+def logl_batch_inputs(n, seed=11, T=T_2YR, windows=(512, 1024, 2048, 4096), f_lo=0.5e-3, f_hi=20e-3,
+                      device="cpu"):
+    """n independent full-window likelihood items of the batched tempered
+    sampler's shape: window W_e drawn from `windows`, centred at f ~
+    U[f_lo, f_hi]; per bin the analytic 3x3 XYZ covariance C = a I + b J
+    (xyz_noise_cov: equal diagonals d and cross terms b, a = d - b), its
+    closed-form inverse (1/a)(I - b/(a + 3b) J), data = a noise draw and
+    model = data - (a second noise draw), so the residual has the noise
+    statistics of a model that fits.  Noise: n = sqrt(a/2)(u - mean(u))
+    + sqrt((a + 3b)/2) mean(u) for complex unit normals u (covariance
+    C/2 on the real and imaginary parts).
+
+    Generated with torch on `device`.  Returns dict W [n] int32, off [n]
+    int64 (bin offsets, identical for data, model and invC), data and
+    model complex128 [sum 3 W] (item e: X, Y, Z of W_e bins at
+    3 off[e]), invC float64 [sum 6 W] (6 planes XX YY ZZ XY XZ YZ at
+    6 off[e])."""
+    import torch
+    rng = np.random.default_rng(seed)
+    W = rng.choice(np.asarray(windows), n).astype(np.int32)
+    qmin = (rng.uniform(f_lo, f_hi, n) * T).astype(np.int64) - W // 2
+    off = np.zeros(n, dtype=np.int64)
+    off[1:] = np.cumsum(W[:-1].astype(np.int64))
+    nb = int(W.astype(np.int64).sum())
+    dev = torch.device(device)
+    Wt = torch.as_tensor(W, device=dev, dtype=torch.int64)
+    ot = torch.as_tensor(off, device=dev)
+    item = torch.repeat_interleave(torch.arange(n, device=dev), Wt)
+    j = torch.arange(nb, device=dev) - ot[item]
+    f = (torch.as_tensor(qmin, device=dev, dtype=torch.float64)[item] + j.to(torch.float64)) / T
+    Spm = 5.76e-30 / (PI2 * f * CLIGHT) / (PI2 * f * CLIGHT) * (1.0 + (0.4e-3 / f) ** 2) * (1.0 + (f / 8.0e-3) ** 4)
+    Sop = 1.28e-22 * (PI2 * f / CLIGHT) * (PI2 * f / CLIGHT) * (1.0 + (2.0e-3 / f) ** 4)
+    x = f / FSTAR
+    cx = torch.cos(x)
+    tr = torch.sin(x) ** 2
+    d = 16.0 * tr * (2.0 * (1.0 + cx * cx) * Spm + Sop) / 4.0
+    b = -8.0 * tr * cx * (4.0 * Spm + Sop) / 4.0
+    a = d - b
+    g = torch.Generator(device=dev)
+    g.manual_seed(int(seed))
+
+    def draw():
+        u = torch.complex(torch.randn((3, nb), generator=g, device=dev, dtype=torch.float64),
+                          torch.randn((3, nb), generator=g, device=dev, dtype=torch.float64))
+        m = u.mean(dim=0, keepdim=True)
+        return torch.sqrt(a / 2.0) * (u - m) + torch.sqrt((a + 3.0 * b) / 2.0) * m
+
+    # layout: item e, channel c, bin j -> c*W_e + j past 3*off_e (6*off_e for invC)
+    idx3 = 3 * ot[item] + j
+    data = torch.empty(3 * nb, dtype=torch.complex128, device=dev)
+    model = torch.empty_like(data)
+    nd, nr = draw(), draw()
+    for c in range(3):
+        data[idx3 + c * Wt[item]] = nd[c]
+        model[idx3 + c * Wt[item]] = nd[c] - nr[c]
+    k = b / (a + 3.0 * b)
+    idiag = (1.0 - k) / a
+    ioff = -k / a
+    invC = torch.empty(6 * nb, dtype=torch.float64, device=dev)
+    idx6 = 6 * ot[item] + j
+    for c in range(6):
+        invC[idx6 + c * Wt[item]] = idiag if c < 3 else ioff
+    return dict(W=W, off=off, data=data, model=model, invC=invC, qmin=qmin, T=T, seed=seed)

@@ -124,6 +124,28 @@ int glass_hip_fd_source_derivs(const glass_hip_orbit *orbit, const char *format,
                                long long invC_stride, int mode, double *d_logL, double *d_grad,
                                double *d_fisher, void *stream);
 
+/* Batched full-window likelihood: n independent calls of
+ *   double gaussian_log_likelihood(struct Data *data, struct Model *model)
+ * (utils/src/glass_model.h:219, utils/src/glass_model.c:726-774), Nchannel
+ * 3, in one launch (no reference counterpart as a batch).  Item e has a
+ * window of W[e] bins; its data occupy 3 channels (X, Y, Z) x 2 W[e]
+ * doubles at d_data + 6*data_off[e], its model the same at d_model +
+ * 6*model_off[e], its inverse covariance the 6 planes XX, YY, ZZ, XY, XZ,
+ * YZ of W[e] doubles at d_invC + 6*invC_off[e] (offsets in bins; items
+ * may share data or invC by sharing offsets).  The plan holds the item
+ * table (host arrays at create, copied) on the current device; an eval
+ * writes d_logL[e] = -1/2 chi^2 of item e.  Evals on one plan must not
+ * overlap (they share the plan's partial sums). */
+typedef struct glass_hip_logl_batch glass_hip_logl_batch;
+
+int glass_hip_logl_batch_create(glass_hip_logl_batch **plan, int n, const int *W,
+                                const long long *data_off, const long long *model_off,
+                                const long long *invC_off);
+int glass_hip_logl_batch_destroy(glass_hip_logl_batch *plan);
+int glass_hip_logl_batch_eval(glass_hip_logl_batch *plan, const double *d_data,
+                              const double *d_model, const double *d_invC, double *d_logL,
+                              void *stream);
+
 #ifdef __cplusplus
 }
 #endif
