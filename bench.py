@@ -347,6 +347,7 @@ def main():
     ap.add_argument("--no-fisher", action="store_true")
     ap.add_argument("--no-wdm", action="store_true")
     ap.add_argument("--no-inner", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default tiles (skip the autotuner)")
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="also run a 256 MiB device copy, the reference point that scales the "
                          "FETCH_SIZE/WRITE_SIZE counters of a profiled run")
@@ -370,6 +371,17 @@ def main():
     plan.set_proposals(cfg["p_chain"], cfg["p_src"], cfg["p_params"])
     torch.cuda.synchronize()
     log(f"inputs + plan: {time.perf_counter() - t0:.1f} s, {E} evaluations per step")
+
+    tune = None
+    if not args.no_tune:
+        t1 = time.perf_counter()
+        tiles, tms = plan.autotune(E, reps=3)
+        tune = {"tiles": {str(32 << k): {"threads": t[0], "grid_mult": t[1], "ms": round(float(tms[k]), 4)}
+                          for k, t in enumerate(tiles) if tms[k] > 0},
+                "seconds": round(time.perf_counter() - t1, 2)}
+        log(f"autotune {tune['seconds']} s: " + ", ".join(f"BW {32 << k}: {t[0]}x{t[1]}"
+                                                          for k, t in enumerate(tiles) if tms[k] > 0))
+    tiles_now = plan.tiles()
 
     for _ in range(args.warmup):
         plan.eval(E, graph=True)
@@ -411,7 +423,8 @@ def main():
     ach_tf = flops / (avg_ms * 1e-3) / 1e12
     ach_gbs = byts / (avg_ms * 1e-3) / 1e9
     roofline = {"bound": "mfma", "pipe": "fp64 VALU (no MFMA: nothing on the path is a dense contraction)",
-                "kernel": f"eval_kernel<{Nd},{min(max(Nd, 64), 1024)}>", "achieved": round(ach_tf, 4),
+                "kernel": f"eval_kernel<{Nd},{tiles_now[kd][0]},{'true' if tiles_now[kd][1] else 'false'}>",
+                "achieved": round(ach_tf, 4),
                 "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(ach_tf / FP64_PEAK_TFLOPS, 5),
                 "traffic": MEASURED_TRAFFIC.get(Nd),
                 "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes of this workload, calibrated "
@@ -513,6 +526,7 @@ def main():
                                "per chain, %d proposals per chain, Tobs 2 yr, windows 512-4096 bins" %
                                (args.groups, args.temps, args.props),
                    "evals_per_step": E, "chains": len(cfg["chain_group"]), "format": "sangria",
+                   "autotune": tune,
                    "graph_replay_ms_per_step": round(graph_ms, 4)},
         "plain": None if plain is None else {"value": round(plain, 1), "unit": "evals/s",
                                              "method": "per-call ucb_waveform + gaussian_log_likelihood "

@@ -15,7 +15,21 @@ _ip = ctypes.POINTER(ctypes.c_int)
 _vp = ctypes.c_void_p
 _i, _d = ctypes.c_int, ctypes.c_double
 
+
+
+class Tile(ctypes.Structure):
+    """glass_hip_tile: threads per workgroup and grid multiplier of one BW
+    class's eval kernel (0 = one workgroup per proposal slot)."""
+    _fields_ = [("nthr", ctypes.c_int), ("gmul", ctypes.c_int)]
+
+
+_tp = ctypes.POINTER(Tile)
+
 _EXTRA += [
+    ("glass_hip_fd_batch_tile_candidates", [_i, _tp, _i, _ip], _i),
+    ("glass_hip_fd_batch_get_tiles", [_vp, _tp], _i),
+    ("glass_hip_fd_batch_set_tiles", [_vp, _tp], _i),
+    ("glass_hip_fd_batch_autotune", [_vp, _i, _vp, _vp, _vp, _i, _tp, _dp, _vp], _i),
     ("glass_hip_fd_batch_create", [ctypes.POINTER(_vp), _vp, ctypes.c_char_p, _d, _d, _i, _ip, _ip, _dp, _dp,
                                    _i, _ip, _ip, _dp, _i], _i),
     ("glass_hip_fd_batch_destroy", [_vp], _i),
@@ -54,6 +68,18 @@ def ucb_alignment(orbit, T, NFFT, data_qmin, params):
                                        int(data_qmin), p.ctypes.data_as(_dp), ctypes.byref(bw),
                                        ctypes.byref(qm), ctypes.byref(im)), "ucb_alignment")
     return bw.value, qm.value, im.value
+
+
+NCLASS = 8   # BW classes 32 << k, k = 0..7
+
+
+def tile_candidates(k):
+    """Compiled (nthr, gmul) launch shapes of BW class k (BW = 32 << k),
+    the default first."""
+    out = (Tile * 64)()
+    n = ctypes.c_int()
+    check(_L().glass_hip_fd_batch_tile_candidates(int(k), out, 64, ctypes.byref(n)), "tile_candidates")
+    return [(out[i].nthr, out[i].gmul) for i in range(min(n.value, 64))]
 
 
 def pack_invC(invC33):
@@ -154,6 +180,27 @@ class FdBatch:
         check(_L().glass_hip_fd_batch_profile_read(self.h, ms.ctypes.data_as(_dp), la.ctypes.data_as(_ip),
                                                    cn.ctypes.data_as(_ip)), "fd_batch_profile_read")
         return ms, la, cn
+
+    def tiles(self):
+        """Current (nthr, gmul) per BW class."""
+        t = (Tile * NCLASS)()
+        check(_L().glass_hip_fd_batch_get_tiles(self.h, t), "fd_batch_get_tiles")
+        return [(x.nthr, x.gmul) for x in t]
+
+    def set_tiles(self, tiles):
+        t = (Tile * NCLASS)(*[Tile(int(a), int(b)) for a, b in tiles])
+        check(_L().glass_hip_fd_batch_set_tiles(self.h, t), "fd_batch_set_tiles")
+
+    def autotune(self, n, reps=3):
+        """Times every compiled tile of each BW class over the first n
+        proposals in the plan's buffers and keeps the fastest; returns
+        (tiles, ms per launch of each class's chosen tile)."""
+        t = (Tile * NCLASS)()
+        ms = np.zeros(NCLASS)
+        s = ctypes.c_void_p(self.torch.cuda.current_stream().cuda_stream)
+        check(_L().glass_hip_fd_batch_autotune(self.h, int(n), _ptr(self.chain), _ptr(self.src), _ptr(self.params),
+                                               int(reps), t, ms.ctypes.data_as(_dp), s), "fd_batch_autotune")
+        return [(x.nthr, x.gmul) for x in t], ms
 
     def state(self):
         self.torch.cuda.current_stream().synchronize()

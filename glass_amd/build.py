@@ -3,8 +3,11 @@
     python -m glass_amd.build        (or glass_amd.build.build())
 """
 import os
+import shutil
 import subprocess
 import sys
+import tempfile
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -13,7 +16,7 @@ SOURCES = ["runtime.hip", "fd_plain.hip", "nwip.hip", "fd_batch.hip", "fisher_fd
            "wdm_ucb.hip", "wdm_transform.hip", "logl_batch.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
+    "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
     # packed-FP32 instructions are kept out of every kernel
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
@@ -25,14 +28,32 @@ FLAGS = [
 EXTRA = os.environ.get("GLASS_HIP_EXTRA_FLAGS", "").split()
 
 
-def build(verbose=False):
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    tmp = LIB + ".tmp"
-    cmd = [HIPCC] + FLAGS + EXTRA + srcs + ["-o", tmp]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True)
-    os.replace(tmp, LIB)
+def build(verbose=False, jobs=None):
+    """Compiles every source to an object in parallel (a temporary folder),
+    then links libglass_hip.so next to this file."""
+    jobs = jobs or min(8, os.cpu_count() or 1)
+    tmpd = tempfile.mkdtemp(prefix="glass_hip_build_")
+    try:
+        def cc(src):
+            obj = os.path.join(tmpd, os.path.splitext(src)[0] + ".o")
+            cmd = [HIPCC] + FLAGS + EXTRA + ["-c", os.path.join(CSRC, src), "-o", obj]
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            r = subprocess.run(cmd, capture_output=True, text=True)
+            if r.returncode:
+                sys.stderr.write(r.stderr)
+                raise subprocess.CalledProcessError(r.returncode, cmd)
+            return obj
+        with ThreadPoolExecutor(jobs) as ex:
+            objs = list(ex.map(cc, SOURCES))
+        tmp = LIB + ".tmp"
+        cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-o", tmp]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        os.replace(tmp, LIB)
+    finally:
+        shutil.rmtree(tmpd, ignore_errors=True)
     return LIB
 
 

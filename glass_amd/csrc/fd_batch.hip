@@ -9,12 +9,15 @@
 //   2. align_kernel: ucb_alignment per proposal on the device; each block
 //      histograms its proposals into the 8 BW classes (32..4096) in LDS and
 //      reserves its slots with one atomic per class
-//   3. per BW class, an eval_kernel<N> launched over all proposals: block w
-//      takes entry w of its class list (blocks past the class count exit),
-//      builds the template in LDS (slow series, 6 FFTs, TDI) and reduces
-//      (r|dh) - (dh|dh)/2 over the union of the old and new supports only,
-//      reading the chain residual, the group's invC and the source's stored
-//      template with coalesced loads.
+//   3. per BW class, an eval_kernel<N, NTHR, LOOP> over all proposals,
+//      launched with the class's tile (threads per workgroup; one
+//      workgroup per proposal slot, or a multiple of the resident capacity
+//      striding over the class list), chosen by glass_hip_fd_batch_autotune
+//      from the compiled candidates: each workgroup takes an entry of its
+//      class list, builds the template in LDS (slow series, 6 FFTs, TDI)
+//      and reduces (r|dh) - (dh|dh)/2 over the union of the old and new
+//      supports only, reading the chain residual, the group's invC and the
+//      source's stored template with coalesced loads.
 // The full-window likelihood is therefore never re-read per proposal: the
 // algorithmic HBM traffic per proposal is ~144 bytes per template bin, and
 // proposals of one chain find their window in L2.
@@ -165,16 +168,13 @@ __device__ inline void load_w(const double* __restrict__ iC, int W, int j, doubl
     for (int q = 0; q < 6; q++) w[q] = iC[(size_t)q * W + j];
 }
 
-// Fused waveform + delta-logL + snr for the proposals of one BW class.
+// Fused waveform + delta-logL + snr of class-list entry w.
 template <int N, int NTHR>
-__global__ __launch_bounds__(NTHR) void eval_kernel(const int* __restrict__ list, const int* __restrict__ count,
-                                                    Items it, BatchDev B, const int* __restrict__ p_imin,
-                                                    double* __restrict__ logl_out, double* __restrict__ snr_out)
+__device__ __forceinline__ void eval_item(int w, double2* lds, const int* __restrict__ list, Items it,
+                                          BatchDev B, const int* __restrict__ p_imin,
+                                          double* __restrict__ logl_out, double* __restrict__ snr_out)
 {
     using S = FdShape<N, NTHR>;
-    __shared__ double2 lds[S::LDS];
-    const int cnt = *count;
-    GH_ITEM_LOOP
     {
         const int e = list[w];
         const int c = item_chain(it, e), s = item_src(it, e);
@@ -245,6 +245,39 @@ __global__ __launch_bounds__(NTHR) void eval_kernel(const int* __restrict__ list
         }
     }
 }
+
+// The proposals of one BW class.  LOOP = false: one workgroup per proposal
+// slot (grid = proposal count; workgroups past the class count exit at
+// once), so the body has no loop and nothing is hoisted across templates.
+// LOOP = true: any grid, each workgroup strides over the class list.
+template <int N, int NTHR, bool LOOP>
+__global__ __launch_bounds__(NTHR) void eval_kernel(const int* __restrict__ list, const int* __restrict__ count,
+                                                    Items it, BatchDev B, const int* __restrict__ p_imin,
+                                                    double* __restrict__ logl_out, double* __restrict__ snr_out)
+{
+    __shared__ double2 lds[FdShape<N, NTHR>::LDS];
+    const int cnt = *count;
+    if constexpr (LOOP)
+    {
+        for (int w = blockIdx.x; w < cnt; w += gridDim.x)
+            eval_item<N, NTHR>(w, lds, list, it, B, p_imin, logl_out, snr_out);
+    }
+    else
+    {
+        if (blockIdx.x < cnt) eval_item<N, NTHR>(blockIdx.x, lds, list, it, B, p_imin, logl_out, snr_out);
+    }
+}
+
+// Compiled launch shapes of the class kernels (tile candidates), default
+// first: (class k, template bins N, workgroup threads).  Workgroups of N
+// threads hold one slow sample per thread; fewer threads take several
+// samples and FFT butterflies each (more registers, fewer waves).  The
+// decimated classes (N >= 2048) keep their per-thread accumulators within
+// the register file only at 512 (and 1024 for N = 2048) threads.
+#define GH_EVAL_TILES(X)                                                                               \
+    X(0, 32, 64) X(1, 64, 64) X(2, 128, 128) X(2, 128, 64) X(3, 256, 256) X(3, 256, 128) X(3, 256, 64) \
+    X(4, 512, 512) X(4, 512, 256) X(4, 512, 128) X(5, 1024, 512) X(5, 1024, 1024) X(5, 1024, 256)     \
+    X(6, 2048, 512) X(6, 2048, 1024) X(7, 4096, 512)
 
 // Template of each listed item into dst slot (slot index = dslot[e] or e).
 template <int N, int NTHR>
@@ -400,8 +433,9 @@ struct glass_hip_fd_batch
     int *i_chain = nullptr, *i_src = nullptr, *i_slot = nullptr;
     double* i_params = nullptr;
     double2* hnew = nullptr;  // accept scratch [C][3][bwcap]
-    int grid[kNClass] = {0};
+    int ncu = 0;
     int tgrid[kNClass] = {0};
+    glass_hip_tile tile[kNClass] = {};  // launch shape of each class's eval kernel
     // graph cache
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -428,35 +462,69 @@ static int dalloc(glass_hip_fd_batch* P, T** p, size_t n)
 #define GH_NTHR_(N) (::glass::fd_threads(N))
 #define GH_NTHR(N) GH_NTHR_(N)
 
-// Persistent grid: workgroups resident per CU as bounded by LDS (160 KB)
-// and by 32 waves per CU, times the CU count.  A larger grid than actually
-// fits only delays some workgroups; the loop bounds stay correct.
-template <int N>
-static int occ_grid(const void* fn, int nthr, int* grid)
+// Workgroups of a class kernel resident per CU, as bounded by LDS (160 KB)
+// and by 32 waves per CU.
+static int resident_per_cu(int N, int nthr)
 {
-    (void)fn;
-    int dev = 0, ncu = 0;
-    GH_CHECK(hipGetDevice(&dev));
-    GH_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    const int lds = (int)sizeof(double2) * FdShape<N, GH_NTHR_(N)>::LDS + 512;
+    const int ns = N <= 1024 ? N : GH_DECIM_NS;
+    const int lds = (int)sizeof(double2) * 6 * ns + 512;
     int per = (160 * 1024) / lds;
-    int wv = 32 / ((nthr + 63) / 64);
+    const int wv = 32 / ((nthr + 63) / 64);
     if (wv < per) per = wv;
-    if (per < 1) per = 1;
-    *grid = ncu * per;
-    return GLASS_HIP_OK;
+    return per < 1 ? 1 : per;
 }
-
 
 static int setup_grids(glass_hip_fd_batch* P)
 {
-    int rc;
-#define GH_G(k, N)                                                                                          \
-    if ((rc = occ_grid<N>((const void*)eval_kernel<N, GH_NTHR(N)>, GH_NTHR(N), &P->grid[k]))) return rc;  \
-    if ((rc = occ_grid<N>((const void*)tmpl_kernel<N, GH_NTHR(N)>, GH_NTHR(N), &P->tgrid[k]))) return rc;
-    GH_G(0, 32) GH_G(1, 64) GH_G(2, 128) GH_G(3, 256) GH_G(4, 512) GH_G(5, 1024) GH_G(6, 2048) GH_G(7, 4096)
-#undef GH_G
+    GH_CHECK(hipDeviceGetAttribute(&P->ncu, hipDeviceAttributeMultiprocessorCount, P->device));
+    for (int k = 0; k < kNClass; k++)
+    {
+        const int N = 32 << k;
+        P->tgrid[k] = P->ncu * resident_per_cu(N, fd_threads(N));
+        P->tile[k] = glass_hip_tile{fd_threads(N), 0};
+    }
     return GLASS_HIP_OK;
+}
+
+// Grid multipliers of the tile candidates: 0 = one workgroup per proposal
+// slot, m > 0 = m x (resident workgroups per CU) x CUs, striding.
+constexpr int kGmul[] = {0, 1, 2, 4};
+constexpr int kNGmul = sizeof(kGmul) / sizeof(kGmul[0]);
+
+static bool tile_compiled(int k, int nthr)
+{
+#define GH_X(kk, N, T) \
+    if (k == kk && nthr == T) return true;
+    GH_EVAL_TILES(GH_X)
+#undef GH_X
+    return false;
+}
+
+static int tile_grid(const glass_hip_fd_batch* P, int k, glass_hip_tile t, int n)
+{
+    if (t.gmul <= 0) return n;
+    const long long g = (long long)t.gmul * P->ncu * resident_per_cu(32 << k, t.nthr);
+    return (int)(g < n ? g : n);
+}
+
+// The class-k eval kernel under tile t over the n proposal slots.
+static int launch_eval(glass_hip_fd_batch* P, int k, glass_hip_tile t, int n, const int* list, const int* cnt,
+                       const Items& it, double* logl_out, double* snr_out, hipStream_t s)
+{
+    const int g = tile_grid(P, k, t, n);
+#define GH_X(kk, N, T)                                                                                           \
+    if (k == kk && t.nthr == T)                                                                                  \
+    {                                                                                                            \
+        if (t.gmul > 0)                                                                                          \
+            eval_kernel<N, T, true><<<g, T, 0, s>>>(list, cnt, it, P->B, P->p_imin, logl_out, snr_out);          \
+        else                                                                                                     \
+            eval_kernel<N, T, false><<<g, T, 0, s>>>(list, cnt, it, P->B, P->p_imin, logl_out, snr_out);         \
+        return GLASS_HIP_OK;                                                                                     \
+    }
+    GH_EVAL_TILES(GH_X)
+#undef GH_X
+    set_error("batch: no compiled tile for class %d with %d threads", k, t.nthr);
+    return GLASS_HIP_EINVAL;
 }
 
 // alignment + per-class launches for items; mode 0: eval, 1: templates
@@ -476,22 +544,27 @@ static int launch_items(glass_hip_fd_batch* P, int n, const Items& it, int mode,
     {
         const int* list = P->cls_list + (size_t)k * P->cls_cap;
         const int* cnt = P->cls + k;
-        int g = GH_PERSISTENT ? (mode == 0 ? P->grid[k] : P->tgrid[k]) : n;
-        if (g > n) g = n;
         const bool pr = P->prof && mode == 0;
         if (pr) GH_CHECK(hipEventRecord(P->ev[k][0], s));
-#define GH_L(kk, N)                                                                                             \
-    case kk:                                                                                                    \
-        if (mode == 0)                                                                                          \
-            eval_kernel<N, GH_NTHR(N)><<<g, GH_NTHR(N), 0, s>>>(list, cnt, it, P->B, P->p_imin, logl_out, snr_out); \
-        else                                                                                                    \
-            tmpl_kernel<N, GH_NTHR(N)><<<g, GH_NTHR(N), 0, s>>>(list, cnt, it, P->B, dst, dslot);              \
-        break;
-        switch (k)
+        if (mode == 0)
         {
-            GH_L(0, 32) GH_L(1, 64) GH_L(2, 128) GH_L(3, 256) GH_L(4, 512) GH_L(5, 1024) GH_L(6, 2048) GH_L(7, 4096)
+            if (int rc = launch_eval(P, k, P->tile[k], n, list, cnt, it, logl_out, snr_out, s)) return rc;
         }
+        else
+        {
+            int g = GH_PERSISTENT ? P->tgrid[k] : n;
+            if (g > n) g = n;
+#define GH_L(kk, N)                                                                               \
+    case kk:                                                                                      \
+        tmpl_kernel<N, GH_NTHR(N)><<<g, GH_NTHR(N), 0, s>>>(list, cnt, it, P->B, dst, dslot);    \
+        break;
+            switch (k)
+            {
+                GH_L(0, 32) GH_L(1, 64) GH_L(2, 128) GH_L(3, 256) GH_L(4, 512) GH_L(5, 1024) GH_L(6, 2048)
+                GH_L(7, 4096)
+            }
 #undef GH_L
+        }
         if (pr) GH_CHECK(hipEventRecord(P->ev[k][1], s));
     }
     GH_CHECK(hipGetLastError());
@@ -558,6 +631,8 @@ static int check_err(glass_hip_fd_batch* P, hipStream_t s)
     }
     return GLASS_HIP_OK;
 }
+
+static int drop_graph(glass_hip_fd_batch* P);
 
 extern "C" int glass_hip_fd_batch_create(glass_hip_fd_batch** plan, const glass_hip_orbit* orbit, const char* format,
                                          double T, double t0, int ngroup, const int* group_qmin, const int* group_W,
@@ -750,10 +825,7 @@ extern "C" int glass_hip_fd_batch_eval_graph(glass_hip_fd_batch* P, int nprop, c
     for (int i = 0; i < 5 && same; i++) same = key[i] == P->g_key[i];
     if (!same)
     {
-        if (P->gexec) GH_CHECK(hipGraphExecDestroy(P->gexec));
-        if (P->graph) GH_CHECK(hipGraphDestroy(P->graph));
-        P->gexec = nullptr;
-        P->graph = nullptr;
+        if (int rc = drop_graph(P)) return rc;
         // capture on a private stream so the caller's stream may be the
         // legacy default stream
         hipStream_t cs;
@@ -863,4 +935,137 @@ extern "C" int glass_hip_fd_batch_state(glass_hip_fd_batch* P, double* logL, dou
     if (src_params)
         GH_CHECK(hipMemcpy(src_params, P->B.sparams, sizeof(double) * P->C * kMaxSrc * 8, hipMemcpyDeviceToHost));
     return GLASS_HIP_OK;
+}
+
+// ------------------------------------------------------------- tiles ---
+
+static int drop_graph(glass_hip_fd_batch* P)
+{
+    if (P->gexec) GH_CHECK(hipGraphExecDestroy(P->gexec));
+    if (P->graph) GH_CHECK(hipGraphDestroy(P->graph));
+    P->gexec = nullptr;
+    P->graph = nullptr;
+    P->g_nprop = -1;
+    return GLASS_HIP_OK;
+}
+
+extern "C" int glass_hip_fd_batch_tile_candidates(int k, glass_hip_tile* out, int max, int* n)
+{
+    if (k < 0 || k >= kNClass || !n || (max > 0 && !out))
+    {
+        set_error("fd_batch_tile_candidates: bad arguments");
+        return GLASS_HIP_EINVAL;
+    }
+    int c = 0;
+#define GH_X(kk, N, T)                                              \
+    if (k == kk)                                                    \
+        for (int m = 0; m < kNGmul; m++, c++)                       \
+            if (c < max) out[c] = glass_hip_tile{T, kGmul[m]};
+    GH_EVAL_TILES(GH_X)
+#undef GH_X
+    *n = c;
+    return GLASS_HIP_OK;
+}
+
+extern "C" int glass_hip_fd_batch_get_tiles(glass_hip_fd_batch* P, glass_hip_tile* tiles)
+{
+    if (!P || !tiles)
+    {
+        set_error("fd_batch_get_tiles: bad arguments");
+        return GLASS_HIP_EINVAL;
+    }
+    for (int k = 0; k < kNClass; k++) tiles[k] = P->tile[k];
+    return GLASS_HIP_OK;
+}
+
+extern "C" int glass_hip_fd_batch_set_tiles(glass_hip_fd_batch* P, const glass_hip_tile* tiles)
+{
+    if (!P || !tiles)
+    {
+        set_error("fd_batch_set_tiles: bad arguments");
+        return GLASS_HIP_EINVAL;
+    }
+    for (int k = 0; k < kNClass; k++)
+        if (!tile_compiled(k, tiles[k].nthr) || tiles[k].gmul < 0 || tiles[k].gmul > 64)
+        {
+            set_error("fd_batch_set_tiles: class %d (BW %d): no compiled tile with %d threads, grid x%d", k,
+                      32 << k, tiles[k].nthr, tiles[k].gmul);
+            return GLASS_HIP_EINVAL;
+        }
+    GH_CHECK(hipSetDevice(P->device));
+    if (int rc = drop_graph(P)) return rc;
+    for (int k = 0; k < kNClass; k++) P->tile[k] = tiles[k];
+    return GLASS_HIP_OK;
+}
+
+extern "C" int glass_hip_fd_batch_autotune(glass_hip_fd_batch* P, int nprop, const int* d_chain, const int* d_src,
+                                           const double* d_params, int reps, glass_hip_tile* best,
+                                           double* best_ms, void* stream)
+{
+    if (!P || nprop <= 0 || nprop > P->maxprop || !d_chain || !d_src || !d_params || reps < 1)
+    {
+        set_error("fd_batch_autotune: bad arguments (nprop %d, max %d, reps %d)", nprop, P ? P->maxprop : 0, reps);
+        return GLASS_HIP_EINVAL;
+    }
+    GH_CHECK(hipSetDevice(P->device));
+    hipStream_t s = (hipStream_t)stream;
+    Items it{d_chain, d_src, d_params, nullptr};
+    // class lists of these proposals, as an eval builds them
+    zero_int_kernel<<<1, 64, 0, s>>>(P->cls, kNClass + 1);
+    align_kernel<<<(nprop + 255) / 256, 256, 0, s>>>(nprop, it, P->B, P->p_bw, P->p_imin, P->cls, P->cls_list,
+                                                     P->cls_cap, P->cls + kNClass);
+    GH_CHECK(hipGetLastError());
+    int cnt[kNClass + 1];
+    GH_CHECK(hipMemcpyAsync(cnt, P->cls, sizeof(cnt), hipMemcpyDeviceToHost, s));
+    GH_CHECK(hipStreamSynchronize(s));
+    if (cnt[kNClass])
+    {
+        set_error("fd_batch_autotune: %d proposals with BW outside 32..%d", cnt[kNClass], P->B.bwcap);
+        return GLASS_HIP_EINVAL;
+    }
+    if (int rc = drop_graph(P)) return rc;
+    hipEvent_t e0, e1;
+    GH_CHECK(hipEventCreate(&e0));
+    GH_CHECK(hipEventCreate(&e1));
+    int rc = GLASS_HIP_OK;
+    for (int k = 0; k < kNClass && !rc; k++)
+    {
+        double bms = 0.0;
+        if (cnt[k] > 0)
+        {
+            glass_hip_tile cand[16];
+            int nc = 0;
+            glass_hip_fd_batch_tile_candidates(k, cand, 16, &nc);
+            const int* list = P->cls_list + (size_t)k * P->cls_cap;
+            const int* cntp = P->cls + k;
+            bms = 1e300;
+            for (int c = 0; c < nc && c < 16 && !rc; c++)
+            {
+                // one untimed launch, then reps timed ones; results go to
+                // the plan's own logL/snr buffers
+                if ((rc = launch_eval(P, k, cand[c], nprop, list, cntp, it, P->p_logl, P->p_snr, s))) break;
+                if (hipEventRecord(e0, s) != hipSuccess) rc = GLASS_HIP_ERUNTIME;
+                for (int r = 0; r < reps && !rc; r++)
+                    rc = launch_eval(P, k, cand[c], nprop, list, cntp, it, P->p_logl, P->p_snr, s);
+                float ms = 0.f;
+                if (!rc && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                            hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                {
+                    set_error("fd_batch_autotune: timing failed");
+                    rc = GLASS_HIP_ERUNTIME;
+                }
+                if (!rc && ms / reps < bms)
+                {
+                    bms = ms / reps;
+                    P->tile[k] = cand[c];
+                }
+            }
+        }
+        if (best) best[k] = P->tile[k];
+        if (best_ms) best_ms[k] = bms;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (!rc && hipGetLastError() != hipSuccess) rc = GLASS_HIP_ERUNTIME;
+    return rc;
 }

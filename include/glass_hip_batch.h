@@ -99,6 +99,41 @@ int glass_hip_fd_batch_profile(glass_hip_fd_batch *plan, int on);
 int glass_hip_fd_batch_profile_read(glass_hip_fd_batch *plan, double *ms, int *launches,
                                     int *count);
 
+/* Launch shape ("tile") of one BW class's fused eval kernel (no reference
+ * counterpart): nthr threads per workgroup (one template per workgroup;
+ * fewer threads than template bins take several bins each), and the grid:
+ * gmul 0 = one workgroup per proposal slot, gmul > 0 = gmul x (workgroups
+ * resident per CU) x CUs workgroups striding over the class's proposals.
+ * Class k covers BW = 32 << k, k = 0..7.  Every tile gives the same
+ * results up to the order of the final sums (~1e-16 relative). */
+typedef struct glass_hip_tile
+{
+    int nthr;
+    int gmul;
+} glass_hip_tile;
+
+/* The compiled tiles of class k: writes up to max of them to out and their
+ * number to *n. */
+int glass_hip_fd_batch_tile_candidates(int k, glass_hip_tile *out, int max, int *n);
+
+/* The plan's current tiles (tiles[8]). */
+int glass_hip_fd_batch_get_tiles(glass_hip_fd_batch *plan, glass_hip_tile *tiles);
+
+/* Sets all eight tiles (each a compiled candidate of its class, e.g. from
+ * an earlier autotune); drops the captured graph. */
+int glass_hip_fd_batch_set_tiles(glass_hip_fd_batch *plan, const glass_hip_tile *tiles);
+
+/* Autotuner: buckets the nprop proposals (device arrays, as for an eval)
+ * into BW classes, times every compiled tile of each non-empty class over
+ * them (one untimed launch, then reps timed launches, HIP events on
+ * stream) and keeps the fastest per class in the plan.  Returns the chosen
+ * tiles (best[8], may be NULL) and their mean launch times in ms
+ * (best_ms[8], 0 for empty classes; may be NULL).  Synchronous; overwrites
+ * the plan's own logL/snr buffers; drops the captured graph. */
+int glass_hip_fd_batch_autotune(glass_hip_fd_batch *plan, int nprop, const int *d_chain,
+                                const int *d_src, const double *d_params, int reps,
+                                glass_hip_tile *best, double *best_ms, void *stream);
+
 /* Copies the chains' current logL (host array of nchain) and, if not
  * NULL, their current source parameters (nchain*8*8). Synchronous. */
 int glass_hip_fd_batch_state(glass_hip_fd_batch *plan, double *logL, double *src_params);
