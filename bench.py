@@ -208,7 +208,27 @@ def wdm_leg(gl, synth, n=4096, steps=5, seed=7, cpu_seconds=6.0, nthreads=0, par
             X[lst] = Y[lst] = Z[lst] = 0.0
             nplain += 1
         rate = nplain / (time.perf_counter() - t0)
-    res = dict(value=n / (ms * 1e-3), ms=ms, n=n, layers=layers, transform_ms=tms, ND=ND, plain=rate)
+    # wavelet Fisher (ucb_fisher_wavelet before the eigensolve): n / 16
+    # sources per launch (16 templates each) vs the per-call drop-in
+    nf = n // 16
+    Pf = P[:nf].contiguous()
+    B.fisher(Pf)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(steps):
+        B.fisher(Pf)
+    e1.record()
+    torch.cuda.synchronize()
+    fms = e0.elapsed_time(e1) / steps
+    frate = None
+    if plain:
+        nfp, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < 1.0:
+            W.ucb_fisher_wavelet(w, orb, T, cfg["params"][nfp % n], cfg["invC9"])
+            nfp += 1
+        frate = nfp / (time.perf_counter() - t0)
+    res = dict(value=n / (ms * 1e-3), ms=ms, n=n, layers=layers, transform_ms=tms, ND=ND, plain=rate,
+               fisher=dict(n=nf, ms=fms, value=nf / (fms * 1e-3), plain=frate))
     B.close()
     # parity and CPU baseline with the oracle
     try:
@@ -474,6 +494,11 @@ def main():
                              "algorithmic_GBs": round(byts / (wl["transform_ms"] * 1e-3) / 1e9, 1),
                              "hbm_frac": round(byts / (wl["transform_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                              "normwise_err_vs_oracle": wl.get("transform_normwise_err")},
+               "fisher": {"value": round(wl["fisher"]["value"], 1), "unit": "wavelet FD Fisher evals/s",
+                          "sources_per_launch": wl["fisher"]["n"], "ms_per_launch": round(wl["fisher"]["ms"], 4),
+                          "plain_dropin": None if wl["fisher"]["plain"] is None else round(wl["fisher"]["plain"], 1),
+                          "method": "ucb_fisher_wavelet central differences: 16 templates + 36-element "
+                                    "nine-term Gram per source"},
                "parity": wl.get("parity"),
                "config": "Tobs 1 yr, NT 512 x NF 4096 (15 s cadence), 4096 templates (config-0 parameter "
                          "distributions), stationary XYZ pixel noise; plain = per-call ucb_waveform_wavelet "

@@ -869,6 +869,214 @@ static int set_lds_limits()
     return rc;
 }
 
+// ------------------------------------------------ K3: wavelet Fisher --
+//
+// ucb_fisher_wavelet (ucb/src/glass_ucb_waveform.c:524-696) before the
+// eigensolve, for sources whose 16 perturbed templates (slot 16 s + 2 i:
+// params[i] + eps/2, slot + 1: params[i] - eps/2) are already in the
+// template buffers.  One workgroup per source: the layer band the 16 pixel
+// lists span becomes a dense local window of NT x (band) pixels in scratch;
+// per parameter the minus template is scattered into a zeroed window and
+// the derivative (h+ - h-)/eps written at the plus template's pixels with
+// time index strictly inside (imin, imax); the 36 Fisher elements are then
+// reduced over the window with the nine invC terms per pixel.
+
+constexpr int kFishThreads = 256;
+constexpr int kFishMaxBand = 6;  // layers the 16 templates of a source may span
+
+struct FishArgs
+{
+    const int* nwave;      // [16 n]
+    const int* list;       // [16 n][3 NT]
+    const double* values;  // [16 n][3][3 NT]
+    const double* invC;    // plane p of channel pair (a,b) = pmap[3a+b], element k - koff
+    long long pstride;     // doubles between planes
+    int koff, klen;        // data index of element 0; elements per plane
+    int pmap[9];
+    int npix;
+    double* D;             // scratch [n][8][3][kFishMaxBand NT]
+    double* M;             // scratch [n][3][kFishMaxBand NT]
+    double* fisher;        // [n][64]
+};
+
+__device__ inline double fisher_terms(const double (&u)[3], const double (&v)[3], const double (&C)[9])
+{
+    // XX, YY, ZZ, XY, XZ, YZ, YX, ZX, ZY (glass_ucb_waveform.c:608-617)
+    double f = u[0] * v[0] * C[0];
+    f += u[1] * v[1] * C[4];
+    f += u[2] * v[2] * C[8];
+    f += u[0] * v[1] * C[1];
+    f += u[0] * v[2] * C[2];
+    f += u[1] * v[2] * C[5];
+    f += u[1] * v[0] * C[3];
+    f += u[2] * v[0] * C[6];
+    f += u[2] * v[1] * C[7];
+    return f;
+}
+
+__global__ void __launch_bounds__(kFishThreads) fisher_kernel(Geo g, int n, FishArgs a)
+{
+    const int s = blockIdx.x;
+    if (s >= n) return;
+    const int NT = g.NT, tid = threadIdx.x, cap = 3 * NT;
+    const double invstep = 1. / 1.0e-6;
+    __shared__ int band[2];
+    __shared__ double red[36][kFishThreads / 64];
+    if (tid == 0)
+    {
+        band[0] = 0x7fffffff;
+        band[1] = -1;
+    }
+    __syncthreads();
+    int jlo = 0x7fffffff, jhi = -1;
+    for (int t = 0; t < 16; t++)
+    {
+        const int e = 16 * s + t;
+        const int nw = a.nwave[e];
+        const int* L = a.list + (size_t)e * cap;
+        for (int m = tid; m < nw; m += kFishThreads)
+        {
+            const int j = (L[m] + g.kmin) / NT;
+            jlo = min(jlo, j);
+            jhi = max(jhi, j);
+        }
+    }
+    atomicMin(&band[0], jlo);
+    atomicMax(&band[1], jhi);
+    __syncthreads();
+    jlo = band[0];
+    jhi = band[1];
+    double* F = a.fisher + (size_t)s * 64;
+    if (jhi < jlo || jhi - jlo + 1 > kFishMaxBand)
+    {
+        // no pixels: a zero matrix; a band too wide: NaN marks the failure
+        const double v = jhi < jlo ? 0.0 : __builtin_nan("");
+        for (int q = tid; q < 64; q += kFishThreads) F[q] = v;
+        return;
+    }
+    const int W = (jhi - jlo + 1) * NT;
+    double* D = a.D + (size_t)s * 24 * kFishMaxBand * NT;
+    double* M = a.M + (size_t)s * 3 * kFishMaxBand * NT;
+    for (int q = tid; q < 24 * W; q += kFishThreads) D[q] = 0.0;
+    const int base = jlo * NT;
+    for (int i = 0; i < 8; i++)
+    {
+        const int ep = 16 * s + 2 * i, em = ep + 1;
+        for (int q = tid; q < 3 * W; q += kFishThreads) M[q] = 0.0;
+        __syncthreads();
+        {
+            const int nw = a.nwave[em];
+            const int* L = a.list + (size_t)em * cap;
+            const double* V = a.values + (size_t)em * 3 * cap;
+            for (int m = tid; m < nw; m += kFishThreads)
+            {
+                const int q = L[m] + g.kmin - base;
+                for (int ch = 0; ch < 3; ch++) M[ch * W + q] = V[ch * cap + m];
+            }
+        }
+        __syncthreads();
+        {
+            const int nw = a.nwave[ep];
+            const int* L = a.list + (size_t)ep * cap;
+            const double* V = a.values + (size_t)ep * 3 * cap;
+            for (int m = tid; m < nw; m += kFishThreads)
+            {
+                const int k = L[m];
+                const int it = (k + g.kmin) % NT;
+                if (it > g.imin && it < g.imax && k >= 0 && k < a.npix)
+                {
+                    const int q = k + g.kmin - base;
+                    for (int ch = 0; ch < 3; ch++)
+                        D[(i * 3 + ch) * W + q] = (V[ch * cap + m] - M[ch * W + q]) * invstep;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    double acc[36];
+#pragma unroll
+    for (int p = 0; p < 36; p++) acc[p] = 0.0;
+    bool miss = false;
+    for (int q = tid; q < W; q += kFishThreads)
+    {
+        const int k = base + q - g.kmin;
+        if (k < 0 || k >= a.npix) continue;
+        double d[8][3];
+        bool any = false;
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+#pragma unroll
+            for (int ch = 0; ch < 3; ch++)
+            {
+                d[i][ch] = D[(i * 3 + ch) * W + q];
+                any |= d[i][ch] != 0.0;
+            }
+        if (!any) continue;
+        const int ki = k - a.koff;
+        if (ki < 0 || ki >= a.klen)
+        {
+            miss = true;
+            continue;
+        }
+        double C[9];
+#pragma unroll
+        for (int p = 0; p < 9; p++) C[p] = a.invC[a.pmap[p] * a.pstride + ki];
+        int idx = 0;
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+#pragma unroll
+            for (int j = i; j < 8; j++) acc[idx++] += fisher_terms(d[i], d[j], C);
+    }
+    const int lane = tid & 63, wv = tid >> 6;
+#pragma unroll
+    for (int p = 0; p < 36; p++)
+    {
+        const double v = wave_sum(acc[p]);
+        if (lane == 0) red[p][wv] = v;
+    }
+    __shared__ int missed;
+    if (tid == 0) missed = 0;
+    __syncthreads();
+    if (miss) atomicOr(&missed, 1);
+    __syncthreads();
+    if (tid < 36)
+    {
+        double f = 0.0;
+        for (int w = 0; w < kFishThreads / 64; w++) f += red[tid][w];
+        if (f != f) f = 10.0;
+        if (missed) f = __builtin_nan("");
+        // element tid of the upper triangle, row-major
+        int i = 0, r = tid;
+        while (r >= 8 - i)
+        {
+            r -= 8 - i;
+            i++;
+        }
+        const int j = i + r;
+        F[i * 8 + j] = f;
+        F[j * 8 + i] = f;
+    }
+}
+
+// The 16 perturbed parameter vectors of each of n sources (slot 16 s + 2 i
+// and + 1), as ucb_fisher_wavelet forms them (glass_ucb_waveform.c:556-576).
+__global__ void fisher_params_kernel(int n, const double* __restrict__ params, double* __restrict__ out)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= 16 * n) return;
+    const int s = q / 16, t = q % 16, i = t / 2, sgn = t % 2;
+    const double epsilon = 1.0e-6;
+    double* o = out + (size_t)q * 8;
+    for (int k = 0; k < 8; k++) o[k] = params[(size_t)s * 8 + k];
+    if (sgn == 0) o[i] += epsilon / 2.;
+    else o[i] -= epsilon / 2.;
+    if (i == 1 || i == 4)
+    {
+        if (sgn == 0 && o[i] > 1.0) o[i] = 1.0;
+        if (sgn == 1 && o[i] < -1.0) o[i] = -1.0;
+    }
+}
+
 // -------------------------------------------------- geometry cache ------
 
 struct GeoKey
@@ -1020,6 +1228,12 @@ struct glass_hip_wdm_batch
     int* list = nullptr;
     double* values = nullptr;
     int* nwave = nullptr;
+    // wavelet Fisher: perturbed parameters [max_templates x 8], derivative
+    // and minus-template windows [max_templates / 16 sources] (allocated on
+    // first use)
+    double* fparams = nullptr;
+    double* fD = nullptr;
+    double* fM = nullptr;
 };
 
 extern "C" int glass_hip_wdm_batch_create(glass_hip_wdm_batch** plan, const glass_hip_wdm* w, int Norb,
@@ -1074,7 +1288,7 @@ extern "C" int glass_hip_wdm_batch_destroy(glass_hip_wdm_batch* P)
     if (!P) return GLASS_HIP_OK;
     (void)hipSetDevice(P->device);
     for (void* p : {(void*)P->data, (void*)P->invC, (void*)P->scratch, (void*)P->list, (void*)P->values,
-                    (void*)P->nwave})
+                    (void*)P->nwave, (void*)P->fparams, (void*)P->fD, (void*)P->fM})
         if (p) (void)hipFree(p);
     delete P;
     return GLASS_HIP_OK;
@@ -1132,5 +1346,153 @@ extern "C" int glass_hip_wdm_batch_scratch(glass_hip_wdm_batch* P, int e, double
     GH_CHECK(hipSetDevice(P->device));
     GH_CHECK(hipDeviceSynchronize());
     GH_CHECK(hipMemcpy(out, P->scratch + ns * (size_t)e, sizeof(double) * ns, hipMemcpyDeviceToHost));
+    return GLASS_HIP_OK;
+}
+
+// ------------------------------------------------------ wavelet Fisher --
+
+extern "C" int glass_hip_ucb_fisher_wavelet(const glass_hip_wdm* w, int Norb, const double* orbit_t,
+                                            const double* orbit_x, const double* orbit_y, const double* orbit_z,
+                                            double Tobs, double t0, const double* params, const double* invC, int N,
+                                            double* fisher)
+{
+    if (!w || !orbit_t || !orbit_x || !orbit_y || !orbit_z || !params || !invC || !fisher)
+        return set_error("ucb_fisher_wavelet: NULL argument"), GLASS_HIP_EINVAL;
+    if (N != w->kmax - w->kmin)
+        return set_error("ucb_fisher_wavelet: N = %d but kmax - kmin = %d", N, w->kmax - w->kmin), GLASS_HIP_EINVAL;
+    ThreadCtx* c;
+    int rc = ctx_get(&c);
+    if (rc) return rc;
+    const GeoHost* G;
+    if ((rc = geo_get(w, Norb, orbit_t, orbit_x, orbit_y, orbit_z, Tobs, t0, &G))) return rc;
+    const int NT = w->NT;
+    // the 16 perturbed parameter vectors (glass_ucb_waveform.c:556-576) and
+    // the layer band their templates cover (ucb_wavelet_layers, the rule the
+    // template kernels apply)
+    double p16[16 * 8];
+    int jlo = 1 << 30, jhi = -(1 << 30);
+    for (int q = 0; q < 16; q++)
+    {
+        const int i = q / 2, sgn = q % 2;
+        const double epsilon = 1.0e-6;
+        double* o = p16 + 8 * q;
+        std::memcpy(o, params, sizeof(double) * 8);
+        if (sgn == 0) o[i] += epsilon / 2.;
+        else o[i] -= epsilon / 2.;
+        if (i == 1 || i == 4)
+        {
+            if (sgn == 0 && o[i] > 1.0) o[i] = 1.0;
+            if (sgn == 1 && o[i] < -1.0) o[i] = -1.0;
+        }
+        int js, jw;
+        wavelet_layers(o[0], o[7], Tobs, w->df, w->A, &js, &jw);
+        jlo = std::min(jlo, js);
+        jhi = std::max(jhi, js + jw - 1);
+    }
+    const int nb = jhi - jlo + 1;
+    if (nb < 1 || nb > kFishMaxBand)
+        return set_error("ucb_fisher_wavelet: templates span %d layers (at most %d)", nb, kFishMaxBand),
+               GLASS_HIP_EINVAL;
+    const int klen = nb * NT, koff = jlo * NT - w->kmin;
+    // device: params 128 | scratch | values | invC band 9 klen | D | M | fisher 64 | list | nwave
+    const size_t nscr = scratch_doubles(Norb);
+    const size_t nD = 24 * (size_t)kFishMaxBand * NT, nM = 3 * (size_t)kFishMaxBand * NT;
+    const size_t nd = 128 + 16 * nscr + 16 * 9 * (size_t)NT + 9 * (size_t)klen + nD + nM + 64;
+    const size_t bytes = sizeof(double) * nd + sizeof(int) * (16 * 3 * (size_t)NT + 16);
+    const size_t hbytes = sizeof(double) * (128 + 9 * (size_t)klen + 64);
+    if ((rc = c->dbuf.ensure(bytes)) || (rc = c->hbuf.ensure(hbytes))) return rc;
+    double* dp = (double*)c->dbuf.p;
+    double* dscr = dp + 128;
+    double* dval = dscr + 16 * nscr;
+    double* dinv = dval + 16 * 9 * (size_t)NT;
+    double* dD = dinv + 9 * (size_t)klen;
+    double* dM = dD + nD;
+    double* dF = dM + nM;
+    int* dlist = (int*)(dF + 64);
+    int* dnw = dlist + 16 * 3 * NT;
+    double* hp = (double*)c->hbuf.p;
+    std::memcpy(hp, p16, sizeof(p16));
+    double* hinv = hp + 128;
+    for (int p = 0; p < 9; p++)
+        for (int q = 0; q < klen; q++)
+        {
+            const long long k = (long long)koff + q;
+            hinv[(size_t)p * klen + q] = (k >= 0 && k < N) ? invC[(size_t)p * N + k] : 0.0;
+        }
+    GH_CHECK(hipMemcpyAsync(dp, hp, sizeof(double) * 128, hipMemcpyHostToDevice, c->stream));
+    GH_CHECK(hipMemcpyAsync(dinv, hinv, sizeof(double) * 9 * klen, hipMemcpyHostToDevice, c->stream));
+    LayerArgs a{};
+    a.nwave = dnw;
+    a.list = dlist;
+    a.values = dval;
+    if ((rc = launch_templates(*G, 16, dp, dscr, a, c->stream))) return rc;
+    FishArgs fa{};
+    fa.nwave = dnw;
+    fa.list = dlist;
+    fa.values = dval;
+    fa.invC = dinv;
+    fa.pstride = klen;
+    fa.koff = koff;
+    fa.klen = klen;
+    for (int p = 0; p < 9; p++) fa.pmap[p] = p;
+    fa.npix = N;
+    fa.D = dD;
+    fa.M = dM;
+    fa.fisher = dF;
+    hipLaunchKernelGGL(fisher_kernel, dim3(1), dim3(kFishThreads), 0, c->stream, G->g, 1, fa);
+    GH_CHECK(hipGetLastError());
+    double* hF = hinv + 9 * (size_t)klen;
+    GH_CHECK(hipMemcpyAsync(hF, dF, sizeof(double) * 64, hipMemcpyDeviceToHost, c->stream));
+    GH_CHECK(hipStreamSynchronize(c->stream));
+    for (int q = 0; q < 64; q++)
+        if (hF[q] != hF[q]) return set_error("ucb_fisher_wavelet: template pixels outside the invC band"),
+                                   GLASS_HIP_ERUNTIME;
+    std::memcpy(fisher, hF, sizeof(double) * 64);
+    return GLASS_HIP_OK;
+}
+
+extern "C" int glass_hip_wdm_batch_fisher(glass_hip_wdm_batch* P, int n, const double* d_params, double* d_fisher,
+                                          void* stream)
+{
+    if (!P || !d_params || !d_fisher) return set_error("wdm_batch_fisher: NULL argument"), GLASS_HIP_EINVAL;
+    if (n < 0 || 16LL * n > P->max_templates)
+        return set_error("wdm_batch_fisher: n = %d sources need %lld template slots of %d", n, 16LL * n,
+                         P->max_templates),
+               GLASS_HIP_EINVAL;
+    if (n == 0) return GLASS_HIP_OK;
+    GH_CHECK(hipSetDevice(P->device));
+    const int NT = P->G->g.NT;
+    const size_t nsrc = (size_t)P->max_templates / 16;
+    if (!P->fparams)
+    {
+        GH_CHECK(hipMalloc(&P->fparams, sizeof(double) * 8 * (size_t)P->max_templates));
+        GH_CHECK(hipMalloc(&P->fD, sizeof(double) * 24 * (size_t)kFishMaxBand * NT * nsrc));
+        GH_CHECK(hipMalloc(&P->fM, sizeof(double) * 3 * (size_t)kFishMaxBand * NT * nsrc));
+    }
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(fisher_params_kernel, dim3((16 * n + 255) / 256), dim3(256), 0, s, n, d_params, P->fparams);
+    GH_CHECK(hipGetLastError());
+    LayerArgs a{};
+    a.nwave = P->nwave;
+    a.list = P->list;
+    a.values = P->values;
+    int rc;
+    if ((rc = launch_templates(*P->G, 16 * n, P->fparams, P->scratch, a, s))) return rc;
+    FishArgs fa{};
+    fa.nwave = P->nwave;
+    fa.list = P->list;
+    fa.values = P->values;
+    fa.invC = P->invC;
+    fa.pstride = P->npix;
+    fa.koff = 0;
+    fa.klen = P->npix;
+    const int pm[9] = {0, 3, 4, 3, 1, 5, 4, 5, 2};  // XX YY ZZ XY XZ YZ planes, symmetric
+    for (int p = 0; p < 9; p++) fa.pmap[p] = pm[p];
+    fa.npix = P->npix;
+    fa.D = P->fD;
+    fa.M = P->fM;
+    fa.fisher = d_fisher;
+    hipLaunchKernelGGL(fisher_kernel, dim3(n), dim3(kFishThreads), 0, s, P->G->g, n, fa);
+    GH_CHECK(hipGetLastError());
     return GLASS_HIP_OK;
 }

@@ -59,6 +59,8 @@ _SIGS = [
     ("glass_hip_wdm_batch_eval", [_vp, _i, _vp, _vp, _vp, _vp], _i),
     ("glass_hip_wdm_batch_templates", [_vp, _i, _ip, _ip, _dp], _i),
     ("glass_hip_wdm_batch_scratch", [_vp, _i, _dp, _i], _i),
+    ("glass_hip_ucb_fisher_wavelet", [_W, _i, _dp, _dp, _dp, _dp, _d, _d, _dp, _dp, _i, _dp], _i),
+    ("glass_hip_wdm_batch_fisher", [_vp, _i, _vp, _vp, _vp], _i),
 ]
 _declared = False
 
@@ -151,6 +153,21 @@ def ucb_waveform_wavelet(w, orb, Tobs, params, t0=0.0, X=None, Y=None, Z=None):
     return lst[:nw.value].copy(), X, Y, Z
 
 
+def ucb_fisher_wavelet(w, orb, Tobs, params, invC9, t0=0.0):
+    """ucb_fisher_wavelet up to the eigensolve -> Fisher [8, 8].
+    orb = (t, x, y, z); invC9 [9, npix] row-major 3x3 planes."""
+    t, x, y, z = (_f64(a) for a in orb)
+    p, ic = _f64(params), _f64(invC9)
+    if ic.shape != (9, w.npix):
+        raise ValueError("invC9 must be [9, npix]")
+    F = np.zeros((8, 8))
+    check(_L().glass_hip_ucb_fisher_wavelet(ctypes.byref(w), len(t), t.ctypes.data_as(_dp), x.ctypes.data_as(_dp),
+                                            y.ctypes.data_as(_dp), z.ctypes.data_as(_dp), float(Tobs), float(t0),
+                                            p.ctypes.data_as(_dp), ic.ctypes.data_as(_dp), w.npix,
+                                            F.ctypes.data_as(_dp)), "ucb_fisher_wavelet")
+    return F
+
+
 def snr_wavelet(X, Y, Z, invC9, lst):
     """snr_wavelet: X,Y,Z [N], invC9 [9,N] (row-major 3x3 planes), list."""
     X, Y, Z, ic = _f64(X), _f64(Y), _f64(Z), _f64(invC9)
@@ -215,6 +232,18 @@ class WdmBatch:
         check(_L().glass_hip_wdm_batch_eval(self._p, n, _vp(p.data_ptr()), _vp(self.dlogL.data_ptr()),
                                             _vp(self.snr.data_ptr()), _vp(s)), "wdm_batch_eval")
         return self.dlogL[:n], self.snr[:n]
+
+    def fisher(self, params, stream=None):
+        """Wavelet-domain FD Fisher matrices [n, 8, 8] (device) of n <=
+        max_templates / 16 sources (overwrites the template buffers)."""
+        import torch
+        p = params.contiguous().to(torch.float64)
+        n = p.shape[0]
+        out = torch.empty((n, 8, 8), dtype=torch.float64, device=p.device)
+        s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        check(_L().glass_hip_wdm_batch_fisher(self._p, n, _vp(p.data_ptr()), _vp(out.data_ptr()), _vp(s)),
+              "wdm_batch_fisher")
+        return out
 
     def templates(self, n):
         """(nwave [n], list [n, 3NT], values [n, 3, 3NT]) of the last eval,

@@ -95,6 +95,24 @@ double glass_hip_gaussian_log_likelihood_wavelet(const glass_hip_wdm *wdm, const
                                                  const double *mZ, const int *mlist, int Nlist,
                                                  const double *invC, int N);
 
+/* Replaces void ucb_fisher_wavelet(struct Orbit *orbit, struct Data *data,
+ *   struct Source *source, struct Noise *noise)
+ * (ucb/src/glass_ucb_waveform.h, glass_ucb_waveform.c:524-696) up to the
+ * eigensolve: central differences (epsilon = 1e-6 on the parameter array,
+ * cosines clipped to +-1 on the outer side) of ucb_waveform_wavelet, taken
+ * on the plus template's pixel list at pixels whose time index lies
+ * strictly inside (imin, imax); fisher[8*i+j] = the nine wavelet_nwip terms
+ * XX, YY, ZZ, XY, XZ, YZ, YX, ZX, ZY over all pixels, NaN -> 10.  Orbit and
+ * grid as glass_hip_ucb_waveform_wavelet; invC 9 row-major 3x3 planes of
+ * N = kmax - kmin pixels (only the layers the templates touch are read).
+ * The reference then calls matrix_eigenstuff (LAPACKE_dgeev) and, if an
+ * eigenvalue is not positive, adds 0.1 to the diagonal once more
+ * (:620-686); that stays on the host. */
+int glass_hip_ucb_fisher_wavelet(const glass_hip_wdm *wdm, int Norb, const double *orbit_t,
+                                 const double *orbit_x, const double *orbit_y,
+                                 const double *orbit_z, double Tobs, double t0,
+                                 const double *params, const double *invC, int N, double *fisher);
+
 /* ------------------------------------------------ device-pointer API --- */
 
 /* Full forward transform on device memory: d_time (ND samples) ->
@@ -129,6 +147,14 @@ int glass_hip_wdm_batch_eval(glass_hip_wdm_batch *plan, int n, const double *d_p
  * list[n*3*NT], values[n*3*3*NT] (channel-major per template). */
 int glass_hip_wdm_batch_templates(glass_hip_wdm_batch *plan, int n, int *nwave, int *list,
                                   double *values);
+/* ucb_fisher_wavelet (as glass_hip_ucb_fisher_wavelet) for n sources in
+ * one launch sequence (no reference counterpart as a batch): d_params[n*8]
+ * (device) -> d_fisher[n*64] (device), with the plan's invC (6 symmetric
+ * planes).  Uses 16 template slots per source (16 n <= max_templates) and
+ * overwrites the plan's template buffers.  A source whose 16 templates span
+ * more than 6 layers gets NaN entries. */
+int glass_hip_wdm_batch_fisher(glass_hip_wdm_batch *plan, int n, const double *d_params,
+                               double *d_fisher, void *stream);
 /* Diagnostics: the response stage of template e of the last eval (host copy,
  * synchronous): 64 header doubles, the barycentre phase spline (4 x Norb:
  * values, then the three coefficient rows) and the six TDI splines

@@ -227,6 +227,13 @@ double orc_gaussian_log_likelihood_wavelet(const orc_wdm *w, const double *dX, c
                                            const double *dZ, const double *mX, const double *mY,
                                            const double *mZ, const int *mlist, int Nlist,
                                            const double *invC, int Npix);
+/* ucb_fisher_wavelet (ucb/src/glass_ucb_waveform.c:524-696) before the
+ * eigensolve: central differences (epsilon 1e-6) of the wavelet template on
+ * the plus template's pixel list, nine-term wavelet_nwip Fisher over all
+ * pixels, NaN -> 10; invC 9 planes of Npix; fisher[64] row-major. */
+int    orc_ucb_fisher_wavelet(const orc_wdm *w, int Norb, const double *ot, const double *oc,
+                              double Tobs, double t0, const double *params, const double *invC,
+                              int Npix, double *fisher);
 /* CPU baseline of the wavelet workload: per template the compact waveform,
  * snr_wavelet and the logL change the template makes (see orc_wdm.c);
  * data 3 planes and invC 9 planes of Npix.  Returns wall seconds. */

@@ -318,6 +318,8 @@ def _wdm_signatures(L):
     L.orc_wdm_eval_batch.argtypes = [W, c_int, _dp, _dp, c_d, c_d, c_int, _dp, _dp, _dp, c_int,
                                      _dp, _dp, _ip, c_int]
     L.orc_wdm_eval_batch.restype = c_d
+    L.orc_ucb_fisher_wavelet.argtypes = [W, c_int, _dp, _dp, c_d, c_d, _dp, _dp, c_int, _dp]
+    L.orc_ucb_fisher_wavelet.restype = c_int
 
 
 class Wdm:
@@ -484,6 +486,19 @@ def ucb_wavelet_response(orb, Tobs, params, t0=0.0):
     lib().orc_ucb_wavelet_response(N, _d(t), _d(coef), float(Tobs), float(t0), _d(p), _d(ps), _d(pc), _d(a3),
                                    _d(p3))
     return ps, pc, a3, p3
+
+
+def ucb_fisher_wavelet(w, orb, Tobs, params, invC9, t0=0.0):
+    """ucb_fisher_wavelet before the eigensolve -> Fisher [8, 8]."""
+    t, _, coef = orb
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    ic = np.ascontiguousarray(invC9, dtype=np.float64)
+    F = np.zeros((8, 8))
+    rc = lib().orc_ucb_fisher_wavelet(w.ref, len(t), _d(t), _d(coef), float(Tobs), float(t0), _d(p), _d(ic),
+                                      ic.shape[-1], _d(F))
+    if rc:
+        raise MemoryError("orc_ucb_fisher_wavelet")
+    return F
 
 
 def snr_wavelet(X, Y, Z, invC9, lst):

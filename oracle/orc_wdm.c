@@ -945,3 +945,122 @@ double orc_wdm_eval_batch(const orc_wdm *w, int Norb, const double *ot, const do
     return 0.0;
 #endif
 }
+
+/* ucb_fisher_wavelet (ucb/src/glass_ucb_waveform.c:524-696) up to the
+ * eigensolve.  Per parameter i: templates at params +- epsilon/2 (epsilon =
+ * 1e-6 on the parameter array; cosines clipped to <= 1 on the plus and >= -1
+ * on the minus side, :560-576), derivative (h+ - h-)/epsilon on the PLUS
+ * template's pixel list only, at pixels whose time index lies strictly
+ * inside (imin, imax) (:592-603); h- there is the minus template's value
+ * or 0.  Fisher element (i <= j) = sum of nine wavelet_nwip over every pixel
+ * in the order XX, YY, ZZ, XY, XZ, YZ, YX, ZX, ZY (:608-617), NaN -> 10.0.
+ *
+ * The reference's derivative arrays are zero outside the union of the plus
+ * lists, so each of its sums over all data->N pixels adds exact zeros
+ * there; summing over the window [klo, khi) of written pixels in ascending
+ * pixel order gives the same result to the bit (for finite invC).
+ * invC: 9 row-major planes of Npix.  Returns 0, or -1 on allocation
+ * failure. */
+#ifndef O_NP
+#define O_NP 8
+#endif
+
+int orc_ucb_fisher_wavelet(const orc_wdm *w, int Norb, const double *ot, const double *oc, double Tobs,
+                           double t0, const double *params, const double *invC, int Npix, double *fisher)
+{
+    const double epsilon = 1.0e-6;
+    const double invstep = 1. / (epsilon);
+    const int cap = 4 * w->NT;
+    int *lp = malloc(16 * (size_t)cap * sizeof(int));
+    int nw[16];
+    double *vp = malloc(16 * 3 * (size_t)cap * sizeof(double));
+    if (!lp || !vp)
+    {
+        free(lp);
+        free(vp);
+        return -1;
+    }
+    /* slot 2i: plus template of parameter i, 2i+1: minus */
+    for (int i = 0; i < O_NP; i++)
+        for (int sgn = 0; sgn < 2; sgn++)
+        {
+            double p[O_NP];
+            memcpy(p, params, sizeof(p));
+            if (sgn == 0) p[i] += epsilon / 2.;
+            else p[i] -= epsilon / 2.;
+            if (i == 1 || i == 4)
+            {
+                if (sgn == 0 && p[i] > 1.0) p[i] = 1.0;
+                if (sgn == 1 && p[i] < -1.0) p[i] = -1.0;
+            }
+            int s = 2 * i + sgn;
+            double *v = vp + (size_t)s * 3 * cap;
+            orc_ucb_waveform_wavelet_compact(w, Norb, ot, oc, Tobs, t0, p, lp + (size_t)s * cap, &nw[s], v, v + cap,
+                                             v + 2 * cap);
+        }
+    int klo = Npix, khi = 0;
+    for (int s = 0; s < 16; s++)
+        for (int n = 0; n < nw[s]; n++)
+        {
+            int k = lp[(size_t)s * cap + n];
+            if (k < klo) klo = k;
+            if (k + 1 > khi) khi = k + 1;
+        }
+    if (khi <= klo) klo = khi = 0;
+    const int nloc = khi - klo;
+    double *dh = calloc((size_t)O_NP * 3 * (nloc ? nloc : 1), sizeof(double));
+    double *hm = malloc(3 * (size_t)(nloc ? nloc : 1) * sizeof(double));
+    if (!dh || !hm)
+    {
+        free(lp);
+        free(vp);
+        free(dh);
+        free(hm);
+        return -1;
+    }
+    for (int i = 0; i < O_NP; i++)
+    {
+        const int sp = 2 * i, sm = 2 * i + 1;
+        /* the minus template scattered into a zeroed array (:583-590) */
+        memset(hm, 0, 3 * (size_t)nloc * sizeof(double));
+        for (int n = 0; n < nw[sm]; n++)
+        {
+            int k = lp[(size_t)sm * cap + n];
+            for (int ch = 0; ch < 3; ch++) hm[ch * nloc + (k - klo)] = vp[((size_t)sm * 3 + ch) * cap + n];
+        }
+        for (int n = 0; n < nw[sp]; n++)
+        {
+            int k = lp[(size_t)sp * cap + n];
+            int i_wdm = (k + w->kmin) % w->NT;
+            if (i_wdm > w->imin && i_wdm < w->imax && k >= 0 && k < Npix)
+                for (int ch = 0; ch < 3; ch++)
+                    dh[((size_t)i * 3 + ch) * nloc + (k - klo)] =
+                        (vp[((size_t)sp * 3 + ch) * cap + n] - hm[ch * nloc + (k - klo)]) * invstep;
+        }
+    }
+    /* (a, b) channel pairs of the nine wavelet_nwip terms, in order */
+    static const int ta[9] = {0, 1, 2, 0, 0, 1, 1, 2, 2};
+    static const int tb[9] = {0, 1, 2, 1, 2, 2, 0, 0, 1};
+    for (int i = 0; i < O_NP; i++)
+        for (int j = i; j < O_NP; j++)
+        {
+            double F = 0.0;
+            for (int t = 0; t < 9; t++)
+            {
+                const double *a = dh + ((size_t)i * 3 + ta[t]) * nloc;
+                const double *b = dh + ((size_t)j * 3 + tb[t]) * nloc;
+                const double *c = invC + (size_t)(3 * ta[t] + tb[t]) * Npix + klo;
+                double arg = 0.0;
+                for (int q = 0; q < nloc; q++) arg += a[q] * b[q] * c[q];
+                F += arg;
+            }
+            if (F != F) F = 10.0;
+            fisher[i * O_NP + j] = F;
+            fisher[j * O_NP + i] = F;
+        }
+    free(lp);
+    free(vp);
+    free(dh);
+    free(hm);
+    return 0;
+}

@@ -206,3 +206,87 @@ def test_wavelet_bad_inputs(gl, W, g2):
     with pytest.raises(GlassHipError):
         X = np.zeros(8)
         W.snr_wavelet(X, X, X, np.zeros((9, 8)), [0, 9])
+
+
+# ------------------------------------------------------ wavelet Fisher --
+
+def boundary_params(p, df, j0):
+    """params with f0 (1 - v_earth) = j0 df exactly and fdot = 0: the plus
+    and minus f0 steps of ucb_fisher_wavelet then take different layer
+    sets (jmin j0 vs j0 - 1), so only the plus template's pixels carry
+    derivatives (glass_ucb_waveform.c:592-603)."""
+    VE = 0.00010103671    # VEARTH of ucb_wavelet_layers (ucb/src/glass_ucb_waveform.c:1003-1037)
+    q = p.copy()
+    q[0] = j0 * df / (1.0 - VE) * T1
+    q[7] = 0.0
+    return q
+
+
+def fisher_err(F, R):
+    """max_ij |F - R| / sqrt(R_ii R_jj)."""
+    d = np.sqrt(np.abs(np.diag(R)))
+    return float(np.max(np.abs(F - R) / np.outer(d, d)))
+
+
+def test_fisher_wavelet_dropin_vs_oracle(orc, W, g2):
+    """Drop-in vs the oracle with correctly rounded sin/cos: the central
+    differences divide the ~1e-14 template differences by eps = 1e-6, so
+    the matrices agree to ~1e-8 of their scale; bound 1e-6, as for the
+    Fourier-domain FD Fisher."""
+    ow, gw, oorb, gorb = g2
+    cfg = synth.config2(n=20, seed=14)
+    P = list(cfg["params"])
+    P += [boundary_params(cfg["params"][0], ow.df, 611), boundary_params(cfg["params"][1], ow.df, 300)]
+    for q in P[-2:]:
+        a, b = q.copy(), q.copy()
+        a[0] += 5e-7
+        b[0] -= 5e-7
+        assert orc.ucb_wavelet_layers(ow, T1, a) != orc.ucb_wavelet_layers(ow, T1, b)
+    worst = 0.0
+    for e, p in enumerate(P):
+        F = W.ucb_fisher_wavelet(gw, gorb, T1, p, cfg["invC9"])
+        with orc.cr_trig():
+            R = orc.ucb_fisher_wavelet(ow, oorb, T1, p, cfg["invC9"])
+        err = fisher_err(F, R)
+        worst = max(worst, err)
+        assert np.array_equal(F, F.T)
+        assert err < 1e-6, (e, err)
+    print("wavelet Fisher drop-in worst normalised error vs oracle", worst)
+
+
+def test_fisher_wavelet_boundary_quirk(orc, g2):
+    """At a layer boundary the plus-list rule is what the result follows:
+    summing derivatives over the union of both lists would change F."""
+    ow, gw, oorb, gorb = g2
+    cfg = synth.config2(n=2, seed=15)
+    p = boundary_params(cfg["params"][0], ow.df, 611)
+    e = 1e-6
+    pp, pm = p.copy(), p.copy()
+    pp[0] += e / 2
+    pm[0] -= e / 2
+    lp, _ = orc.ucb_waveform_wavelet(ow, oorb, T1, pp)
+    lm, _ = orc.ucb_waveform_wavelet(ow, oorb, T1, pm)
+    assert len(np.setdiff1d(lm, lp)) > 0      # the minus template has pixels the plus one lacks
+
+
+def test_wdm_batch_fisher_matches_dropin(orc, W, g2):
+    import torch
+    ow, gw, oorb, gorb = g2
+    cfg = synth.config2(n=24, seed=16)
+    P = cfg["params"].copy()
+    P[5] = boundary_params(P[5], ow.df, 450)
+    B = W.WdmBatch(gw, gorb, T1, cfg["data"], cfg["invC6"], max_templates=16 * 24)
+    F = B.fisher(torch.tensor(P, device="cuda")).cpu().numpy()
+    # same templates and kernel; the batch reads 6 symmetric invC planes
+    # where the drop-in reads all 9 (numpy's inverse is symmetric only to
+    # an ulp)
+    for e in range(0, 24, 3):
+        R = W.ucb_fisher_wavelet(gw, gorb, T1, P[e], cfg["invC9"])
+        assert fisher_err(F[e], R) < 1e-13, (e, fisher_err(F[e], R))
+    with orc.cr_trig():
+        R = orc.ucb_fisher_wavelet(ow, oorb, T1, P[5], cfg["invC9"])
+    assert fisher_err(F[5], R) < 1e-6
+    from glass_amd import GlassHipError
+    with pytest.raises(GlassHipError):
+        B.fisher(torch.tensor(np.tile(P, (2, 1)), device="cuda"))   # 48 sources > 384 / 16
+    B.close()

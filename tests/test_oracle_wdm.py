@@ -287,3 +287,75 @@ def test_wdm_whitening_layer_var(orc):
         acc += np.mean(pix ** 2, axis=1)
     emp = acc[1:NF] / reps
     assert abs(np.mean(emp / lv[1:NF]) - 1) < 1e-2
+
+
+# ----------------------------------------------- ucb_fisher_wavelet pins --
+
+def _edge_inner(w, lst):
+    i = (lst + w.kmin) % w.NT
+    return (i > w.imin) & (i < w.imax)
+
+
+def _quad9(u, v, ic9, k):
+    """sum over pixels k of u^T C v with the nine planes (u, v: [3, len(k)])."""
+    return sum(np.sum(u[a] * v[b] * ic9[3 * a + b, k]) for a in range(3) for b in range(3))
+
+
+def test_fisher_wavelet_lnA_is_edge_snr2(orc, grid2):
+    """The template is linear in A = exp(params[3]), so the central
+    difference of params[3] is (exp(dp) - exp(dm))/eps h, with dp, dm the
+    steps actually taken (p3 +- eps/2 round to the ulp of p3 ~ 50, 7e-15,
+    i.e. ~1e-8 of eps; p+ - p3 is exact): F[3][3] = that factor^2 times
+    the sum of h^T C h over the template's pixels strictly inside (imin,
+    imax).  The two templates differ in every rounding of their A-scaled
+    intermediates, ~1e-15/eps per pixel and correlated along the splines:
+    residuals of 1e-10..1e-7 relative; the bound 1e-6 still fails a wrong
+    step, edge rule or channel pair by orders of magnitude."""
+    w, orb = grid2
+    cfg = synth.config2(n=8, seed=12)
+    eps = 1e-6
+    for p in cfg["params"]:
+        F = orc.ucb_fisher_wavelet(w, orb, T1, p, cfg["invC9"])
+        lst, v = orc.ucb_waveform_wavelet(w, orb, T1, p)
+        m = _edge_inner(w, lst)
+        dp = (p[3] + eps / 2.0) - p[3]
+        dm = (p[3] - eps / 2.0) - p[3]
+        want = _quad9(v[:, m], v[:, m], cfg["invC9"], lst[m]) * ((np.exp(dp) - np.exp(dm)) / eps) ** 2
+        assert abs(F[3, 3] - want) < 1e-6 * want, (F[3, 3], want)
+        # the pixels outside the edges do contribute to snr: the rule matters
+        assert _quad9(v, v, cfg["invC9"], lst) > 1.01 * want
+        assert np.array_equal(F, F.T)
+
+
+def test_fisher_wavelet_vs_five_point(orc, grid2):
+    """Against a Fisher built independently from five-point central
+    differences with larger steps on the same pixels: agreement set by the
+    oracle's 1e-6 central-difference noise (~1e-5 on the f0 terms)."""
+    w, orb = grid2
+    cfg = synth.config2(n=2, seed=13)
+    steps = np.array([1e-3, 1e-4, 1e-4, 1e-4, 1e-4, 1e-4, 1e-4, 1e-3])
+    for p in cfg["params"]:
+        p = p.copy()
+        p[1] = np.clip(p[1], -0.9, 0.9)
+        p[4] = np.clip(p[4], -0.9, 0.9)
+        F = orc.ucb_fisher_wavelet(w, orb, T1, p, cfg["invC9"])
+        lst0, _ = orc.ucb_waveform_wavelet(w, orb, T1, p)
+        m0 = _edge_inner(w, lst0)
+        keep = lst0[m0]
+        pos = {int(k): n for n, k in enumerate(keep)}
+        D = np.zeros((8, 3, len(keep)))
+        for i in range(8):
+            acc = np.zeros((3, len(keep)))
+            for c, s in ((8, 1), (-8, -1), (-1, 2), (1, -2)):
+                q = p.copy()
+                q[i] += s * steps[i]
+                l, v = orc.ucb_waveform_wavelet(w, orb, T1, q)
+                assert np.array_equal(l, lst0)
+                acc += c * v[:, m0]
+            D[i] = acc / (12 * steps[i])
+        G = np.array([[_quad9(D[i], D[j], cfg["invC9"], keep) for j in range(8)] for i in range(8)])
+        norm = np.sqrt(np.outer(np.diag(G), np.diag(G)))
+        err = np.max(np.abs(F - G) / norm)
+        print("five-point vs oracle FD Fisher, normalised", err)
+        assert err < 1e-3, err
+        del pos
