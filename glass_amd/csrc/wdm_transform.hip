@@ -452,3 +452,113 @@ extern "C" int glass_hip_wavelet_transform_inverse_freq(const glass_hip_wdm* w, 
     GH_CHECK(hipStreamSynchronize(c->stream));
     return GLASS_HIP_OK;
 }
+
+// ------------------------------------------------------------ by layers --
+//
+// wavelet_transform_timefreq_by_layers (utils/src/glass_wavelet.c:544-611):
+// N = NT (Nlayers + 1) heterodyned samples -> rFFT(N) -> per layer m =
+// jmin .. jmin + Nlayers - 1 the window-weighted band of NT bins around
+// (m - jmin + 1) NT / 2, inverse FFT(NT) / NT, re or +-im by the (n + m)
+// parity, scaled by sqrt(NF) / (Nlayers + 1); pixel (n, m) at n Nlayers +
+// m - jmin, the rest of the N samples zero.  One workgroup; the real input
+// is transformed as a complex series with zero imaginary part.
+
+namespace glass {
+namespace {
+
+constexpr int kBlThreads = 256;
+
+template <int NT, int LP1>
+__global__ void __launch_bounds__(kBlThreads) by_layers_kernel(const double* __restrict__ in, double* __restrict__ out,
+                                                               const double* __restrict__ win, int jmin, int NF,
+                                                               const double2* __restrict__ tw,
+                                                               const double2* __restrict__ tw3)
+{
+    constexpr int N = NT * LP1, L = LP1 - 1;
+    __shared__ double2 F[N];
+    __shared__ double2 B[L * NT];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < N; i += kBlThreads) F[i] = make_double2(in[i], 0.0);
+    __syncthreads();
+    constexpr bool three = N % 3 == 0;
+    lds_fft<N, 1, kBlThreads>(F, three ? tw3 : tw, (three ? kTw3N : kTwN) / N);
+    for (int it = tid; it < L * NT; it += kBlThreads)
+    {
+        const int off = it / NT, jj = it % NT;
+        const int j = jj - NT / 2;
+        const int m = jmin + off;
+        const int idx = (off + 1) * NT / 2 + j;
+        double2 v = make_double2(0.0, 0.0);
+        if (jj > 0 && idx >= 0 && idx <= N / 2)
+        {
+            double wt = win[j < 0 ? -j : j];
+            if (j == 0 && (m == 0 || m == NF)) wt /= 2.0;
+            // conjugated: the forward FFT then gives the conjugate of the inverse
+            v = make_double2(wt * F[idx].x, -(wt * F[idx].y));
+        }
+        B[off * NT + jj] = v;
+    }
+    __syncthreads();
+    lds_fft<NT, L, kBlThreads>(B, tw, kTwN / NT);
+    const double norm = sqrt((double)NF) / (L + 1.0);
+    for (int k = tid; k < N; k += kBlThreads)
+    {
+        double val = 0.0;
+        if (k < NT * L)
+        {
+            const int n = k / L, off = k % L;
+            const int m = jmin + off;
+            const double2 r = B[off * NT + n];
+            const double re = r.x / NT, im = -r.y / NT;
+            if (m == 0 || m == NF) val = (n % 2 == 0) ? re * M_SQRT2 * norm : 0.0;
+            else
+            {
+                const int sgn = (m % 2 == 0) ? 1 : -1;
+                val = ((n + m) % 2 == 0) ? re * norm : sgn * im * norm;
+            }
+        }
+        out[k] = val;
+    }
+}
+
+}  // namespace
+}  // namespace glass
+
+extern "C" int glass_hip_wavelet_transform_timefreq_by_layers(const glass_hip_wdm* w, int jmin, int Nlayers,
+                                                              const double* window, double* timedata)
+{
+    using namespace glass;
+    if (!w || !window || !timedata) return set_error("wavelet_transform_timefreq_by_layers: NULL argument"), GLASS_HIP_EINVAL;
+    const int NT = w->NT;
+    if ((NT != 256 && NT != 512) || Nlayers < 1 || Nlayers > 3 || jmin < 0 || jmin + Nlayers - 1 > w->NF)
+        return set_error("wavelet_transform_timefreq_by_layers: NT = %d (256 or 512), Nlayers = %d (1..3), "
+                         "jmin = %d (layers within 0..NF = %d)", NT, Nlayers, jmin, w->NF),
+               GLASS_HIP_EINVAL;
+    ThreadCtx* c;
+    int rc = ctx_get(&c);
+    if (rc) return rc;
+    const double2 *tw, *tw3;
+    if ((rc = twiddles(&tw)) || (rc = twiddles3(&tw3))) return rc;
+    const int N = NT * (Nlayers + 1), nw = NT / 2 + 1;
+    const size_t bytes = sizeof(double) * (2 * (size_t)N + nw);
+    if ((rc = c->dbuf.ensure(bytes)) || (rc = c->hbuf.ensure(bytes))) return rc;
+    double* d_in = (double*)c->dbuf.p;
+    double* d_out = d_in + N;
+    double* d_win = d_out + N;
+    double* h = (double*)c->hbuf.p;
+    std::memcpy(h, timedata, sizeof(double) * N);
+    std::memcpy(h + 2 * (size_t)N, window, sizeof(double) * nw);
+    GH_CHECK(hipMemcpyAsync(d_in, h, sizeof(double) * N, hipMemcpyHostToDevice, c->stream));
+    GH_CHECK(hipMemcpyAsync(d_win, h + 2 * (size_t)N, sizeof(double) * nw, hipMemcpyHostToDevice, c->stream));
+#define GH_BL(nt, lp1)                                                                                     \
+    if (NT == nt && Nlayers + 1 == lp1)                                                                    \
+        hipLaunchKernelGGL((by_layers_kernel<nt, lp1>), dim3(1), dim3(kBlThreads), 0, c->stream, d_in, d_out, \
+                           d_win, jmin, w->NF, tw, tw3);
+    GH_BL(256, 2) GH_BL(256, 3) GH_BL(256, 4) GH_BL(512, 2) GH_BL(512, 3) GH_BL(512, 4)
+#undef GH_BL
+    GH_CHECK(hipGetLastError());
+    GH_CHECK(hipMemcpyAsync(h, d_out, sizeof(double) * N, hipMemcpyDeviceToHost, c->stream));
+    GH_CHECK(hipStreamSynchronize(c->stream));
+    std::memcpy(timedata, h, sizeof(double) * N);
+    return GLASS_HIP_OK;
+}

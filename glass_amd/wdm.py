@@ -59,6 +59,7 @@ _SIGS = [
     ("glass_hip_wdm_batch_eval", [_vp, _i, _vp, _vp, _vp, _vp], _i),
     ("glass_hip_wdm_batch_templates", [_vp, _i, _ip, _ip, _dp], _i),
     ("glass_hip_wdm_batch_scratch", [_vp, _i, _dp, _i], _i),
+    ("glass_hip_wavelet_transform_timefreq_by_layers", [_W, _i, _i, _dp, _dp], _i),
     ("glass_hip_ucb_fisher_wavelet", [_W, _i, _dp, _dp, _dp, _dp, _d, _d, _dp, _dp, _i, _dp], _i),
     ("glass_hip_wdm_batch_fisher", [_vp, _i, _vp, _vp, _vp], _i),
 ]
@@ -128,6 +129,19 @@ def wavelet_transform_inverse_freq(w, wdmdata):
     check(_L().glass_hip_wavelet_transform_inverse_freq(ctypes.byref(w), p.ctypes.data_as(_dp),
                                                         out.ctypes.data_as(_dp)), "wavelet_transform_inverse_freq")
     return out
+
+
+def wavelet_transform_timefreq_by_layers(w, jmin, nlayers, window, timedata):
+    """wavelet_transform_timefreq_by_layers on a copy of the NT (nlayers+1)
+    samples; returns the transformed array."""
+    x = np.array(timedata, dtype=np.float64, copy=True)
+    win = _f64(window)
+    if x.size != w.NT * (nlayers + 1) or win.size != w.NT // 2 + 1:
+        raise ValueError("timedata must hold NT (nlayers + 1) samples and window NT/2 + 1")
+    check(_L().glass_hip_wavelet_transform_timefreq_by_layers(ctypes.byref(w), int(jmin), int(nlayers),
+                                                              win.ctypes.data_as(_dp), x.ctypes.data_as(_dp)),
+          "wavelet_transform_timefreq_by_layers")
+    return x
 
 
 def ucb_waveform_wavelet(w, orb, Tobs, params, t0=0.0, X=None, Y=None, Z=None):

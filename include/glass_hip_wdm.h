@@ -60,6 +60,16 @@ int glass_hip_wavelet_transform_freq(const glass_hip_wdm *wdm, const double *fre
 int glass_hip_wavelet_transform_inverse_freq(const glass_hip_wdm *wdm, const double *wdmdata,
                                              double *freqdata);
 
+/* Replaces void wavelet_transform_timefreq_by_layers(struct Wavelets *wdm,
+ *   int jmin, int Nlayers, double *window, double *timedata)
+ * (utils/src/glass_wavelet.h:119, glass_wavelet.c:544-611): in place on
+ * NT (Nlayers + 1) heterodyned samples; window NT/2 + 1 doubles (the
+ * (Nlayers+1)-layer filter of build_wdm_filter_freq).  Pixel (n, m) lands
+ * at n Nlayers + m - jmin, the remaining samples are zeroed.  NT 256 or 512,
+ * Nlayers 1..3 (ucb_wavelet_layers' range). */
+int glass_hip_wavelet_transform_timefreq_by_layers(const glass_hip_wdm *wdm, int jmin, int Nlayers,
+                                                   const double *window, double *timedata);
+
 /* Replaces void ucb_waveform_wavelet(struct Orbit *orbit,
  *   struct Wavelets *wdm, double Tobs, double t0, double *params,
  *   int *wavelet_list, int *Nwavelet, double *X, double *Y, double *Z)

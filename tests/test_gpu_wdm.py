@@ -75,6 +75,23 @@ def test_transform_impulse_known_answer(W, NF, NT):
     assert np.all(np.abs(F - ref) <= 1e-10 + 1e-5 * np.abs(ref))
 
 
+@pytest.mark.parametrize("NT,L,jmin", [(512, 1, 7), (512, 2, 300), (512, 3, 4000), (256, 2, 0), (256, 3, 4094),
+                                       (256, 1, 4096)])
+def test_by_layers_vs_oracle(orc, W, NT, L, jmin):
+    """wavelet_transform_timefreq_by_layers drop-in, including the m = 0 and
+    m = NF edge layers (normwise 1e-12)."""
+    NF = 4096
+    ow, gw = grids(orc, W, NF, NT, T1 / NT)
+    win = orc.wdm_filter(L + 1, NT, 0.0, True)
+    x = np.random.default_rng(NT + 10 * L + jmin).standard_normal(NT * (L + 1))
+    ref = orc.wavelet_transform_by_layers(ow, jmin, L, win, x)
+    got = W.wavelet_transform_timefreq_by_layers(gw, jmin, L, win, x)
+    err = normwise(got, ref)
+    print(NT, L, jmin, "by-layers normwise", err)
+    assert err < 1e-12
+    assert np.all(got[NT * L:] == 0.0)
+
+
 def test_wdm_transform_device_api(W):
     import torch
     gw = W.Wdm.grid(4096, 512, T1 / 512)
