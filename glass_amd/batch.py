@@ -30,6 +30,7 @@ _EXTRA += [
     ("glass_hip_fd_batch_get_tiles", [_vp, _tp], _i),
     ("glass_hip_fd_batch_set_tiles", [_vp, _tp], _i),
     ("glass_hip_fd_batch_autotune", [_vp, _i, _vp, _vp, _vp, _i, _tp, _dp, _vp], _i),
+    ("glass_hip_fd_batch_derivs", [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp], _i),
     ("glass_hip_fd_batch_create", [ctypes.POINTER(_vp), _vp, ctypes.c_char_p, _d, _d, _i, _ip, _ip, _dp, _dp,
                                    _i, _ip, _ip, _dp, _i], _i),
     ("glass_hip_fd_batch_destroy", [_vp], _i),
@@ -201,6 +202,23 @@ class FdBatch:
         check(_L().glass_hip_fd_batch_autotune(self.h, int(n), _ptr(self.chain), _ptr(self.src), _ptr(self.params),
                                                int(reps), t, ms.ctypes.data_as(_dp), s), "fd_batch_autotune")
         return [(x.nthr, x.gmul) for x in t], ms
+
+    def derivs(self, chain, src):
+        """Exact (L, grad [n, 8], Fisher [n, 8, 8]) of sources (chain[e],
+        src[e]) at their current parameters (device tensors, current
+        stream)."""
+        t = self.torch
+        dev = self.chain.device
+        c = t.as_tensor(np.asarray(chain, dtype=np.int32)).to(dev) if not t.is_tensor(chain) else chain
+        s_ = t.as_tensor(np.asarray(src, dtype=np.int32)).to(dev) if not t.is_tensor(src) else src
+        n = int(c.numel())
+        L = t.empty(n, dtype=t.float64, device=dev)
+        g = t.empty((n, 8), dtype=t.float64, device=dev)
+        F = t.empty((n, 8, 8), dtype=t.float64, device=dev)
+        st = ctypes.c_void_p(t.cuda.current_stream().cuda_stream)
+        check(_L().glass_hip_fd_batch_derivs(self.h, n, _ptr(c), _ptr(s_), _ptr(L), _ptr(g), _ptr(F), st),
+              "fd_batch_derivs")
+        return L, g, F
 
     def state(self):
         self.torch.cuda.current_stream().synchronize()

@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "../../include/glass_hip_batch.h"
+#include "fd_derivs.h"
 #include "fd_waveform.h"
 #include "reduce.h"
 #include "runtime.h"
@@ -55,8 +56,16 @@ struct BatchDev
     double* sparams;          // [C][8][8]
     int* s_bw;                // [C*8]
     int* s_imin;              // [C*8]
-    double2* tmpl;            // [C*8][3][bwcap]
-    int bwcap;                // template slot size: largest BW any window allows
+    // templates: chain c's live sources s < c_nsrc[c] hold [3][cap] complex
+    // at tmpl + c_toff[c] + 3 s cap, cap = g_cap[group] (the largest BW the
+    // window allows: the largest power of two <= W); one accept slot of
+    // [3][cap] per chain at hnew + c_hoff[c]
+    double2* tmpl;
+    double2* hnew;
+    const int* g_cap;
+    const long long* c_toff;
+    const long long* c_hoff;
+    int bwcap;                // largest g_cap
     // constants
     double T, t0, L, fstar;
     const double* pos;  // positions table, kPosN samples
@@ -78,6 +87,12 @@ __device__ inline int item_q(const Items& it, int e) { return it.sel ? it.sel[e]
 __device__ inline const double* item_params(const Items& it, int e) { return it.params + (size_t)item_q(it, e) * 8; }
 __device__ inline int item_chain(const Items& it, int e) { return it.chain[item_q(it, e)]; }
 __device__ inline int item_src(const Items& it, int e) { return it.src[item_q(it, e)]; }
+
+__device__ inline int chain_cap(const BatchDev& B, int c) { return B.g_cap[B.c_group[c]]; }
+__device__ inline double2* src_tmpl(const BatchDev& B, int c, int s)
+{
+    return B.tmpl + B.c_toff[c] + (size_t)s * 3 * chain_cap(B, c);
+}
 
 __global__ void zero_int_kernel(int* p, int n)
 {
@@ -113,7 +128,7 @@ __global__ __launch_bounds__(256) void align_kernel(int n, Items it, BatchDev B,
         bw_out[e] = BW;
         imin_out[e] = qmin - B.g_qmin[g];
         k = 31 - __clz(BW) - 5;  // log2(BW) - 5
-        if (BW > B.bwcap || k < 0 || k >= kNClass || (BW & (BW - 1))) k = kNClass;
+        if (BW > B.g_cap[g] || k < 0 || k >= kNClass || (BW & (BW - 1))) k = kNClass;
         slot = atomicAdd(&hist[k], 1);
     }
     __syncthreads();
@@ -187,7 +202,8 @@ __device__ __forceinline__ void eval_item(int w, double2* lds, const int* __rest
         const int so = c * kMaxSrc + s;
         const int imin_n = p_imin[e];
         const int imin_o = B.s_imin[so], bw_o = B.s_bw[so];
-        const double2* ho = B.tmpl + (size_t)so * 3 * B.bwcap;
+        const int cap = B.g_cap[g];
+        const double2* ho = src_tmpl(B, c, s);
         const double2* R = B.resid + (size_t)B.c_roff[c] * 3;
         const double* iC = B.invC + (size_t)B.g_off[g] * 6;
         double acc = 0.0, s2 = 0.0;
@@ -211,7 +227,7 @@ __device__ __forceinline__ void eval_item(int w, double2* lds, const int* __rest
 #pragma unroll
                 for (int ch = 0; ch < 3; ch++)
                 {
-                    double2 h0 = inold ? ho[ch * B.bwcap + mo] : make_double2(0.0, 0.0);
+                    double2 h0 = inold ? ho[ch * cap + mo] : make_double2(0.0, 0.0);
                     d[ch] = make_double2(v[ch].x - h0.x, v[ch].y - h0.y);
                     r[ch] = R[(size_t)ch * W + j];
                 }
@@ -229,7 +245,7 @@ __device__ __forceinline__ void eval_item(int w, double2* lds, const int* __rest
 #pragma unroll
             for (int ch = 0; ch < 3; ch++)
             {
-                double2 h0 = ho[ch * B.bwcap + mo];
+                double2 h0 = ho[ch * cap + mo];
                 d[ch] = make_double2(-h0.x, -h0.y);
                 r[ch] = R[(size_t)ch * W + j];
             }
@@ -279,11 +295,11 @@ __global__ __launch_bounds__(NTHR) void eval_kernel(const int* __restrict__ list
     X(4, 512, 512) X(4, 512, 256) X(4, 512, 128) X(5, 1024, 512) X(5, 1024, 1024) X(5, 1024, 256)     \
     X(6, 2048, 512) X(6, 2048, 1024) X(7, 4096, 512)
 
-// Template of each listed item into dst slot (slot index = dslot[e] or e).
+// Template of each listed item into its source's slot, or (to_hnew) into
+// its chain's accept slot.
 template <int N, int NTHR>
 __global__ __launch_bounds__(NTHR) void tmpl_kernel(const int* __restrict__ list, const int* __restrict__ count,
-                                                    Items it, BatchDev B, double2* __restrict__ dst,
-                                                    const int* __restrict__ dslot)
+                                                    Items it, BatchDev B, int to_hnew)
 {
     using S = FdShape<N, NTHR>;
     __shared__ double2 lds[S::LDS];
@@ -294,7 +310,8 @@ __global__ __launch_bounds__(NTHR) void tmpl_kernel(const int* __restrict__ list
         const UcbConst& k = ucb_setup_shared(item_params(it, e), 8, B.T, B.t0, B.L, B.fstar);
         double2 dl[S::PER][6];
         fd_link_spectra<N, NTHR>(k, lds, opaque(B.pos), kPosN / N, opaque(B.tw), kTwN / N, dl);
-        double2* o = dst + (size_t)(dslot ? dslot[e] : e) * 3 * B.bwcap;
+        const int c = item_chain(it, e), cap = chain_cap(B, c);
+        double2* o = to_hnew ? B.hnew + B.c_hoff[c] : src_tmpl(B, c, item_src(it, e));
 #pragma unroll
         for (int kk = 0; kk < S::PER; kk++)
         {
@@ -303,36 +320,36 @@ __global__ __launch_bounds__(NTHR) void tmpl_kernel(const int* __restrict__ list
             double2 v[5];
             fd_tdi<N>(k, m, B.fmt, dl[kk], v);
 #pragma unroll
-            for (int ch = 0; ch < 3; ch++) o[ch * B.bwcap + m] = v[ch];
+            for (int ch = 0; ch < 3; ch++) o[ch * cap + m] = v[ch];
         }
     }
 }
 
 // r_c = d_g - sum_s h_cs (add_ucb_model, ucb/src/glass_ucb_model.c:192-232)
-__global__ void resid_kernel(BatchDev B)
+// One workgroup per chain (grid = chain count), striding over its window.
+__global__ __launch_bounds__(256) void resid_kernel(BatchDev B)
 {
-    const int c = blockIdx.y;
-    const int g = B.c_group[c], W = B.g_W[g];
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= W) return;
+    const int c = blockIdx.x;
+    const int g = B.c_group[c], W = B.g_W[g], cap = B.g_cap[g];
     const double2* D = B.data + (size_t)B.g_off[g] * 3;
     double2* R = B.resid + (size_t)B.c_roff[c] * 3;
-    for (int ch = 0; ch < 3; ch++)
-    {
-        double2 v = D[(size_t)ch * W + j];
-        for (int s = 0; s < B.c_nsrc[c]; s++)
+    for (int j = threadIdx.x; j < W; j += blockDim.x)
+        for (int ch = 0; ch < 3; ch++)
         {
-            const int so = c * kMaxSrc + s;
-            const int m = j - B.s_imin[so];
-            if (m >= 0 && m < B.s_bw[so])
+            double2 v = D[(size_t)ch * W + j];
+            for (int s = 0; s < B.c_nsrc[c]; s++)
             {
-                double2 h = B.tmpl[(size_t)so * 3 * B.bwcap + ch * B.bwcap + m];
-                v.x -= h.x;
-                v.y -= h.y;
+                const int so = c * kMaxSrc + s;
+                const int m = j - B.s_imin[so];
+                if (m >= 0 && m < B.s_bw[so])
+                {
+                    const double2 h = src_tmpl(B, c, s)[ch * cap + m];
+                    v.x -= h.x;
+                    v.y -= h.y;
+                }
             }
+            R[(size_t)ch * W + j] = v;
         }
-        R[(size_t)ch * W + j] = v;
-    }
 }
 
 // logL_c = -1/2 (r_c | r_c) over the whole window
@@ -355,50 +372,52 @@ __global__ __launch_bounds__(256) void chain_logl_kernel(BatchDev B)
     if (threadIdx.x == 0) B.logl[c] = -0.5 * acc;
 }
 
-// Accept: residual += h_old - h_new over the union of supports.
-__global__ void accept_resid_kernel(int nacc, Items it, BatchDev B,
-                                    const double2* __restrict__ hnew, const int* __restrict__ bw_new,
-                                    const int* __restrict__ imin_new)
+// Accept: residual += h_old - h_new over the union of supports; one
+// workgroup per accepted proposal (grid = nacc), striding over the union.
+__global__ __launch_bounds__(256) void accept_resid_kernel(int nacc, Items it, BatchDev B,
+                                                           const int* __restrict__ bw_new,
+                                                           const int* __restrict__ imin_new)
 {
-    const int a = blockIdx.y;
+    const int a = blockIdx.x;
     if (a >= nacc) return;
     const int c = item_chain(it, a), s = item_src(it, a), so = c * kMaxSrc + s;
     const int g = B.c_group[c], W = B.g_W[g];
     const int in = imin_new[a], bn = bw_new[a], io = B.s_imin[so], bo = B.s_bw[so];
     const int lo = max(0, min(in, io)), hi = min(W, max(in + bn, io + bo));
-    const int j = lo + blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= hi) return;
     double2* R = B.resid + (size_t)B.c_roff[c] * 3;
-    const double2* ho = B.tmpl + (size_t)so * 3 * B.bwcap;
-    const double2* hn = hnew + (size_t)a * 3 * B.bwcap;
-    for (int ch = 0; ch < 3; ch++)
-    {
-        double2 v = R[(size_t)ch * W + j];
-        int mo = j - io, mn = j - in;
-        if (mo >= 0 && mo < bo)
+    const int cap = B.g_cap[g];
+    const double2* ho = src_tmpl(B, c, s);
+    const double2* hn = B.hnew + B.c_hoff[c];
+    for (int j = lo + threadIdx.x; j < hi; j += blockDim.x)
+        for (int ch = 0; ch < 3; ch++)
         {
-            v.x += ho[ch * B.bwcap + mo].x;
-            v.y += ho[ch * B.bwcap + mo].y;
+            double2 v = R[(size_t)ch * W + j];
+            const int mo = j - io, mn = j - in;
+            if (mo >= 0 && mo < bo)
+            {
+                v.x += ho[ch * cap + mo].x;
+                v.y += ho[ch * cap + mo].y;
+            }
+            if (mn >= 0 && mn < bn)
+            {
+                v.x -= hn[ch * cap + mn].x;
+                v.y -= hn[ch * cap + mn].y;
+            }
+            R[(size_t)ch * W + j] = v;
         }
-        if (mn >= 0 && mn < bn)
-        {
-            v.x -= hn[ch * B.bwcap + mn].x;
-            v.y -= hn[ch * B.bwcap + mn].y;
-        }
-        R[(size_t)ch * W + j] = v;
-    }
 }
 
-__global__ void accept_commit_kernel(int nacc, Items it, BatchDev B, const double2* __restrict__ hnew,
-                                     const int* __restrict__ bw_new, const int* __restrict__ imin_new,
+__global__ void accept_commit_kernel(int nacc, Items it, BatchDev B, const int* __restrict__ bw_new,
+                                     const int* __restrict__ imin_new,
                                      const double* __restrict__ logl_new)
 {
     const int a = blockIdx.x;
     if (a >= nacc) return;
     const int c = item_chain(it, a), s = item_src(it, a), so = c * kMaxSrc + s;
-    const double2* hn = hnew + (size_t)a * 3 * B.bwcap;
-    double2* ho = B.tmpl + (size_t)so * 3 * B.bwcap;
-    for (int i = threadIdx.x; i < 3 * B.bwcap; i += blockDim.x) ho[i] = hn[i];
+    const int cap = chain_cap(B, c);
+    const double2* hn = B.hnew + B.c_hoff[c];
+    double2* ho = src_tmpl(B, c, s);
+    for (int i = threadIdx.x; i < 3 * cap; i += blockDim.x) ho[i] = hn[i];
     if (threadIdx.x < 8) B.sparams[(size_t)so * 8 + threadIdx.x] = item_params(it, a)[threadIdx.x];
     if (threadIdx.x == 0)
     {
@@ -411,6 +430,62 @@ __global__ void accept_commit_kernel(int nacc, Items it, BatchDev B, const doubl
 }  // namespace glass
 
 using namespace glass;
+
+// Exact derivatives of chain sources at their current parameters: per-item
+// arguments of the derivative kernels (fd_derivs.h) and BW-class lists
+// (BW 32..1024, classes 0..5).  Items outside those (another BW, a chain or
+// source index out of range) get NaN outputs.
+constexpr int kNDClass = 6;
+
+struct DerivItems
+{
+    int *pidx, *imin, *W, *tcap;
+    long long *roff, *ioff, *toff;
+    int* cnt;   // [kNDClass]
+    int* list;  // [kNDClass][cap]
+};
+
+__global__ __launch_bounds__(256) void derivs_prep_kernel(int n, const int* __restrict__ chain,
+                                                          const int* __restrict__ src, int nchain, BatchDev B,
+                                                          DerivItems D, int cap, double* __restrict__ outL,
+                                                          double* __restrict__ grad, double* __restrict__ fisher)
+{
+    __shared__ int hist[kNDClass + 1], base[kNDClass + 1];
+    if (threadIdx.x <= kNDClass) hist[threadIdx.x] = 0;
+    __syncthreads();
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    int k = -1, slot = 0;
+    if (e < n)
+    {
+        const int c = chain[e], s = src[e];
+        k = kNDClass;
+        if (c >= 0 && c < nchain && s >= 0 && s < B.c_nsrc[c])
+        {
+            const int so = c * kMaxSrc + s, g = B.c_group[c];
+            D.pidx[e] = so;
+            D.imin[e] = B.s_imin[so];
+            D.W[e] = B.g_W[g];
+            D.roff[e] = B.c_roff[c] * 3;
+            D.ioff[e] = B.g_off[g] * 6;
+            D.toff[e] = B.c_toff[c] + (long long)s * 3 * B.g_cap[g];
+            D.tcap[e] = B.g_cap[g];
+            const int bw = B.s_bw[so];
+            if (bw >= 32 && bw <= 1024 && (bw & (bw - 1)) == 0) k = 31 - __clz(bw) - 5;
+        }
+        slot = atomicAdd(&hist[k], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < kNDClass && hist[threadIdx.x]) base[threadIdx.x] = atomicAdd(&D.cnt[threadIdx.x], hist[threadIdx.x]);
+    __syncthreads();
+    if (k >= 0 && k < kNDClass) D.list[k * cap + base[k] + slot] = e;
+    if (k == kNDClass)
+    {
+        const double nan = __builtin_nan("");
+        outL[e] = nan;
+        for (int q = 0; q < 8; q++) grad[(size_t)e * 8 + q] = nan;
+        for (int q = 0; q < 64; q++) fisher[(size_t)e * 64 + q] = nan;
+    }
+}
 
 struct glass_hip_fd_batch
 {
@@ -432,7 +507,8 @@ struct glass_hip_fd_batch
     // init items: (chain, src) of every live source, its params gathered
     int *i_chain = nullptr, *i_src = nullptr, *i_slot = nullptr;
     double* i_params = nullptr;
-    double2* hnew = nullptr;  // accept scratch [C][3][bwcap]
+    size_t tmpl_bytes = 0;    // template slots + accept slots
+    DerivItems D{};
     int ncu = 0;
     int tgrid[kNClass] = {0};
     glass_hip_tile tile[kNClass] = {};  // launch shape of each class's eval kernel
@@ -529,7 +605,7 @@ static int launch_eval(glass_hip_fd_batch* P, int k, glass_hip_tile t, int n, co
 
 // alignment + per-class launches for items; mode 0: eval, 1: templates
 static int launch_items(glass_hip_fd_batch* P, int n, const Items& it, int mode, double* logl_out, double* snr_out,
-                        double2* dst, const int* dslot, hipStream_t s)
+                        int to_hnew, hipStream_t s)
 {
     if (n <= 0) return GLASS_HIP_OK;
     if (n > P->cls_cap)
@@ -556,7 +632,7 @@ static int launch_items(glass_hip_fd_batch* P, int n, const Items& it, int mode,
             if (g > n) g = n;
 #define GH_L(kk, N)                                                                               \
     case kk:                                                                                      \
-        tmpl_kernel<N, GH_NTHR(N)><<<g, GH_NTHR(N), 0, s>>>(list, cnt, it, P->B, dst, dslot);    \
+        tmpl_kernel<N, GH_NTHR(N)><<<g, GH_NTHR(N), 0, s>>>(list, cnt, it, P->B, to_hnew);       \
         break;
             switch (k)
             {
@@ -610,10 +686,9 @@ static int rebuild(glass_hip_fd_batch* P, hipStream_t s)
     gather_params_kernel<<<(n * 8 + 255) / 256, 256, 0, s>>>(n, P->i_slot, P->B.sparams, P->i_params);
     Items it{P->i_chain, P->i_src, P->i_params, nullptr};
     int rc;
-    if ((rc = launch_items(P, n, it, 1, nullptr, nullptr, P->B.tmpl, P->i_slot, s))) return rc;
+    if ((rc = launch_items(P, n, it, 1, nullptr, nullptr, 0, s))) return rc;
     slot_align_kernel<<<(n + 255) / 256, 256, 0, s>>>(n, P->i_slot, P->p_bw, P->p_imin, P->B.s_bw, P->B.s_imin);
-    dim3 rg((P->Wmax + 255) / 256, P->C);
-    resid_kernel<<<rg, 256, 0, s>>>(P->B);
+    resid_kernel<<<P->C, 256, 0, s>>>(P->B);
     chain_logl_kernel<<<P->C, 256, 0, s>>>(P->B);
     GH_CHECK(hipGetLastError());
     return GLASS_HIP_OK;
@@ -695,16 +770,32 @@ extern "C" int glass_hip_fd_batch_create(glass_hip_fd_batch** plan, const glass_
     P->Wmax = Wmax;
     P->cls_cap = std::max(max_prop, std::max(P->ninit, nchain));
     GH_CHECK(hipGetDevice(&P->device));
-    // largest BW any window allows: 2 * 2^floor(log2(W/2)) <= W
+    // template slot capacity per window: the largest BW the window allows,
+    // 2 * 2^floor(log2(W/2)) <= W (ucb_bandwidth clamps to it), at most 4096
+    std::vector<int> gcap(ngroup);
     P->B.bwcap = 32;
-    while (P->B.bwcap * 2 <= Wmax && P->B.bwcap < kPosN) P->B.bwcap *= 2;
+    for (int g = 0; g < ngroup; g++)
+    {
+        int cp = 32;
+        while (cp * 2 <= group_W[g] && cp < kPosN) cp *= 2;
+        gcap[g] = cp;
+        P->B.bwcap = std::max(P->B.bwcap, cp);
+    }
+    std::vector<long long> toff(nchain + 1, 0), hoff(nchain + 1, 0);
+    for (int c = 0; c < nchain; c++)
+    {
+        const long long cp = gcap[chain_group[c]];
+        toff[c + 1] = toff[c] + (long long)chain_nsrc[c] * 3 * cp;
+        hoff[c + 1] = hoff[c] + 3 * cp;
+    }
+    P->tmpl_bytes = sizeof(double2) * (size_t)(toff[nchain] + hoff[nchain]);
     auto fail = [&](int r) {
         glass_hip_fd_batch_destroy(P);
         return r;
     };
-    int *g_qmin, *g_W, *c_group, *c_nsrc;
-    long long *g_off, *c_roff;
-    double2 *ddata, *resid, *tmpl;
+    int *g_qmin, *g_W, *c_group, *c_nsrc, *g_cap;
+    long long *g_off, *c_roff, *c_toff, *c_hoff;
+    double2 *ddata, *resid, *tmpl, *hnew;
     double *dinvC, *logl, *sparams;
     int *s_bw, *s_imin;
     const size_t nslot = (size_t)nchain * kMaxSrc;
@@ -713,21 +804,28 @@ extern "C" int glass_hip_fd_batch_create(glass_hip_fd_batch** plan, const glass_
         (rc = dalloc(P, &c_group, nchain)) || (rc = dalloc(P, &c_nsrc, nchain)) || (rc = dalloc(P, &c_roff, nchain)) ||
         (rc = dalloc(P, &resid, 3 * (size_t)roff[nchain])) || (rc = dalloc(P, &logl, nchain)) ||
         (rc = dalloc(P, &sparams, nslot * 8)) || (rc = dalloc(P, &s_bw, nslot)) || (rc = dalloc(P, &s_imin, nslot)) ||
-        (rc = dalloc(P, &tmpl, nslot * 3 * (size_t)P->B.bwcap)) || (rc = dalloc(P, &P->p_chain, P->cls_cap)) ||
+        (rc = dalloc(P, &tmpl, (size_t)toff[nchain])) || (rc = dalloc(P, &hnew, (size_t)hoff[nchain])) ||
+        (rc = dalloc(P, &g_cap, ngroup)) || (rc = dalloc(P, &c_toff, nchain)) || (rc = dalloc(P, &c_hoff, nchain)) ||
+        (rc = dalloc(P, &P->p_chain, P->cls_cap)) ||
         (rc = dalloc(P, &P->p_src, P->cls_cap)) || (rc = dalloc(P, &P->p_params, (size_t)P->cls_cap * 8)) ||
         (rc = dalloc(P, &P->p_logl, P->cls_cap)) || (rc = dalloc(P, &P->p_snr, P->cls_cap)) ||
         (rc = dalloc(P, &P->p_bw, P->cls_cap)) || (rc = dalloc(P, &P->p_imin, P->cls_cap)) ||
         (rc = dalloc(P, &P->cls, kNClass + 1)) || (rc = dalloc(P, &P->cls_list, (size_t)kNClass * P->cls_cap)) ||
         (rc = dalloc(P, &P->i_chain, P->ninit)) || (rc = dalloc(P, &P->i_src, P->ninit)) ||
         (rc = dalloc(P, &P->i_slot, P->ninit)) || (rc = dalloc(P, &P->i_params, (size_t)P->ninit * 8)) ||
-        (rc = dalloc(P, &P->hnew, (size_t)nchain * 3 * P->B.bwcap)))
+        (rc = dalloc(P, &P->D.pidx, P->cls_cap)) || (rc = dalloc(P, &P->D.imin, P->cls_cap)) ||
+        (rc = dalloc(P, &P->D.W, P->cls_cap)) || (rc = dalloc(P, &P->D.tcap, P->cls_cap)) ||
+        (rc = dalloc(P, &P->D.roff, P->cls_cap)) || (rc = dalloc(P, &P->D.ioff, P->cls_cap)) ||
+        (rc = dalloc(P, &P->D.toff, P->cls_cap)) || (rc = dalloc(P, &P->D.cnt, kNDClass)) ||
+        (rc = dalloc(P, &P->D.list, (size_t)kNDClass * P->cls_cap)))
         return fail(rc);
     auto up = [&](void* d, const void* h, size_t b) { return hipMemcpy(d, h, b, hipMemcpyHostToDevice); };
     if (up(g_qmin, group_qmin, sizeof(int) * ngroup) || up(g_W, group_W, sizeof(int) * ngroup) ||
         up(g_off, goff.data(), sizeof(long long) * ngroup) || up(ddata, data, 48 * (size_t)goff[ngroup]) ||
         up(dinvC, invC, 48 * (size_t)goff[ngroup]) || up(c_group, chain_group, sizeof(int) * nchain) ||
         up(c_nsrc, chain_nsrc, sizeof(int) * nchain) || up(c_roff, roff.data(), sizeof(long long) * nchain) ||
-        up(sparams, src_params, sizeof(double) * nslot * 8) ||
+        up(sparams, src_params, sizeof(double) * nslot * 8) || up(g_cap, gcap.data(), sizeof(int) * ngroup) ||
+        up(c_toff, toff.data(), sizeof(long long) * nchain) || up(c_hoff, hoff.data(), sizeof(long long) * nchain) ||
         (P->ninit && (up(P->i_chain, ic.data(), sizeof(int) * P->ninit) ||
                       up(P->i_src, is.data(), sizeof(int) * P->ninit) ||
                       up(P->i_slot, islot.data(), sizeof(int) * P->ninit))))
@@ -753,6 +851,10 @@ extern "C" int glass_hip_fd_batch_create(glass_hip_fd_batch** plan, const glass_
     B.s_bw = s_bw;
     B.s_imin = s_imin;
     B.tmpl = tmpl;
+    B.hnew = hnew;
+    B.g_cap = g_cap;
+    B.c_toff = c_toff;
+    B.c_hoff = c_hoff;
     B.T = T;
     B.t0 = t0;
     B.L = orbit->L;
@@ -807,7 +909,7 @@ extern "C" int glass_hip_fd_batch_eval(glass_hip_fd_batch* P, int nprop, const i
     }
     GH_CHECK(hipSetDevice(P->device));
     Items it{d_chain, d_src, d_params, nullptr};
-    return launch_items(P, nprop, it, 0, d_logL, d_snr, nullptr, nullptr, (hipStream_t)stream);
+    return launch_items(P, nprop, it, 0, d_logL, d_snr, 0, (hipStream_t)stream);
 }
 
 extern "C" int glass_hip_fd_batch_eval_graph(glass_hip_fd_batch* P, int nprop, const int* d_chain, const int* d_src,
@@ -864,10 +966,9 @@ extern "C" int glass_hip_fd_batch_accept(glass_hip_fd_batch* P, int nacc, const 
     Items it{d_chain, d_src, d_params, d_idx};
     int rc;
     // new templates into the scratch slots 0..nacc (item a -> slot a)
-    if ((rc = launch_items(P, nacc, it, 1, nullptr, nullptr, P->hnew, nullptr, s))) return rc;
-    dim3 g((2 * P->B.bwcap + 255) / 256, nacc);
-    accept_resid_kernel<<<g, 256, 0, s>>>(nacc, it, P->B, P->hnew, P->p_bw, P->p_imin);
-    accept_commit_kernel<<<nacc, 256, 0, s>>>(nacc, it, P->B, P->hnew, P->p_bw, P->p_imin, d_logL);
+    if ((rc = launch_items(P, nacc, it, 1, nullptr, nullptr, 1, s))) return rc;
+    accept_resid_kernel<<<nacc, 256, 0, s>>>(nacc, it, P->B, P->p_bw, P->p_imin);
+    accept_commit_kernel<<<nacc, 256, 0, s>>>(nacc, it, P->B, P->p_bw, P->p_imin, d_logL);
     GH_CHECK(hipGetLastError());
     return GLASS_HIP_OK;
 }
@@ -1068,4 +1169,54 @@ extern "C" int glass_hip_fd_batch_autotune(glass_hip_fd_batch* P, int nprop, con
     (void)hipEventDestroy(e1);
     if (!rc && hipGetLastError() != hipSuccess) rc = GLASS_HIP_ERUNTIME;
     return rc;
+}
+
+// ------------------------------------------------------- derivatives ---
+
+extern "C" int glass_hip_fd_batch_derivs(glass_hip_fd_batch* P, int n, const int* d_chain, const int* d_src,
+                                         double* d_logL, double* d_grad, double* d_fisher, void* stream)
+{
+    if (!P || n < 0 || n > P->cls_cap || (n && (!d_chain || !d_src || !d_logL || !d_grad || !d_fisher)))
+    {
+        set_error("fd_batch_derivs: bad arguments (n %d, capacity %d)", n, P ? P->cls_cap : 0);
+        return GLASS_HIP_EINVAL;
+    }
+    if (n == 0) return GLASS_HIP_OK;
+    GH_CHECK(hipSetDevice(P->device));
+    hipStream_t s = (hipStream_t)stream;
+    zero_int_kernel<<<1, 64, 0, s>>>(P->D.cnt, kNDClass);
+    derivs_prep_kernel<<<(n + 255) / 256, 256, 0, s>>>(n, d_chain, d_src, P->C, P->B, P->D, P->cls_cap, d_logL,
+                                                       d_grad, d_fisher);
+    GH_CHECK(hipGetLastError());
+    const BatchDev& B = P->B;
+    DerivArgs A{};
+    A.params = B.sparams;
+    A.pidx = P->D.pidx;
+    A.imin = P->D.imin;
+    A.iW = P->D.W;
+    A.resid = B.resid;
+    A.iroff = P->D.roff;
+    A.invC = B.invC;
+    A.iioff = P->D.ioff;
+    A.tadd = B.tmpl;
+    A.taddoff = P->D.toff;
+    A.taddcap = P->D.tcap;
+    A.mode = 0;
+    A.T = B.T;
+    A.t0 = B.t0;
+    A.L = B.L;
+    A.fstar = B.fstar;
+    A.pos = B.pos;
+    A.tw = B.tw;
+    A.fmt = B.fmt;
+    A.outL = d_logL;
+    A.grad = d_grad;
+    A.fisher = d_fisher;
+    for (int k = 0; k < kNDClass; k++)
+    {
+        A.list = P->D.list + (size_t)k * P->cls_cap;
+        A.count = P->D.cnt + k;
+        if (int rc = launch_derivs(32 << k, A, n, s)) return rc;
+    }
+    return GLASS_HIP_OK;
 }

@@ -15,6 +15,7 @@
 #include <cmath>
 
 #include "../../include/glass_hip_batch.h"
+#include "fd_derivs.h"
 #include "fd_tangent.h"
 #include "reduce.h"
 #include "runtime.h"
@@ -55,25 +56,6 @@ __device__ inline void block_sum_vec(double (&v)[kNSum], double* out)
     }
     __syncthreads();
 }
-
-struct DerivArgs
-{
-    const double* params;
-    const int* imin;
-    int W;
-    const double2* resid;
-    long long rstride;  // doubles
-    const double* invC;
-    long long istride;  // doubles
-    int mode;           // 0: window-aligned weights; 1: template-local (reference)
-    double T, t0, L, fstar;
-    const double* pos;
-    const double2* tw;
-    int fmt;
-    double* outL;
-    double* grad;
-    double* fisher;
-};
 
 // One pass of series sets [S0, S0+NS): samples -> LDS -> FFT -> TDI into H.
 template <int N, int NTHR, int S0, int NS>
@@ -132,8 +114,13 @@ __global__ __launch_bounds__(NTHR) void deriv_kernel(DerivArgs A)
     __shared__ double2 lds[6 * NSMAX * N];
     __shared__ UcbDiff sdd;
     __shared__ double red[kNSum];
-    const int e = blockIdx.x;
-    const double* p = A.params + (size_t)e * 8;
+    int e = blockIdx.x;
+    if (A.list)
+    {
+        if (e >= *A.count) return;  // uniform over the workgroup
+        e = A.list[e];
+    }
+    const double* p = A.params + (size_t)(A.pidx ? A.pidx[e] : e) * 8;
     const UcbConst& c = ucb_setup_shared(p, 8, A.T, A.t0, A.L, A.fstar);
     if (threadIdx.x == 0) ucb_diff_setup(p, A.T, sdd);
     __syncthreads();
@@ -141,9 +128,11 @@ __global__ __launch_bounds__(NTHR) void deriv_kernel(DerivArgs A)
     double2 H[S::PER][6][3];
     deriv_passes<N, NTHR, 0>(c, dd, lds, A, H);
 
-    const double2* R = A.resid + (size_t)e * (A.rstride / 2);
-    const double* iC = A.invC + (size_t)e * A.istride;
-    const int W = A.W, imin = A.imin[e];
+    const double2* R = A.iroff ? A.resid + A.iroff[e] : A.resid + (size_t)e * (A.rstride / 2);
+    const double* iC = A.iioff ? A.invC + A.iioff[e] : A.invC + (size_t)e * A.istride;
+    const int W = A.iW ? A.iW[e] : A.W, imin = A.imin[e];
+    const double2* Ha = A.tadd ? A.tadd + A.taddoff[e] : nullptr;
+    const int hcap = A.tadd ? A.taddcap[e] : 0;
     const double2 DP = make_double2(c.DPr, c.DPi), DC = make_double2(c.DCr, c.DCi);
     const double2 DP4 = make_double2(dd.DPr4, dd.DPi4), DC4 = make_double2(dd.DCr4, dd.DCi4);
     const double2 DP5 = make_double2(dd.DPr5, dd.DPi5), DC5 = make_double2(dd.DCr5, dd.DCi5);
@@ -177,6 +166,13 @@ __global__ __launch_bounds__(NTHR) void deriv_kernel(DerivArgs A)
             dh[7][ch] = H[k][5][ch];
             const bool inwin = j >= 0 && j < W;
             r[ch] = (A.resid && inwin) ? R[(size_t)ch * W + j] : make_double2(0.0, 0.0);
+            if (Ha && inwin)
+            {
+                // the residual of the whole model plus this source's stored
+                // template: the residual without the source
+                r[ch].x += Ha[(size_t)ch * hcap + m].x;
+                r[ch].y += Ha[(size_t)ch * hcap + m].y;
+            }
         }
         // likelihood terms only where the template lands in the window
         if (A.mode == 0 || (j >= 0 && j < W))
@@ -212,6 +208,23 @@ __global__ __launch_bounds__(NTHR) void deriv_kernel(DerivArgs A)
     }
 }
 
+int launch_derivs(int N, const DerivArgs& A, int grid, hipStream_t s)
+{
+    if (grid <= 0) return GLASS_HIP_OK;
+    switch (N)
+    {
+#define GH_D(NN)                                                                   \
+    case NN:                                                                       \
+        deriv_kernel<NN, fd_threads(NN)><<<grid, fd_threads(NN), 0, s>>>(A);       \
+        break;
+        GH_D(32) GH_D(64) GH_D(128) GH_D(256) GH_D(512) GH_D(1024)
+#undef GH_D
+        default: set_error("fd derivs: BW=%d unsupported (power of two 32..1024)", N); return GLASS_HIP_EINVAL;
+    }
+    GH_CHECK(hipGetLastError());
+    return GLASS_HIP_OK;
+}
+
 }  // namespace glass
 
 using namespace glass;
@@ -239,19 +252,24 @@ extern "C" int glass_hip_fd_source_derivs(const glass_hip_orbit* orbit, const ch
     const double2* tw;
     int rc;
     if ((rc = positions(*orbit, T, t0, 4096, &pos)) || (rc = twiddles(&tw))) return rc;
-    DerivArgs A{d_params, d_imin, W, (const double2*)d_resid, resid_stride, d_invC, invC_stride, mode, T, t0,
-                orbit->L, orbit->fstar, pos, tw, fmt, d_logL, d_grad, d_fisher};
-    hipStream_t s = (hipStream_t)stream;
-    switch (BW)
-    {
-#define GH_D(NN)                                                                   \
-    case NN:                                                                       \
-        deriv_kernel<NN, fd_threads(NN)><<<n, fd_threads(NN), 0, s>>>(A);          \
-        break;
-        GH_D(32) GH_D(64) GH_D(128) GH_D(256) GH_D(512) GH_D(1024)
-#undef GH_D
-        default: set_error("fd_source_derivs: BW=%d unsupported (power of two 32..1024)", BW); return GLASS_HIP_EINVAL;
-    }
-    GH_CHECK(hipGetLastError());
-    return GLASS_HIP_OK;
+    DerivArgs A{};
+    A.params = d_params;
+    A.imin = d_imin;
+    A.W = W;
+    A.resid = (const double2*)d_resid;
+    A.rstride = resid_stride;
+    A.invC = d_invC;
+    A.istride = invC_stride;
+    A.mode = mode;
+    A.T = T;
+    A.t0 = t0;
+    A.L = orbit->L;
+    A.fstar = orbit->fstar;
+    A.pos = pos;
+    A.tw = tw;
+    A.fmt = fmt;
+    A.outL = d_logL;
+    A.grad = d_grad;
+    A.fisher = d_fisher;
+    return launch_derivs(BW, A, n, (hipStream_t)stream);
 }

@@ -29,6 +29,11 @@
  *   data:  per group, 3 channels x W_g complex, groups concatenated
  *   invC:  per group, 6 planes x W_g: XX, YY, ZZ, XY, XZ, YZ
  *   src_params: C x 8 x 8 (unused source slots ignored)
+ * Device memory beyond the data: per chain its residual (3 W_g complex)
+ * and, for each live source plus one accept slot, 3 cap_g complex with cap_g
+ * the largest power of two <= W_g (the largest BW ucb_bandwidth allows in
+ * the window); e.g. 4000 windows x 256 chains of ~780 bins with ~5 sources
+ * each take ~38 GB of residuals and ~150 GB of templates.
  * Entry points taking `stream` accept a hipStream_t (NULL = default
  * stream) and only enqueue work: they neither allocate nor synchronise, so
  * a caller may capture them in a hipGraph.  The plan's own graph path is
@@ -158,6 +163,21 @@ int glass_hip_fd_source_derivs(const glass_hip_orbit *orbit, const char *format,
                                long long resid_stride, const double *d_invC,
                                long long invC_stride, int mode, double *d_logL, double *d_grad,
                                double *d_fisher, void *stream);
+
+/* Exact derivatives of chain sources at their current parameters, from
+ * the plan's state (no reference counterpart; for Fisher-matrix proposals,
+ * draw_from_fisher, ucb/src/glass_ucb_proposal.c:348-391): item e = (chain
+ * d_chain[e], source d_src[e]) (device arrays, n <= the plan's capacity,
+ * max(max_prop, live sources, nchain)).  With R the chain's residual plus
+ * the source's stored template (the residual without the source) and the
+ * window's invC, as glass_hip_fd_source_derivs mode 0:
+ *   d_logL[e] = (R|h) - (h|h)/2,  d_grad[8e..] = (R - h | dh/dtheta),
+ *   d_fisher[64e..] = (dh/dtheta_k | dh/dtheta_l).
+ * Sources with BW above 1024, or invalid indices, get NaN outputs.  Graph
+ * safe (no allocation); shares the plan's class lists with evals, so it
+ * must not overlap an eval of the same plan on another stream. */
+int glass_hip_fd_batch_derivs(glass_hip_fd_batch *plan, int n, const int *d_chain, const int *d_src,
+                              double *d_logL, double *d_grad, double *d_fisher, void *stream);
 
 /* Batched full-window likelihood: n independent calls of
  *   double gaussian_log_likelihood(struct Data *data, struct Model *model)
