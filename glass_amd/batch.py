@@ -31,6 +31,10 @@ _EXTRA += [
     ("glass_hip_fd_batch_set_tiles", [_vp, _tp], _i),
     ("glass_hip_fd_batch_autotune", [_vp, _i, _vp, _vp, _vp, _i, _tp, _dp, _vp], _i),
     ("glass_hip_fd_batch_derivs", [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp], _i),
+    ("glass_hip_fd_batch_create_ex", [ctypes.POINTER(_vp), _vp, ctypes.c_char_p, _d, _d, _i, _ip, _ip, _dp, _dp,
+                                      _i, _ip, _ip, _i, _dp, _i], _i),
+    ("glass_hip_fd_batch_sweep_graph", [_vp, _i, _vp, _vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp, _vp], _i),
+    ("glass_hip_fd_batch_memory", [_vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)], _i),
     ("glass_hip_fd_batch_create", [ctypes.POINTER(_vp), _vp, ctypes.c_char_p, _d, _d, _i, _ip, _ip, _dp, _dp,
                                    _i, _ip, _ip, _dp, _i], _i),
     ("glass_hip_fd_batch_destroy", [_vp], _i),
@@ -96,30 +100,45 @@ class FdBatch:
     """A device-resident set of data windows and chains (see the header)."""
 
     def __init__(self, orbit, fmt, T, t0, group_qmin, data, invC, chain_group, chain_nsrc, src_params,
-                 max_prop):
+                 max_prop, flat=False):
         """data: list of complex [3, W_g]; invC: list of [3,3,W_g];
-        src_params: [nchain, 8, 8]."""
+        src_params: [nchain, max_src, 8] (max_src 8 uses
+        glass_hip_fd_batch_create, others glass_hip_fd_batch_create_ex).
+        With flat=True, data is already the flat complex array of all
+        windows and invC the flat [6 W_g] planes per window, and group_W is
+        passed as data's window sizes via the `W` attribute of group_qmin's
+        tuple (group_qmin, group_W)."""
         import torch
         self.torch = torch
         L = _L()
-        W = np.array([d.shape[-1] for d in data], dtype=np.int32)
-        dflat = np.concatenate([np.ascontiguousarray(d, dtype=np.complex128).reshape(-1) for d in data])
-        iflat = np.concatenate([pack_invC(np.asarray(c)).reshape(-1) for c in invC])
+        if flat:
+            group_qmin, W = group_qmin
+            W = np.ascontiguousarray(W, dtype=np.int32)
+            dflat = np.ascontiguousarray(data, dtype=np.complex128)
+            iflat = np.ascontiguousarray(invC, dtype=np.float64)
+        else:
+            W = np.array([d.shape[-1] for d in data], dtype=np.int32)
+            dflat = np.concatenate([np.ascontiguousarray(d, dtype=np.complex128).reshape(-1) for d in data])
+            iflat = np.concatenate([pack_invC(np.asarray(c)).reshape(-1) for c in invC])
         gq = np.ascontiguousarray(group_qmin, dtype=np.int32)
         cg = np.ascontiguousarray(chain_group, dtype=np.int32)
         cn = np.ascontiguousarray(chain_nsrc, dtype=np.int32)
         sp = np.ascontiguousarray(src_params, dtype=np.float64)
-        assert sp.shape == (len(cg), 8, 8)
+        assert sp.ndim == 3 and sp.shape[0] == len(cg) and sp.shape[2] == 8
         self.nchain = len(cg)
+        self.max_src = sp.shape[1]
         self.max_prop = int(max_prop)
         h = _vp()
         dv = _f64(dflat)
-        check(L.glass_hip_fd_batch_create(ctypes.byref(h), ctypes.cast(ctypes.byref(orbit), _vp),
-                                          fmt.encode(), float(T), float(t0), len(W),
-                                          gq.ctypes.data_as(_ip), W.ctypes.data_as(_ip),
-                                          dv.ctypes.data_as(_dp), _f64(iflat).ctypes.data_as(_dp),
-                                          len(cg), cg.ctypes.data_as(_ip), cn.ctypes.data_as(_ip),
-                                          sp.ctypes.data_as(_dp), self.max_prop), "fd_batch_create")
+        iv = _f64(iflat)
+        args = [ctypes.byref(h), ctypes.cast(ctypes.byref(orbit), _vp), fmt.encode(), float(T), float(t0), len(W),
+                gq.ctypes.data_as(_ip), W.ctypes.data_as(_ip), dv.ctypes.data_as(_dp), iv.ctypes.data_as(_dp),
+                len(cg), cg.ctypes.data_as(_ip), cn.ctypes.data_as(_ip)]
+        if self.max_src == 8:
+            check(L.glass_hip_fd_batch_create(*args, sp.ctypes.data_as(_dp), self.max_prop), "fd_batch_create")
+        else:
+            check(L.glass_hip_fd_batch_create_ex(*args, self.max_src, sp.ctypes.data_as(_dp), self.max_prop),
+                  "fd_batch_create_ex")
         self.h = h
         dev = torch.device("cuda")
         self.chain = torch.zeros(self.max_prop, dtype=torch.int32, device=dev)
@@ -220,10 +239,26 @@ class FdBatch:
               "fd_batch_derivs")
         return L, g, F
 
+    def sweep_graph(self, n, dchain, dsrc, dL, dgrad, dfisher):
+        """Eval of the first n proposals in the plan's buffers plus the exact
+        derivatives of the (dchain, dsrc) device tensors, replayed as one
+        hipGraph on the current stream."""
+        st = ctypes.c_void_p(self.torch.cuda.current_stream().cuda_stream)
+        check(_L().glass_hip_fd_batch_sweep_graph(self.h, int(n), _ptr(self.chain), _ptr(self.src),
+                                                  _ptr(self.params), _ptr(self.logL), _ptr(self.snr),
+                                                  int(dchain.numel()), _ptr(dchain), _ptr(dsrc), _ptr(dL),
+                                                  _ptr(dgrad), _ptr(dfisher), st), "fd_batch_sweep_graph")
+
+    def memory(self):
+        """(template bytes, total device bytes) the plan holds."""
+        a, b = ctypes.c_longlong(), ctypes.c_longlong()
+        check(_L().glass_hip_fd_batch_memory(self.h, ctypes.byref(a), ctypes.byref(b)), "fd_batch_memory")
+        return a.value, b.value
+
     def state(self):
         self.torch.cuda.current_stream().synchronize()
         lg = np.zeros(self.nchain)
-        sp = np.zeros((self.nchain, 8, 8))
+        sp = np.zeros((self.nchain, self.max_src, 8))
         check(_L().glass_hip_fd_batch_state(self.h, lg.ctypes.data_as(_dp), sp.ctypes.data_as(_dp)),
               "fd_batch_state")
         return lg, sp

@@ -35,7 +35,8 @@ namespace glass {
 
 int parse_format(const char* format);
 
-constexpr int kMaxSrc = 8;
+constexpr int kMaxSrcDefault = 8;   // source slots per chain of glass_hip_fd_batch_create
+constexpr int kMaxSrcLimit = 64;    // largest max_src of glass_hip_fd_batch_create_ex
 constexpr int kNClass = 8;     // BW = 32 << k, k = 0..7 (32..4096)
 constexpr int kPosN = 4096;    // positions table samples (largest BW)
 
@@ -58,14 +59,12 @@ struct BatchDev
     int* s_imin;              // [C*8]
     // templates: chain c's live sources s < c_nsrc[c] hold [3][cap] complex
     // at tmpl + c_toff[c] + 3 s cap, cap = g_cap[group] (the largest BW the
-    // window allows: the largest power of two <= W); one accept slot of
-    // [3][cap] per chain at hnew + c_hoff[c]
+    // window allows: the largest power of two <= W)
     double2* tmpl;
-    double2* hnew;
     const int* g_cap;
     const long long* c_toff;
-    const long long* c_hoff;
     int bwcap;                // largest g_cap
+    int maxsrc;               // source slots per chain in sparams / s_bw / s_imin
     // constants
     double T, t0, L, fstar;
     const double* pos;  // positions table, kPosN samples
@@ -199,7 +198,7 @@ __device__ __forceinline__ void eval_item(int w, double2* lds, const int* __rest
         double2 dl[S::PER][6];
         fd_link_spectra<N, NTHR>(k, lds, opaque(B.pos), kPosN / N, opaque(B.tw), kTwN / N, dl);
 
-        const int so = c * kMaxSrc + s;
+        const int so = c * B.maxsrc + s;
         const int imin_n = p_imin[e];
         const int imin_o = B.s_imin[so], bw_o = B.s_bw[so];
         const int cap = B.g_cap[g];
@@ -295,11 +294,10 @@ __global__ __launch_bounds__(NTHR) void eval_kernel(const int* __restrict__ list
     X(4, 512, 512) X(4, 512, 256) X(4, 512, 128) X(5, 1024, 512) X(5, 1024, 1024) X(5, 1024, 256)     \
     X(6, 2048, 512) X(6, 2048, 1024) X(7, 4096, 512)
 
-// Template of each listed item into its source's slot, or (to_hnew) into
-// its chain's accept slot.
+// Template of each listed item into its source's slot.
 template <int N, int NTHR>
 __global__ __launch_bounds__(NTHR) void tmpl_kernel(const int* __restrict__ list, const int* __restrict__ count,
-                                                    Items it, BatchDev B, int to_hnew)
+                                                    Items it, BatchDev B)
 {
     using S = FdShape<N, NTHR>;
     __shared__ double2 lds[S::LDS];
@@ -311,7 +309,7 @@ __global__ __launch_bounds__(NTHR) void tmpl_kernel(const int* __restrict__ list
         double2 dl[S::PER][6];
         fd_link_spectra<N, NTHR>(k, lds, opaque(B.pos), kPosN / N, opaque(B.tw), kTwN / N, dl);
         const int c = item_chain(it, e), cap = chain_cap(B, c);
-        double2* o = to_hnew ? B.hnew + B.c_hoff[c] : src_tmpl(B, c, item_src(it, e));
+        double2* o = src_tmpl(B, c, item_src(it, e));
 #pragma unroll
         for (int kk = 0; kk < S::PER; kk++)
         {
@@ -339,7 +337,7 @@ __global__ __launch_bounds__(256) void resid_kernel(BatchDev B)
             double2 v = D[(size_t)ch * W + j];
             for (int s = 0; s < B.c_nsrc[c]; s++)
             {
-                const int so = c * kMaxSrc + s;
+                const int so = c * B.maxsrc + s;
                 const int m = j - B.s_imin[so];
                 if (m >= 0 && m < B.s_bw[so])
                 {
@@ -372,57 +370,55 @@ __global__ __launch_bounds__(256) void chain_logl_kernel(BatchDev B)
     if (threadIdx.x == 0) B.logl[c] = -0.5 * acc;
 }
 
-// Accept: residual += h_old - h_new over the union of supports; one
-// workgroup per accepted proposal (grid = nacc), striding over the union.
-__global__ __launch_bounds__(256) void accept_resid_kernel(int nacc, Items it, BatchDev B,
-                                                           const int* __restrict__ bw_new,
-                                                           const int* __restrict__ imin_new)
+// Accept, in place: (1) residual += h_old over the old support, (2) the new
+// template overwrites the source's slot, (3) residual -= h_new over the new
+// support and the source takes its new parameters.  Per bin the residual
+// sees r + h_old - h_new in that order.  One workgroup per accepted
+// proposal (grid = nacc).
+__global__ __launch_bounds__(256) void accept_add_old_kernel(int nacc, Items it, BatchDev B)
 {
     const int a = blockIdx.x;
     if (a >= nacc) return;
-    const int c = item_chain(it, a), s = item_src(it, a), so = c * kMaxSrc + s;
-    const int g = B.c_group[c], W = B.g_W[g];
-    const int in = imin_new[a], bn = bw_new[a], io = B.s_imin[so], bo = B.s_bw[so];
-    const int lo = max(0, min(in, io)), hi = min(W, max(in + bn, io + bo));
+    const int c = item_chain(it, a), s = item_src(it, a), so = c * B.maxsrc + s;
+    const int g = B.c_group[c], W = B.g_W[g], cap = B.g_cap[g];
+    const int io = B.s_imin[so], bo = B.s_bw[so];
     double2* R = B.resid + (size_t)B.c_roff[c] * 3;
-    const int cap = B.g_cap[g];
     const double2* ho = src_tmpl(B, c, s);
-    const double2* hn = B.hnew + B.c_hoff[c];
-    for (int j = lo + threadIdx.x; j < hi; j += blockDim.x)
+    for (int j = max(0, io) + threadIdx.x; j < min(W, io + bo); j += blockDim.x)
         for (int ch = 0; ch < 3; ch++)
         {
             double2 v = R[(size_t)ch * W + j];
-            const int mo = j - io, mn = j - in;
-            if (mo >= 0 && mo < bo)
-            {
-                v.x += ho[ch * cap + mo].x;
-                v.y += ho[ch * cap + mo].y;
-            }
-            if (mn >= 0 && mn < bn)
-            {
-                v.x -= hn[ch * cap + mn].x;
-                v.y -= hn[ch * cap + mn].y;
-            }
+            v.x += ho[ch * cap + j - io].x;
+            v.y += ho[ch * cap + j - io].y;
             R[(size_t)ch * W + j] = v;
         }
 }
 
-__global__ void accept_commit_kernel(int nacc, Items it, BatchDev B, const int* __restrict__ bw_new,
-                                     const int* __restrict__ imin_new,
-                                     const double* __restrict__ logl_new)
+__global__ __launch_bounds__(256) void accept_sub_new_kernel(int nacc, Items it, BatchDev B,
+                                                             const int* __restrict__ bw_new,
+                                                             const int* __restrict__ imin_new,
+                                                             const double* __restrict__ logl_new)
 {
     const int a = blockIdx.x;
     if (a >= nacc) return;
-    const int c = item_chain(it, a), s = item_src(it, a), so = c * kMaxSrc + s;
-    const int cap = chain_cap(B, c);
-    const double2* hn = B.hnew + B.c_hoff[c];
-    double2* ho = src_tmpl(B, c, s);
-    for (int i = threadIdx.x; i < 3 * cap; i += blockDim.x) ho[i] = hn[i];
+    const int c = item_chain(it, a), s = item_src(it, a), so = c * B.maxsrc + s;
+    const int g = B.c_group[c], W = B.g_W[g], cap = B.g_cap[g];
+    const int in = imin_new[a], bn = bw_new[a];
+    double2* R = B.resid + (size_t)B.c_roff[c] * 3;
+    const double2* hn = src_tmpl(B, c, s);
+    for (int j = max(0, in) + threadIdx.x; j < min(W, in + bn); j += blockDim.x)
+        for (int ch = 0; ch < 3; ch++)
+        {
+            double2 v = R[(size_t)ch * W + j];
+            v.x -= hn[ch * cap + j - in].x;
+            v.y -= hn[ch * cap + j - in].y;
+            R[(size_t)ch * W + j] = v;
+        }
     if (threadIdx.x < 8) B.sparams[(size_t)so * 8 + threadIdx.x] = item_params(it, a)[threadIdx.x];
     if (threadIdx.x == 0)
     {
-        B.s_bw[so] = bw_new[a];
-        B.s_imin[so] = imin_new[a];
+        B.s_bw[so] = bn;
+        B.s_imin[so] = in;
         B.logl[c] = logl_new[item_q(it, a)];
     }
 }
@@ -461,7 +457,7 @@ __global__ __launch_bounds__(256) void derivs_prep_kernel(int n, const int* __re
         k = kNDClass;
         if (c >= 0 && c < nchain && s >= 0 && s < B.c_nsrc[c])
         {
-            const int so = c * kMaxSrc + s, g = B.c_group[c];
+            const int so = c * B.maxsrc + s, g = B.c_group[c];
             D.pidx[e] = so;
             D.imin[e] = B.s_imin[so];
             D.W[e] = B.g_W[g];
@@ -507,7 +503,9 @@ struct glass_hip_fd_batch
     // init items: (chain, src) of every live source, its params gathered
     int *i_chain = nullptr, *i_src = nullptr, *i_slot = nullptr;
     double* i_params = nullptr;
-    size_t tmpl_bytes = 0;    // template slots + accept slots
+    size_t tmpl_bytes = 0;    // template slots
+    int maxsrc = kMaxSrcDefault;
+    size_t alloc_bytes = 0;   // device memory the plan holds
     DerivItems D{};
     int ncu = 0;
     int tgrid[kNClass] = {0};
@@ -523,6 +521,10 @@ struct glass_hip_fd_batch
     int prof_launches[kNClass] = {0};
     int prof_count[kNClass] = {0};
     const void* g_key[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    // sweep graph (eval + derivatives)
+    hipGraph_t sgraph = nullptr;
+    hipGraphExec_t sgexec = nullptr;
+    long long s_key[12] = {};
 };
 
 template <typename T>
@@ -531,6 +533,7 @@ static int dalloc(glass_hip_fd_batch* P, T** p, size_t n)
     void* q = nullptr;
     GH_CHECK(hipMalloc(&q, sizeof(T) * (n ? n : 1)));
     P->allocs.push_back(q);
+    P->alloc_bytes += sizeof(T) * (n ? n : 1);
     *p = (T*)q;
     return GLASS_HIP_OK;
 }
@@ -605,7 +608,7 @@ static int launch_eval(glass_hip_fd_batch* P, int k, glass_hip_tile t, int n, co
 
 // alignment + per-class launches for items; mode 0: eval, 1: templates
 static int launch_items(glass_hip_fd_batch* P, int n, const Items& it, int mode, double* logl_out, double* snr_out,
-                        int to_hnew, hipStream_t s)
+                        hipStream_t s)
 {
     if (n <= 0) return GLASS_HIP_OK;
     if (n > P->cls_cap)
@@ -632,7 +635,7 @@ static int launch_items(glass_hip_fd_batch* P, int n, const Items& it, int mode,
             if (g > n) g = n;
 #define GH_L(kk, N)                                                                               \
     case kk:                                                                                      \
-        tmpl_kernel<N, GH_NTHR(N)><<<g, GH_NTHR(N), 0, s>>>(list, cnt, it, P->B, to_hnew);       \
+        tmpl_kernel<N, GH_NTHR(N)><<<g, GH_NTHR(N), 0, s>>>(list, cnt, it, P->B);                \
         break;
             switch (k)
             {
@@ -686,7 +689,7 @@ static int rebuild(glass_hip_fd_batch* P, hipStream_t s)
     gather_params_kernel<<<(n * 8 + 255) / 256, 256, 0, s>>>(n, P->i_slot, P->B.sparams, P->i_params);
     Items it{P->i_chain, P->i_src, P->i_params, nullptr};
     int rc;
-    if ((rc = launch_items(P, n, it, 1, nullptr, nullptr, 0, s))) return rc;
+    if ((rc = launch_items(P, n, it, 1, nullptr, nullptr, s))) return rc;
     slot_align_kernel<<<(n + 255) / 256, 256, 0, s>>>(n, P->i_slot, P->p_bw, P->p_imin, P->B.s_bw, P->B.s_imin);
     resid_kernel<<<P->C, 256, 0, s>>>(P->B);
     chain_logl_kernel<<<P->C, 256, 0, s>>>(P->B);
@@ -709,15 +712,33 @@ static int check_err(glass_hip_fd_batch* P, hipStream_t s)
 
 static int drop_graph(glass_hip_fd_batch* P);
 
+extern "C" int glass_hip_fd_batch_create_ex(glass_hip_fd_batch** plan, const glass_hip_orbit* orbit,
+                                            const char* format, double T, double t0, int ngroup,
+                                            const int* group_qmin, const int* group_W, const double* data,
+                                            const double* invC, int nchain, const int* chain_group,
+                                            const int* chain_nsrc, int max_src, const double* src_params,
+                                            int max_prop);
+
 extern "C" int glass_hip_fd_batch_create(glass_hip_fd_batch** plan, const glass_hip_orbit* orbit, const char* format,
                                          double T, double t0, int ngroup, const int* group_qmin, const int* group_W,
                                          const double* data, const double* invC, int nchain, const int* chain_group,
                                          const int* chain_nsrc, const double* src_params, int max_prop)
 {
+    return glass_hip_fd_batch_create_ex(plan, orbit, format, T, t0, ngroup, group_qmin, group_W, data, invC, nchain,
+                                        chain_group, chain_nsrc, kMaxSrcDefault, src_params, max_prop);
+}
+
+extern "C" int glass_hip_fd_batch_create_ex(glass_hip_fd_batch** plan, const glass_hip_orbit* orbit,
+                                            const char* format, double T, double t0, int ngroup,
+                                            const int* group_qmin, const int* group_W, const double* data,
+                                            const double* invC, int nchain, const int* chain_group,
+                                            const int* chain_nsrc, int max_src, const double* src_params,
+                                            int max_prop)
+{
     if (!plan || !orbit || ngroup <= 0 || nchain <= 0 || max_prop < 0 || !group_qmin || !group_W || !data ||
-        !invC || !chain_group || !chain_nsrc || !src_params)
+        !invC || !chain_group || !chain_nsrc || !src_params || max_src < 1 || max_src > kMaxSrcLimit)
     {
-        set_error("fd_batch_create: bad arguments");
+        set_error("fd_batch_create: bad arguments (max_src %d: 1..%d)", max_src, kMaxSrcLimit);
         return GLASS_HIP_EINVAL;
     }
     int fmt = parse_format(format);
@@ -742,7 +763,7 @@ extern "C" int glass_hip_fd_batch_create(glass_hip_fd_batch** plan, const glass_
     for (int c = 0; c < nchain; c++)
     {
         int g = chain_group[c];
-        if (g < 0 || g >= ngroup || chain_nsrc[c] < 0 || chain_nsrc[c] > kMaxSrc)
+        if (g < 0 || g >= ngroup || chain_nsrc[c] < 0 || chain_nsrc[c] > max_src)
         {
             set_error("fd_batch_create: chain %d has group %d / nsrc %d", c, g, chain_nsrc[c]);
             return GLASS_HIP_EINVAL;
@@ -752,7 +773,7 @@ extern "C" int glass_hip_fd_batch_create(glass_hip_fd_batch** plan, const glass_
         {
             ic.push_back(c);
             is.push_back(s);
-            islot.push_back(c * kMaxSrc + s);
+            islot.push_back(c * max_src + s);
         }
     }
     ThreadCtx* cx;
@@ -781,31 +802,29 @@ extern "C" int glass_hip_fd_batch_create(glass_hip_fd_batch** plan, const glass_
         gcap[g] = cp;
         P->B.bwcap = std::max(P->B.bwcap, cp);
     }
-    std::vector<long long> toff(nchain + 1, 0), hoff(nchain + 1, 0);
+    std::vector<long long> toff(nchain + 1, 0);
     for (int c = 0; c < nchain; c++)
-    {
-        const long long cp = gcap[chain_group[c]];
-        toff[c + 1] = toff[c] + (long long)chain_nsrc[c] * 3 * cp;
-        hoff[c + 1] = hoff[c] + 3 * cp;
-    }
-    P->tmpl_bytes = sizeof(double2) * (size_t)(toff[nchain] + hoff[nchain]);
+        toff[c + 1] = toff[c] + (long long)chain_nsrc[c] * 3 * gcap[chain_group[c]];
+    P->tmpl_bytes = sizeof(double2) * (size_t)toff[nchain];
+    P->maxsrc = max_src;
+    P->B.maxsrc = max_src;
     auto fail = [&](int r) {
         glass_hip_fd_batch_destroy(P);
         return r;
     };
     int *g_qmin, *g_W, *c_group, *c_nsrc, *g_cap;
-    long long *g_off, *c_roff, *c_toff, *c_hoff;
-    double2 *ddata, *resid, *tmpl, *hnew;
+    long long *g_off, *c_roff, *c_toff;
+    double2 *ddata, *resid, *tmpl;
     double *dinvC, *logl, *sparams;
     int *s_bw, *s_imin;
-    const size_t nslot = (size_t)nchain * kMaxSrc;
+    const size_t nslot = (size_t)nchain * max_src;
     if ((rc = dalloc(P, &g_qmin, ngroup)) || (rc = dalloc(P, &g_W, ngroup)) || (rc = dalloc(P, &g_off, ngroup)) ||
         (rc = dalloc(P, &ddata, 3 * (size_t)goff[ngroup])) || (rc = dalloc(P, &dinvC, 6 * (size_t)goff[ngroup])) ||
         (rc = dalloc(P, &c_group, nchain)) || (rc = dalloc(P, &c_nsrc, nchain)) || (rc = dalloc(P, &c_roff, nchain)) ||
         (rc = dalloc(P, &resid, 3 * (size_t)roff[nchain])) || (rc = dalloc(P, &logl, nchain)) ||
         (rc = dalloc(P, &sparams, nslot * 8)) || (rc = dalloc(P, &s_bw, nslot)) || (rc = dalloc(P, &s_imin, nslot)) ||
-        (rc = dalloc(P, &tmpl, (size_t)toff[nchain])) || (rc = dalloc(P, &hnew, (size_t)hoff[nchain])) ||
-        (rc = dalloc(P, &g_cap, ngroup)) || (rc = dalloc(P, &c_toff, nchain)) || (rc = dalloc(P, &c_hoff, nchain)) ||
+        (rc = dalloc(P, &tmpl, (size_t)toff[nchain])) || (rc = dalloc(P, &g_cap, ngroup)) ||
+        (rc = dalloc(P, &c_toff, nchain)) ||
         (rc = dalloc(P, &P->p_chain, P->cls_cap)) ||
         (rc = dalloc(P, &P->p_src, P->cls_cap)) || (rc = dalloc(P, &P->p_params, (size_t)P->cls_cap * 8)) ||
         (rc = dalloc(P, &P->p_logl, P->cls_cap)) || (rc = dalloc(P, &P->p_snr, P->cls_cap)) ||
@@ -825,7 +844,7 @@ extern "C" int glass_hip_fd_batch_create(glass_hip_fd_batch** plan, const glass_
         up(dinvC, invC, 48 * (size_t)goff[ngroup]) || up(c_group, chain_group, sizeof(int) * nchain) ||
         up(c_nsrc, chain_nsrc, sizeof(int) * nchain) || up(c_roff, roff.data(), sizeof(long long) * nchain) ||
         up(sparams, src_params, sizeof(double) * nslot * 8) || up(g_cap, gcap.data(), sizeof(int) * ngroup) ||
-        up(c_toff, toff.data(), sizeof(long long) * nchain) || up(c_hoff, hoff.data(), sizeof(long long) * nchain) ||
+        up(c_toff, toff.data(), sizeof(long long) * nchain) ||
         (P->ninit && (up(P->i_chain, ic.data(), sizeof(int) * P->ninit) ||
                       up(P->i_src, is.data(), sizeof(int) * P->ninit) ||
                       up(P->i_slot, islot.data(), sizeof(int) * P->ninit))))
@@ -851,10 +870,8 @@ extern "C" int glass_hip_fd_batch_create(glass_hip_fd_batch** plan, const glass_
     B.s_bw = s_bw;
     B.s_imin = s_imin;
     B.tmpl = tmpl;
-    B.hnew = hnew;
     B.g_cap = g_cap;
     B.c_toff = c_toff;
-    B.c_hoff = c_hoff;
     B.T = T;
     B.t0 = t0;
     B.L = orbit->L;
@@ -877,6 +894,8 @@ extern "C" int glass_hip_fd_batch_destroy(glass_hip_fd_batch* P)
         for (int j = 0; j < 2; j++)
             if (P->ev[k][j]) (void)hipEventDestroy(P->ev[k][j]);
     if (P->graph) (void)hipGraphDestroy(P->graph);
+    if (P->sgexec) (void)hipGraphExecDestroy(P->sgexec);
+    if (P->sgraph) (void)hipGraphDestroy(P->sgraph);
     (void)hipDeviceSynchronize();
     for (void* p : P->allocs) (void)hipFree(p);
     delete P;
@@ -909,7 +928,7 @@ extern "C" int glass_hip_fd_batch_eval(glass_hip_fd_batch* P, int nprop, const i
     }
     GH_CHECK(hipSetDevice(P->device));
     Items it{d_chain, d_src, d_params, nullptr};
-    return launch_items(P, nprop, it, 0, d_logL, d_snr, 0, (hipStream_t)stream);
+    return launch_items(P, nprop, it, 0, d_logL, d_snr, (hipStream_t)stream);
 }
 
 extern "C" int glass_hip_fd_batch_eval_graph(glass_hip_fd_batch* P, int nprop, const int* d_chain, const int* d_src,
@@ -966,9 +985,9 @@ extern "C" int glass_hip_fd_batch_accept(glass_hip_fd_batch* P, int nacc, const 
     Items it{d_chain, d_src, d_params, d_idx};
     int rc;
     // new templates into the scratch slots 0..nacc (item a -> slot a)
-    if ((rc = launch_items(P, nacc, it, 1, nullptr, nullptr, 1, s))) return rc;
-    accept_resid_kernel<<<nacc, 256, 0, s>>>(nacc, it, P->B, P->p_bw, P->p_imin);
-    accept_commit_kernel<<<nacc, 256, 0, s>>>(nacc, it, P->B, P->p_bw, P->p_imin, d_logL);
+    accept_add_old_kernel<<<nacc, 256, 0, s>>>(nacc, it, P->B);
+    if ((rc = launch_items(P, nacc, it, 1, nullptr, nullptr, s))) return rc;
+    accept_sub_new_kernel<<<nacc, 256, 0, s>>>(nacc, it, P->B, P->p_bw, P->p_imin, d_logL);
     GH_CHECK(hipGetLastError());
     return GLASS_HIP_OK;
 }
@@ -1034,7 +1053,7 @@ extern "C" int glass_hip_fd_batch_state(glass_hip_fd_batch* P, double* logL, dou
     if ((rc = check_err(P, nullptr))) return rc;
     GH_CHECK(hipMemcpy(logL, P->B.logl, sizeof(double) * P->C, hipMemcpyDeviceToHost));
     if (src_params)
-        GH_CHECK(hipMemcpy(src_params, P->B.sparams, sizeof(double) * P->C * kMaxSrc * 8, hipMemcpyDeviceToHost));
+        GH_CHECK(hipMemcpy(src_params, P->B.sparams, sizeof(double) * P->C * P->maxsrc * 8, hipMemcpyDeviceToHost));
     return GLASS_HIP_OK;
 }
 
@@ -1044,8 +1063,12 @@ static int drop_graph(glass_hip_fd_batch* P)
 {
     if (P->gexec) GH_CHECK(hipGraphExecDestroy(P->gexec));
     if (P->graph) GH_CHECK(hipGraphDestroy(P->graph));
+    if (P->sgexec) GH_CHECK(hipGraphExecDestroy(P->sgexec));
+    if (P->sgraph) GH_CHECK(hipGraphDestroy(P->sgraph));
     P->gexec = nullptr;
     P->graph = nullptr;
+    P->sgexec = nullptr;
+    P->sgraph = nullptr;
     P->g_nprop = -1;
     return GLASS_HIP_OK;
 }
@@ -1218,5 +1241,65 @@ extern "C" int glass_hip_fd_batch_derivs(glass_hip_fd_batch* P, int n, const int
         A.count = P->D.cnt + k;
         if (int rc = launch_derivs(32 << k, A, n, s)) return rc;
     }
+    return GLASS_HIP_OK;
+}
+
+extern "C" int glass_hip_fd_batch_sweep_graph(glass_hip_fd_batch* P, int nprop, const int* d_chain, const int* d_src,
+                                              const double* d_params, double* d_logL, double* d_snr, int nder,
+                                              const int* d_dchain, const int* d_dsrc, double* d_dlogL,
+                                              double* d_grad, double* d_fisher, void* stream)
+{
+    if (!P)
+    {
+        set_error("fd_batch_sweep_graph: null plan");
+        return GLASS_HIP_EINVAL;
+    }
+    GH_CHECK(hipSetDevice(P->device));
+    const long long key[12] = {nprop, (long long)d_chain, (long long)d_src, (long long)d_params,
+                               (long long)d_logL, (long long)d_snr, nder, (long long)d_dchain,
+                               (long long)d_dsrc, (long long)d_dlogL, (long long)d_grad, (long long)d_fisher};
+    bool same = P->sgexec != nullptr;
+    for (int i = 0; i < 12 && same; i++) same = key[i] == P->s_key[i];
+    if (!same)
+    {
+        if (P->sgexec) GH_CHECK(hipGraphExecDestroy(P->sgexec));
+        if (P->sgraph) GH_CHECK(hipGraphDestroy(P->sgraph));
+        P->sgexec = nullptr;
+        P->sgraph = nullptr;
+        hipStream_t cs;
+        GH_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        GH_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        int rc = glass_hip_fd_batch_eval(P, nprop, d_chain, d_src, d_params, d_logL, d_snr, cs);
+        if (!rc) rc = glass_hip_fd_batch_derivs(P, nder, d_dchain, d_dsrc, d_dlogL, d_grad, d_fisher, cs);
+        hipGraph_t g = nullptr;
+        hipError_t ec = hipStreamEndCapture(cs, &g);
+        (void)hipStreamDestroy(cs);
+        if (rc)
+        {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (ec != hipSuccess)
+        {
+            set_error("sweep graph capture failed: %s", hipGetErrorString(ec));
+            return GLASS_HIP_ERUNTIME;
+        }
+        P->sgraph = g;
+        GH_CHECK(hipGraphInstantiate(&P->sgexec, g, nullptr, nullptr, 0));
+        for (int i = 0; i < 12; i++) P->s_key[i] = key[i];
+    }
+    GH_CHECK(hipGraphLaunch(P->sgexec, (hipStream_t)stream));
+    return GLASS_HIP_OK;
+}
+
+extern "C" int glass_hip_fd_batch_memory(glass_hip_fd_batch* P, long long* template_bytes, long long* total_bytes)
+{
+    if (!P)
+    {
+        set_error("fd_batch_memory: null plan");
+        return GLASS_HIP_EINVAL;
+    }
+    if (template_bytes) *template_bytes = (long long)P->tmpl_bytes;
+    if (total_bytes) *total_bytes = (long long)P->alloc_bytes;
     return GLASS_HIP_OK;
 }

@@ -287,3 +287,132 @@ def logl_batch_inputs(n, seed=11, T=T_2YR, windows=(512, 1024, 2048, 4096), f_lo
     for c in range(6):
         invC[idx6 + c * Wt[item]] = idiag if c < 3 else ioff
     return dict(W=W, off=off, data=data, model=model, invC=invC, qmin=qmin, T=T, seed=seed)
+
+
+class _Windows:
+    """Per-window views [3, W_g] (complex data) or [3, 3, W_g] (invC from
+    the 6 stored planes) over the flat arrays of a segmented band."""
+
+    def __init__(self, flat, off, W, kind):
+        self.flat, self.off, self.W, self.kind = flat, off, W, kind
+
+    def __len__(self):
+        return len(self.W)
+
+    def __getitem__(self, g):
+        o, w = int(self.off[g]), int(self.W[g])
+        if self.kind == "data":
+            return self.flat[3 * o:3 * (o + w)].reshape(3, w)
+        p = self.flat[6 * o:6 * (o + w)].reshape(6, w)
+        return np.stack([p[0], p[3], p[4], p[3], p[1], p[5], p[4], p[5], p[2]]).reshape(3, 3, w)
+
+
+def segment_band(T, nseg, f_lo, f_hi, wmin=64, wmax=2048):
+    """Contiguous segments [q_g, q_g + W_g) tiling the bins of [f_lo, f_hi]
+    with widths W = clip(c q, wmin, wmax) (density-weighted for a
+    log-uniform population: equal source counts per segment where the clip
+    allows), c chosen by bisection so that there are nseg of them."""
+    qlo, qhi = int(f_lo * T), int(f_hi * T)
+
+    def build(c):
+        out, q = [], qlo
+        while q < qhi:
+            w = int(min(max(c * q, wmin), wmax))
+            out.append((q, w))
+            q += w
+        return out
+
+    lo, hi = 1e-9, 1.0
+    for _ in range(80):
+        c = np.sqrt(lo * hi)
+        if len(build(c)) > nseg:
+            lo = c
+        else:
+            hi = c
+    segs = build(hi)
+    q0 = np.array([s[0] for s in segs], dtype=np.int64)
+    W = np.array([s[1] for s in segs], dtype=np.int32)
+    return q0, W
+
+
+def config4(seed=4, nsrc=20000, nseg=4000, ntemp=8, nchain=32, T=T_4YR, f_lo=1e-4, f_hi=2.5e-2,
+            log10A=(-24.0, -21.0), fisher_every=8, waveform=None, alignment=None):
+    """Global-fit UCB block: nsrc binaries with f0 log-uniform in [f_lo,
+    f_hi] and A log-uniform, the band cut into nseg density-weighted
+    segments of 64-2048 bins (segment_band), each segment a data window of
+    stationary XYZ noise (closed-form draw and inverse of C = a I + b J, as
+    logl_batch_inputs) plus its injected sources, and ntemp x nchain chains
+    per segment holding the segment's sources (jittered).  One proposal per
+    chain with sources (a random source, half a jump), and one Fisher item
+    (chain, random source) on every fisher_every-th such chain.
+
+    waveform(params, T, BW) -> complex [5, BW] and alignment(params, T, W,
+    qmin) -> (BW, qmin, imin) build the injections.  Flat layouts: data
+    complex [sum 3 W_g], invC float [sum 6 W_g] (XX YY ZZ XY XZ YZ);
+    data_w / invC_w give per-window views."""
+    rng = np.random.default_rng(seed)
+    q0, W = segment_band(T, nseg, f_lo, f_hi)
+    G = len(W)
+    off = np.zeros(G, dtype=np.int64)
+    off[1:] = np.cumsum(W[:-1].astype(np.int64))
+    nb = int(W.astype(np.int64).sum())
+    # sources
+    P = draw_params(rng, nsrc, T, log10A=log10A)
+    P[:, 0] = np.exp(rng.uniform(np.log(f_lo), np.log(f_hi), nsrc)) * T
+    qs = np.floor(P[:, 0]).astype(np.int64)
+    grp = np.clip(np.searchsorted(q0, qs, side="right") - 1, 0, G - 1)
+    order = np.argsort(grp, kind="stable")
+    P, grp = P[order], grp[order]
+    ng = np.bincount(grp, minlength=G)
+    first = np.zeros(G, dtype=np.int64)
+    first[1:] = np.cumsum(ng[:-1])
+    # noise: per bin C = a I + b J (xyz_noise_cov), closed-form draw and inverse
+    seg_of_bin = np.repeat(np.arange(G), W)
+    f = (q0[seg_of_bin] + (np.arange(nb) - off[seg_of_bin])) / T
+    Spm, Sop = noise_levels_sangria(f)
+    x = f / FSTAR
+    cx, tr = np.cos(x), np.sin(x) ** 2
+    d = 16.0 * tr * (2.0 * (1.0 + cx * cx) * Spm + Sop) / 4.0
+    b = -8.0 * tr * cx * (4.0 * Spm + Sop) / 4.0
+    a = d - b
+    u = rng.standard_normal((3, nb)) + 1j * rng.standard_normal((3, nb))
+    m = u.mean(axis=0, keepdims=True)
+    noise = np.sqrt(a / 2.0) * (u - m) + np.sqrt((a + 3.0 * b) / 2.0) * m
+    data = np.empty(3 * nb, dtype=np.complex128)
+    invC = np.empty(6 * nb, dtype=np.float64)
+    k = b / (a + 3.0 * b)
+    idiag, ioff = (1.0 - k) / a, -k / a
+    for g in range(G):
+        o, w = int(off[g]), int(W[g])
+        sl = slice(o, o + w)
+        dg = noise[:, sl].copy()
+        if waveform is not None:
+            for p in P[first[g]:first[g] + ng[g]]:
+                BW, _, imin = alignment(p, T, w, int(q0[g]))
+                scatter(dg, waveform(p, T, BW)[:3], imin)
+        data[3 * o:3 * (o + w)] = dg.reshape(-1)
+        invC[6 * o:6 * (o + w)] = np.concatenate([idiag[sl]] * 3 + [ioff[sl]] * 3)
+    # chains
+    per = ntemp * nchain
+    C = G * per
+    chain_group = np.repeat(np.arange(G), per).astype(np.int32)
+    chain_nsrc = ng[chain_group].astype(np.int32)
+    max_src = max(8, int(ng.max()))
+    src_params = np.zeros((C, max_src, 8))
+    slots = np.arange(max_src)
+    live = slots[None, :] < chain_nsrc[:, None]
+    srcidx = np.minimum(first[chain_group][:, None] + slots[None, :], nsrc - 1)
+    src_params[live] = P[srcidx[live]]
+    src_params[live] += JUMP * rng.standard_normal((int(live.sum()), 8))
+    _clip_cos(src_params)
+    # proposals: one per chain with sources
+    pc = np.nonzero(chain_nsrc > 0)[0].astype(np.int32)
+    ps = (rng.random(len(pc)) * chain_nsrc[pc]).astype(np.int32)
+    pp = _clip_cos(src_params[pc, ps] + 0.5 * JUMP * rng.standard_normal((len(pc), 8)))
+    fc = pc[::fisher_every]
+    fs = (rng.random(len(fc)) * chain_nsrc[fc]).astype(np.int32)
+    return dict(T=T, group_qmin=q0.astype(np.int32), group_W=W, group_off=off, data_flat=data, invC_flat=invC,
+                data=_Windows(data, off, W, "data"), invC=_Windows(invC, off, W, "invC"),
+                chain_group=chain_group, chain_nsrc=chain_nsrc, max_src=max_src, src_params=src_params,
+                p_chain=pc, p_src=ps, p_params=pp, f_chain=fc.astype(np.int32), f_src=fs,
+                sources=P, nsrc_per_group=ng, seed=seed)

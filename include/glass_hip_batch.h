@@ -28,7 +28,8 @@
  * Layouts (all FP64, complex = interleaved re,im):
  *   data:  per group, 3 channels x W_g complex, groups concatenated
  *   invC:  per group, 6 planes x W_g: XX, YY, ZZ, XY, XZ, YZ
- *   src_params: C x 8 x 8 (unused source slots ignored)
+ *   src_params: C x 8 x 8 (C x max_src x 8 with glass_hip_fd_batch_create_ex;
+ *               unused source slots ignored)
  * Device memory beyond the data: per chain its residual (3 W_g complex)
  * and, for each live source plus one accept slot, 3 cap_g complex with cap_g
  * the largest power of two <= W_g (the largest BW ucb_bandwidth allows in
@@ -60,7 +61,20 @@ int glass_hip_fd_batch_create(glass_hip_fd_batch **plan, const glass_hip_orbit *
                               const double *invC, int nchain, const int *chain_group,
                               const int *chain_nsrc, const double *src_params, int max_prop);
 
+/* Same with max_src source slots per chain (1..64; src_params is nchain x
+ * max_src x 8), for data windows holding more than eight binaries. */
+int glass_hip_fd_batch_create_ex(glass_hip_fd_batch **plan, const glass_hip_orbit *orbit,
+                                 const char *format, double T, double t0, int ngroup,
+                                 const int *group_qmin, const int *group_W, const double *data,
+                                 const double *invC, int nchain, const int *chain_group,
+                                 const int *chain_nsrc, int max_src, const double *src_params,
+                                 int max_prop);
+
 int glass_hip_fd_batch_destroy(glass_hip_fd_batch *plan);
+
+/* Device memory the plan holds: its template slots and in total (bytes). */
+int glass_hip_fd_batch_memory(glass_hip_fd_batch *plan, long long *template_bytes,
+                              long long *total_bytes);
 
 /* Device buffers owned by the plan, for callers that fill proposals on the
  * device: chain [max_prop] int, src [max_prop] int, params [max_prop*8],
@@ -140,7 +154,8 @@ int glass_hip_fd_batch_autotune(glass_hip_fd_batch *plan, int nprop, const int *
                                 glass_hip_tile *best, double *best_ms, void *stream);
 
 /* Copies the chains' current logL (host array of nchain) and, if not
- * NULL, their current source parameters (nchain*8*8). Synchronous. */
+ * NULL, their current source parameters (nchain*max_src*8, max_src = 8
+ * for glass_hip_fd_batch_create). Synchronous. */
 int glass_hip_fd_batch_state(glass_hip_fd_batch *plan, double *logL, double *src_params);
 
 /* Exact derivatives (no reference counterpart; replaces the forward
@@ -178,6 +193,16 @@ int glass_hip_fd_source_derivs(const glass_hip_orbit *orbit, const char *format,
  * must not overlap an eval of the same plan on another stream. */
 int glass_hip_fd_batch_derivs(glass_hip_fd_batch *plan, int n, const int *d_chain, const int *d_src,
                               double *d_logL, double *d_grad, double *d_fisher, void *stream);
+
+/* One sampler sweep as one hipGraph (built on first use for this set of
+ * counts and pointers, replayed after): glass_hip_fd_batch_eval of nprop
+ * proposals followed by glass_hip_fd_batch_derivs of nder (chain, source)
+ * items for Fisher-matrix proposals. */
+int glass_hip_fd_batch_sweep_graph(glass_hip_fd_batch *plan, int nprop, const int *d_chain,
+                                   const int *d_src, const double *d_params, double *d_logL,
+                                   double *d_snr, int nder, const int *d_dchain, const int *d_dsrc,
+                                   double *d_dlogL, double *d_grad, double *d_fisher,
+                                   void *stream);
 
 /* Batched full-window likelihood: n independent calls of
  *   double gaussian_log_likelihood(struct Data *data, struct Model *model)
