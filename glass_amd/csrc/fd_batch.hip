@@ -65,6 +65,7 @@ struct BatchDev
     const long long* c_toff;
     int bwcap;                // largest g_cap
     int maxsrc;               // source slots per chain in sparams / s_bw / s_imin
+    UcbConst* pc;             // per-item template constants, written by align_kernel
     // constants
     double T, t0, L, fstar;
     const double* pos;  // positions table, kPosN samples
@@ -126,6 +127,8 @@ __global__ __launch_bounds__(256) void align_kernel(int n, Items it, BatchDev B,
         int qmin = (int)(f0 * B.T) - BW / 2;
         bw_out[e] = BW;
         imin_out[e] = qmin - B.g_qmin[g];
+        // the template constants of the class kernels, once per item
+        ucb_setup(p, 8, B.T, B.t0, B.L, B.fstar, B.pc[e]);
         k = 31 - __clz(BW) - 5;  // log2(BW) - 5
         if (BW > B.g_cap[g] || k < 0 || k >= kNClass || (BW & (BW - 1))) k = kNClass;
         slot = atomicAdd(&hist[k], 1);
@@ -194,7 +197,7 @@ __device__ __forceinline__ void eval_item(int w, double2* lds, const int* __rest
         const int c = item_chain(it, e), s = item_src(it, e);
         const int g = B.c_group[c];
         const int W = B.g_W[g];
-        const UcbConst& k = ucb_setup_shared(item_params(it, e), 8, B.T, B.t0, B.L, B.fstar);
+        const UcbConst& k = ucb_load_shared(B.pc + e);
         double2 dl[S::PER][6];
         fd_link_spectra<N, NTHR>(k, lds, opaque(B.pos), kPosN / N, opaque(B.tw), kTwN / N, dl);
 
@@ -305,7 +308,7 @@ __global__ __launch_bounds__(NTHR) void tmpl_kernel(const int* __restrict__ list
     GH_ITEM_LOOP
     {
         const int e = list[w];
-        const UcbConst& k = ucb_setup_shared(item_params(it, e), 8, B.T, B.t0, B.L, B.fstar);
+        const UcbConst& k = ucb_load_shared(B.pc + e);
         double2 dl[S::PER][6];
         fd_link_spectra<N, NTHR>(k, lds, opaque(B.pos), kPosN / N, opaque(B.tw), kTwN / N, dl);
         const int c = item_chain(it, e), cap = chain_cap(B, c);
@@ -830,6 +833,7 @@ extern "C" int glass_hip_fd_batch_create_ex(glass_hip_fd_batch** plan, const gla
         (rc = dalloc(P, &P->p_logl, P->cls_cap)) || (rc = dalloc(P, &P->p_snr, P->cls_cap)) ||
         (rc = dalloc(P, &P->p_bw, P->cls_cap)) || (rc = dalloc(P, &P->p_imin, P->cls_cap)) ||
         (rc = dalloc(P, &P->cls, kNClass + 1)) || (rc = dalloc(P, &P->cls_list, (size_t)kNClass * P->cls_cap)) ||
+        (rc = dalloc(P, &P->B.pc, P->cls_cap)) ||
         (rc = dalloc(P, &P->i_chain, P->ninit)) || (rc = dalloc(P, &P->i_src, P->ninit)) ||
         (rc = dalloc(P, &P->i_slot, P->ninit)) || (rc = dalloc(P, &P->i_params, (size_t)P->ninit * 8)) ||
         (rc = dalloc(P, &P->D.pidx, P->cls_cap)) || (rc = dalloc(P, &P->D.imin, P->cls_cap)) ||
