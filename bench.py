@@ -494,10 +494,11 @@ def main():
     if not args.no_tune:
         t1 = time.perf_counter()
         tiles, tms = plan.autotune(E, reps=3)
-        tune = {"tiles": {str(32 << k): {"threads": t[0], "grid_mult": t[1], "ms": round(float(tms[k]), 4)}
+        tune = {"tiles": {str(32 << k): {"threads": t[0], "grid_mult": t[1], "waves": t[2],
+                                         "ms": round(float(tms[k]), 4)}
                           for k, t in enumerate(tiles) if tms[k] > 0},
                 "seconds": round(time.perf_counter() - t1, 2)}
-        log(f"autotune {tune['seconds']} s: " + ", ".join(f"BW {32 << k}: {t[0]}x{t[1]}"
+        log(f"autotune {tune['seconds']} s: " + ", ".join(f"BW {32 << k}: {t[0]}x{t[1]}w{t[2]}"
                                                           for k, t in enumerate(tiles) if tms[k] > 0))
     tiles_now = plan.tiles()
 
@@ -541,7 +542,8 @@ def main():
     ach_tf = flops / (avg_ms * 1e-3) / 1e12
     ach_gbs = byts / (avg_ms * 1e-3) / 1e9
     roofline = {"bound": "mfma", "pipe": "fp64 VALU (no MFMA: nothing on the path is a dense contraction)",
-                "kernel": f"eval_kernel<{Nd},{tiles_now[kd][0]},{'true' if tiles_now[kd][1] else 'false'}>",
+                "kernel": f"eval_kernel<{Nd},{tiles_now[kd][0]},{'true' if tiles_now[kd][1] else 'false'},"
+                          f"{tiles_now[kd][2]}>",
                 "achieved": round(ach_tf, 4),
                 "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(ach_tf / FP64_PEAK_TFLOPS, 5),
                 "traffic": MEASURED_TRAFFIC.get(Nd),

@@ -18,9 +18,10 @@ _i, _d = ctypes.c_int, ctypes.c_double
 
 
 class Tile(ctypes.Structure):
-    """glass_hip_tile: threads per workgroup and grid multiplier of one BW
-    class's eval kernel (0 = one workgroup per proposal slot)."""
-    _fields_ = [("nthr", ctypes.c_int), ("gmul", ctypes.c_int)]
+    """glass_hip_tile: threads per workgroup, grid multiplier (0 = one
+    workgroup per proposal slot) and waves per SIMD of one BW class's eval
+    kernel."""
+    _fields_ = [("nthr", ctypes.c_int), ("gmul", ctypes.c_int), ("waves", ctypes.c_int)]
 
 
 _tp = ctypes.POINTER(Tile)
@@ -79,12 +80,12 @@ NCLASS = 8   # BW classes 32 << k, k = 0..7
 
 
 def tile_candidates(k):
-    """Compiled (nthr, gmul) launch shapes of BW class k (BW = 32 << k),
-    the default first."""
-    out = (Tile * 64)()
+    """Compiled (nthr, gmul, waves) launch shapes of BW class k (BW = 32 <<
+    k), the default first."""
+    out = (Tile * 128)()
     n = ctypes.c_int()
-    check(_L().glass_hip_fd_batch_tile_candidates(int(k), out, 64, ctypes.byref(n)), "tile_candidates")
-    return [(out[i].nthr, out[i].gmul) for i in range(min(n.value, 64))]
+    check(_L().glass_hip_fd_batch_tile_candidates(int(k), out, 128, ctypes.byref(n)), "tile_candidates")
+    return [(out[i].nthr, out[i].gmul, out[i].waves) for i in range(min(n.value, 128))]
 
 
 def pack_invC(invC33):
@@ -202,13 +203,13 @@ class FdBatch:
         return ms, la, cn
 
     def tiles(self):
-        """Current (nthr, gmul) per BW class."""
+        """Current (nthr, gmul, waves) per BW class."""
         t = (Tile * NCLASS)()
         check(_L().glass_hip_fd_batch_get_tiles(self.h, t), "fd_batch_get_tiles")
-        return [(x.nthr, x.gmul) for x in t]
+        return [(x.nthr, x.gmul, x.waves) for x in t]
 
     def set_tiles(self, tiles):
-        t = (Tile * NCLASS)(*[Tile(int(a), int(b)) for a, b in tiles])
+        t = (Tile * NCLASS)(*[Tile(int(a), int(b), int(c)) for a, b, c in tiles])
         check(_L().glass_hip_fd_batch_set_tiles(self.h, t), "fd_batch_set_tiles")
 
     def autotune(self, n, reps=3):
@@ -220,7 +221,7 @@ class FdBatch:
         s = ctypes.c_void_p(self.torch.cuda.current_stream().cuda_stream)
         check(_L().glass_hip_fd_batch_autotune(self.h, int(n), _ptr(self.chain), _ptr(self.src), _ptr(self.params),
                                                int(reps), t, ms.ctypes.data_as(_dp), s), "fd_batch_autotune")
-        return [(x.nthr, x.gmul) for x in t], ms
+        return [(x.nthr, x.gmul, x.waves) for x in t], ms
 
     def derivs(self, chain, src):
         """Exact (L, grad [n, 8], Fisher [n, 8, 8]) of sources (chain[e],

@@ -268,8 +268,10 @@ __device__ __forceinline__ void eval_item(int w, double2* lds, const int* __rest
 // slot (grid = proposal count; workgroups past the class count exit at
 // once), so the body has no loop and nothing is hoisted across templates.
 // LOOP = true: any grid, each workgroup strides over the class list.
-template <int N, int NTHR, bool LOOP>
-__global__ __launch_bounds__(NTHR) void eval_kernel(const int* __restrict__ list, const int* __restrict__ count,
+// MINW: waves per SIMD the register allocation must allow (1: the
+// compiler's choice; more trades register spills for resident waves).
+template <int N, int NTHR, bool LOOP, int MINW>
+__global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(MINW, 8))) void eval_kernel(const int* __restrict__ list, const int* __restrict__ count,
                                                     Items it, BatchDev B, const int* __restrict__ p_imin,
                                                     double* __restrict__ logl_out, double* __restrict__ snr_out)
 {
@@ -287,15 +289,20 @@ __global__ __launch_bounds__(NTHR) void eval_kernel(const int* __restrict__ list
 }
 
 // Compiled launch shapes of the class kernels (tile candidates), default
-// first: (class k, template bins N, workgroup threads).  Workgroups of N
-// threads hold one slow sample per thread; fewer threads take several
-// samples and FFT butterflies each (more registers, fewer waves).  The
-// decimated classes (N >= 2048) keep their per-thread accumulators within
-// the register file only at 512 (and 1024 for N = 2048) threads.
-#define GH_EVAL_TILES(X)                                                                               \
-    X(0, 32, 64) X(1, 64, 64) X(2, 128, 128) X(2, 128, 64) X(3, 256, 256) X(3, 256, 128) X(3, 256, 64) \
-    X(4, 512, 512) X(4, 512, 256) X(4, 512, 128) X(5, 1024, 512) X(5, 1024, 1024) X(5, 1024, 256)     \
-    X(6, 2048, 512) X(6, 2048, 1024) X(7, 4096, 512)
+// first: (class k, template bins N, workgroup threads, waves per SIMD).
+// Workgroups of N threads hold one slow sample per thread; fewer threads
+// take several samples and FFT butterflies each (more registers, fewer
+// waves).  Up to 512 bins the default allocation (~98 VGPRs) leaves 4 waves
+// per SIMD while LDS would hold more workgroups: the 5- and 6-wave variants
+// spill some registers to keep more templates in flight across the FFT
+// barriers.  N = 1024 is LDS-bound (one workgroup per CU); the decimated
+// classes (N >= 2048) keep their accumulators in registers only at 512
+// (and 1024 for N = 2048) threads.
+#define GH_EVAL_TILES(X)                                                                                      \
+    X(0, 32, 64, 1) X(1, 64, 64, 1) X(1, 64, 64, 6) X(2, 128, 128, 1) X(2, 128, 128, 6) X(2, 128, 64, 1)     \
+    X(3, 256, 256, 1) X(3, 256, 256, 5) X(3, 256, 256, 6) X(3, 256, 128, 1) X(4, 512, 512, 1)                \
+    X(4, 512, 512, 6) X(4, 512, 256, 1) X(5, 1024, 512, 1) X(5, 1024, 1024, 1) X(5, 1024, 256, 1)            \
+    X(6, 2048, 512, 1) X(6, 2048, 1024, 1) X(7, 4096, 512, 1)
 
 // Template of each listed item into its source's slot.
 template <int N, int NTHR>
@@ -545,13 +552,14 @@ static int dalloc(glass_hip_fd_batch* P, T** p, size_t n)
 #define GH_NTHR(N) GH_NTHR_(N)
 
 // Workgroups of a class kernel resident per CU, as bounded by LDS (160 KB)
-// and by 32 waves per CU.
-static int resident_per_cu(int N, int nthr)
+// and by the waves per SIMD the register allocation allows (4 by default,
+// `waves` when forced higher).
+static int resident_per_cu(int N, int nthr, int waves)
 {
     const int ns = N <= 1024 ? N : GH_DECIM_NS;
     const int lds = (int)sizeof(double2) * 6 * ns + 512;
     int per = (160 * 1024) / lds;
-    const int wv = 32 / ((nthr + 63) / 64);
+    const int wv = 4 * (waves > 4 ? waves : 4) / ((nthr + 63) / 64);
     if (wv < per) per = wv;
     return per < 1 ? 1 : per;
 }
@@ -562,8 +570,8 @@ static int setup_grids(glass_hip_fd_batch* P)
     for (int k = 0; k < kNClass; k++)
     {
         const int N = 32 << k;
-        P->tgrid[k] = P->ncu * resident_per_cu(N, fd_threads(N));
-        P->tile[k] = glass_hip_tile{fd_threads(N), 0};
+        P->tgrid[k] = P->ncu * resident_per_cu(N, fd_threads(N), 1);
+        P->tile[k] = glass_hip_tile{fd_threads(N), 0, 1};
     }
     return GLASS_HIP_OK;
 }
@@ -573,10 +581,10 @@ static int setup_grids(glass_hip_fd_batch* P)
 constexpr int kGmul[] = {0, 1, 2, 4};
 constexpr int kNGmul = sizeof(kGmul) / sizeof(kGmul[0]);
 
-static bool tile_compiled(int k, int nthr)
+static bool tile_compiled(int k, int nthr, int waves)
 {
-#define GH_X(kk, N, T) \
-    if (k == kk && nthr == T) return true;
+#define GH_X(kk, N, T, MW) \
+    if (k == kk && nthr == T && waves == MW) return true;
     GH_EVAL_TILES(GH_X)
 #undef GH_X
     return false;
@@ -585,7 +593,7 @@ static bool tile_compiled(int k, int nthr)
 static int tile_grid(const glass_hip_fd_batch* P, int k, glass_hip_tile t, int n)
 {
     if (t.gmul <= 0) return n;
-    const long long g = (long long)t.gmul * P->ncu * resident_per_cu(32 << k, t.nthr);
+    const long long g = (long long)t.gmul * P->ncu * resident_per_cu(32 << k, t.nthr, t.waves);
     return (int)(g < n ? g : n);
 }
 
@@ -594,18 +602,18 @@ static int launch_eval(glass_hip_fd_batch* P, int k, glass_hip_tile t, int n, co
                        const Items& it, double* logl_out, double* snr_out, hipStream_t s)
 {
     const int g = tile_grid(P, k, t, n);
-#define GH_X(kk, N, T)                                                                                           \
-    if (k == kk && t.nthr == T)                                                                                  \
+#define GH_X(kk, N, T, MW)                                                                                       \
+    if (k == kk && t.nthr == T && t.waves == MW)                                                                 \
     {                                                                                                            \
         if (t.gmul > 0)                                                                                          \
-            eval_kernel<N, T, true><<<g, T, 0, s>>>(list, cnt, it, P->B, P->p_imin, logl_out, snr_out);          \
+            eval_kernel<N, T, true, MW><<<g, T, 0, s>>>(list, cnt, it, P->B, P->p_imin, logl_out, snr_out);      \
         else                                                                                                     \
-            eval_kernel<N, T, false><<<g, T, 0, s>>>(list, cnt, it, P->B, P->p_imin, logl_out, snr_out);         \
+            eval_kernel<N, T, false, MW><<<g, T, 0, s>>>(list, cnt, it, P->B, P->p_imin, logl_out, snr_out);     \
         return GLASS_HIP_OK;                                                                                     \
     }
     GH_EVAL_TILES(GH_X)
 #undef GH_X
-    set_error("batch: no compiled tile for class %d with %d threads", k, t.nthr);
+    set_error("batch: no compiled tile for class %d with %d threads, %d waves", k, t.nthr, t.waves);
     return GLASS_HIP_EINVAL;
 }
 
@@ -1085,10 +1093,10 @@ extern "C" int glass_hip_fd_batch_tile_candidates(int k, glass_hip_tile* out, in
         return GLASS_HIP_EINVAL;
     }
     int c = 0;
-#define GH_X(kk, N, T)                                              \
+#define GH_X(kk, N, T, MW)                                          \
     if (k == kk)                                                    \
         for (int m = 0; m < kNGmul; m++, c++)                       \
-            if (c < max) out[c] = glass_hip_tile{T, kGmul[m]};
+            if (c < max) out[c] = glass_hip_tile{T, kGmul[m], MW};
     GH_EVAL_TILES(GH_X)
 #undef GH_X
     *n = c;
@@ -1114,10 +1122,10 @@ extern "C" int glass_hip_fd_batch_set_tiles(glass_hip_fd_batch* P, const glass_h
         return GLASS_HIP_EINVAL;
     }
     for (int k = 0; k < kNClass; k++)
-        if (!tile_compiled(k, tiles[k].nthr) || tiles[k].gmul < 0 || tiles[k].gmul > 64)
+        if (!tile_compiled(k, tiles[k].nthr, tiles[k].waves) || tiles[k].gmul < 0 || tiles[k].gmul > 64)
         {
-            set_error("fd_batch_set_tiles: class %d (BW %d): no compiled tile with %d threads, grid x%d", k,
-                      32 << k, tiles[k].nthr, tiles[k].gmul);
+            set_error("fd_batch_set_tiles: class %d (BW %d): no compiled tile with %d threads, %d waves, grid x%d",
+                      k, 32 << k, tiles[k].nthr, tiles[k].waves, tiles[k].gmul);
             return GLASS_HIP_EINVAL;
         }
     GH_CHECK(hipSetDevice(P->device));
@@ -1161,13 +1169,13 @@ extern "C" int glass_hip_fd_batch_autotune(glass_hip_fd_batch* P, int nprop, con
         double bms = 0.0;
         if (cnt[k] > 0)
         {
-            glass_hip_tile cand[16];
+            glass_hip_tile cand[64];
             int nc = 0;
-            glass_hip_fd_batch_tile_candidates(k, cand, 16, &nc);
+            glass_hip_fd_batch_tile_candidates(k, cand, 64, &nc);
             const int* list = P->cls_list + (size_t)k * P->cls_cap;
             const int* cntp = P->cls + k;
             bms = 1e300;
-            for (int c = 0; c < nc && c < 16 && !rc; c++)
+            for (int c = 0; c < nc && c < 64 && !rc; c++)
             {
                 // one untimed launch, then reps timed ones; results go to
                 // the plan's own logL/snr buffers

@@ -120,15 +120,19 @@ int glass_hip_fd_batch_profile_read(glass_hip_fd_batch *plan, double *ms, int *l
 
 /* Launch shape ("tile") of one BW class's fused eval kernel (no reference
  * counterpart): nthr threads per workgroup (one template per workgroup;
- * fewer threads than template bins take several bins each), and the grid:
- * gmul 0 = one workgroup per proposal slot, gmul > 0 = gmul x (workgroups
- * resident per CU) x CUs workgroups striding over the class's proposals.
- * Class k covers BW = 32 << k, k = 0..7.  Every tile gives the same
- * results up to the order of the final sums (~1e-16 relative). */
+ * fewer threads than template bins take several bins each), the grid
+ * (gmul 0 = one workgroup per proposal slot, gmul > 0 = gmul x (workgroups
+ * resident per CU) x CUs workgroups striding over the class's proposals),
+ * and waves: the waves per SIMD the kernel's register allocation must
+ * allow (1 = the compiler's own choice; 5 or 6 spill registers to keep more
+ * workgroups resident).  Class k covers BW = 32 << k, k = 0..7.  Every
+ * tile gives the same results up to the order of the final sums (~1e-16
+ * relative). */
 typedef struct glass_hip_tile
 {
     int nthr;
     int gmul;
+    int waves;
 } glass_hip_tile;
 
 /* The compiled tiles of class k: writes up to max of them to out and their

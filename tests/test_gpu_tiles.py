@@ -90,10 +90,13 @@ def test_bad_tiles(setup, gl):
     cfg, E, plan, l0, s0 = setup
     t = plan.tiles()
     bad = list(t)
-    bad[4] = (96, 0)
+    bad[4] = (96, 0, 1)
     with pytest.raises(gl.GlassHipError):
         plan.set_tiles(bad)
-    bad[4] = (512, -1)
+    bad[4] = (512, 0, 3)
+    with pytest.raises(gl.GlassHipError):
+        plan.set_tiles(bad)
+    bad[4] = (512, -1, 1)
     with pytest.raises(gl.GlassHipError):
         plan.set_tiles(bad)
     assert plan.tiles() == t
