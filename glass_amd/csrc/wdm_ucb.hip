@@ -332,11 +332,25 @@ static int check_grid(const glass_hip_wdm* w, int Norb)
 // TDI splines (6 x 4N: amp X, phase X, amp Y, phase Y, amp Z, phase Z).
 __host__ __device__ inline size_t scratch_doubles(int N) { return 64 + 28 * (size_t)N; }
 
-constexpr int kRespThreads = 128;
+constexpr int kRespThreads = 256;
 
-// LDS of K1 in doubles: ssb 8N | M,Mf 6N | As 3N | flip,pjump 6N | phase_sc N |
-// z 6N | y2 6N
-__host__ __device__ inline size_t resp_lds_bytes(int N) { return sizeof(double) * 36 * (size_t)N + sizeof(Hdr); }
+// LDS of K1 in doubles: ssb 8N | M,Mf 6N | As 3N | flip,pjump 6N | phase_sc N.
+// The sweep scratch of the splines lives in regions dead by then: the two
+// barycentre sweeps use M,Mf (not yet written), the six TDI sweeps use ssb
+// (z) and As + flip (y2), dead after the signed amplitude/phase step.
+__host__ __device__ inline size_t resp_lds_bytes(int N) { return sizeof(double) * 24 * (size_t)N + sizeof(Hdr); }
+
+// Per-template constants, one thread per template, into each template's
+// scratch header (read by K1 and K2).
+__global__ void __launch_bounds__(256) hdr_kernel(Geo g, int n, const double* __restrict__ params,
+                                                 double* __restrict__ scratch)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    Hdr h;
+    make_hdr(params + 8 * (size_t)e, g.Tobs, g.df, g.A, &h);
+    *(Hdr*)(scratch + scratch_doubles(g.Norb) * (size_t)e) = h;
+}
 
 // One lane's tridiagonal sweep of the natural spline through y (values at
 // the knots described by prep): writes z and y2 (N each).
@@ -392,16 +406,12 @@ __global__ void __launch_bounds__(kRespThreads) resp_kernel(Geo g, int n, const 
     double* As = Mx + 6 * N;         // |M| X,Y,Z
     double* fl = As + 3 * N;         // flip X,Y,Z | pjump X,Y,Z
     double* phsc = fl + 6 * N;       // phase at spacecraft 1
-    double* zz = phsc + N;           // 6N sweep scratch
-    double* y2s = zz + 6 * N;        // 6N second-derivative coefficients
-    Hdr* H = (Hdr*)(y2s + 6 * N);
+    Hdr* H = (Hdr*)(phsc + N);
     double* scr = scratch + scratch_doubles(N) * (size_t)e;
+    (void)params;
 
-    if (tid == 0)
-    {
-        make_hdr(params + 8 * (size_t)e, g.Tobs, g.df, g.A, H);
-        *(Hdr*)scr = *H;
-    }
+    // the header hdr_kernel wrote
+    for (int i = tid; i < (int)(sizeof(Hdr) / sizeof(double)); i += kRespThreads) ((double*)H)[i] = scr[i];
     __syncthreads();
     const Hdr h = *H;
     // barycentre amplitude and phase at the orbit knots
@@ -412,7 +422,7 @@ __global__ void __launch_bounds__(kRespThreads) resp_kernel(Geo g, int n, const 
         ssb[4 * N + i] = ucb_phase(t, h.f0, h.phi0, h.fdot);
     }
     __syncthreads();
-    if (tid < 2) spline_sweep(N, g.prep_o, ssb + 4 * N * tid, zz + N * tid, ssb + 4 * N * tid + 2 * N);
+    if (tid < 2) spline_sweep(N, g.prep_o, ssb + 4 * N * tid, Mx + N * tid, ssb + 4 * N * tid + 2 * N);
     __syncthreads();
     for (int i = tid; i < 2 * N; i += kRespThreads)
     {
@@ -572,10 +582,11 @@ __global__ void __launch_bounds__(kRespThreads) resp_kernel(Geo g, int n, const 
     }
     __syncthreads();
     // six TDI splines on the response times: spline s = 2*ch + (0 amp, 1 phase)
+    double* y2s = As;  // 6N over As and the flip half of fl
     if (tid < 6)
     {
         const int ch = tid >> 1, ph = tid & 1;
-        spline_sweep(N, g.prep_s, Mx + (3 * ph + ch) * N, zz + tid * N, y2s + tid * N);
+        spline_sweep(N, g.prep_s, Mx + (3 * ph + ch) * N, ssb + tid * N, y2s + tid * N);
     }
     __syncthreads();
     for (int it = tid; it < 6 * N; it += kRespThreads)
@@ -839,6 +850,7 @@ static int launch_templates(const GeoHost& G, int n, const double* d_params, dou
 {
     const Geo& g = G.g;
     if (n <= 0) return GLASS_HIP_OK;
+    hipLaunchKernelGGL(hdr_kernel, dim3((n + 255) / 256), dim3(256), 0, s, g, n, d_params, d_scratch);
     hipLaunchKernelGGL(resp_kernel, dim3(n), dim3(kRespThreads), resp_lds_bytes(g.Norb), s, g, n, d_params,
                        d_scratch);
     GH_CHECK(hipGetLastError());
