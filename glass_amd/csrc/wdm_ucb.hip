@@ -392,7 +392,13 @@ __device__ inline void spline_finish(int N, int i, const double* __restrict__ pr
     *y3o = (y2[i + 1] - y2[i]) / (3.0 * dx);
 }
 
-__global__ void __launch_bounds__(kRespThreads) resp_kernel(Geo g, int n, const double* __restrict__ params,
+// a[i] of a 3-element register array for a runtime i in 0..2, by selects
+// (an indexed private array would live in scratch memory)
+__device__ inline double pick3(const double (&a)[3], int i) { return i == 0 ? a[0] : (i == 1 ? a[1] : a[2]); }
+
+// four waves per SIMD (128 VGPRs): four 256-thread workgroups per CU,
+// as many as the 24N-double LDS footprint allows
+__global__ void __launch_bounds__(kRespThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) resp_kernel(Geo g, int n, const double* __restrict__ params,
                                                             double* __restrict__ scratch)
 {
 #pragma clang fp contract(off)
@@ -477,14 +483,15 @@ __global__ void __launch_bounds__(kRespThreads) resp_kernel(Geo g, int n, const 
             Acp[i] = dc / (1.0 + kn[i]);
             Acm[i] = dc / (1.0 - kn[i]);
         }
-        const int vtx[3] = {ch, (ch + 1) % 3, (ch + 2) % 3};
         const double t = g.tssb[m];
         double M = 0.0, Mf = 0.0;
 #pragma unroll 1
         for (int q = 0; q < 8; q++)
         {
             const TdiTerm T = c_tdi[q];
-            const double te = t - kr[vtx[T.v]] - T.m1 * L[vtx[T.i1]] - T.m2 * L[vtx[T.i2]];
+            // vertex permutation of channel ch: vtx[i] = (ch + i) mod 3
+            const int vv = (ch + T.v) % 3, v1 = (ch + T.i1) % 3, v2 = (ch + T.i2) % 3;
+            const double te = t - pick3(kr, vv) - T.m1 * L[v1] - T.m2 * L[v2];
             const int j = even_index(te, g.ot0, g.ods, N);
             const double dx = te - g.ot[j];
             const double a = spline_at(ssb, N, j, dx);
@@ -497,9 +504,9 @@ __global__ void __launch_bounds__(kRespThreads) resp_kernel(Geo g, int n, const 
             const double hc = a * (h.Ac * h.c2p * sp - h.Ap * h.s2p * cp);
             const double hpf = a * (-h.Ap * h.c2p * sp + h.Ac * h.s2p * cp);
             const double hcf = a * (h.Ac * h.c2p * cp + h.Ap * h.s2p * sp);
-            const int j1 = vtx[T.j1], j2 = vtx[T.j2];
-            const double P1 = T.pm1 ? App[j1] : Apm[j1], Q1 = T.pm1 ? Acp[j1] : Acm[j1];
-            const double P2 = T.pm2 ? App[j2] : Apm[j2], Q2 = T.pm2 ? Acp[j2] : Acm[j2];
+            const int j1 = (ch + T.j1) % 3, j2 = (ch + T.j2) % 3;
+            const double P1 = T.pm1 ? pick3(App, j1) : pick3(Apm, j1), Q1 = T.pm1 ? pick3(Acp, j1) : pick3(Acm, j1);
+            const double P2 = T.pm2 ? pick3(App, j2) : pick3(Apm, j2), Q2 = T.pm2 ? pick3(Acp, j2) : pick3(Acm, j2);
             double v;
             v = hp * P1 + hc * Q1;
             M = T.s1 > 0 ? M + v : M - v;
