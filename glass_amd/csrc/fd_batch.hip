@@ -439,9 +439,9 @@ using namespace glass;
 
 // Exact derivatives of chain sources at their current parameters: per-item
 // arguments of the derivative kernels (fd_derivs.h) and BW-class lists
-// (BW 32..1024, classes 0..5).  Items outside those (another BW, a chain or
+// (BW 32..2048, classes 0..6).  Items outside those (another BW, a chain or
 // source index out of range) get NaN outputs.
-constexpr int kNDClass = 6;
+constexpr int kNDClass = 7;
 
 struct DerivItems
 {
@@ -476,7 +476,7 @@ __global__ __launch_bounds__(256) void derivs_prep_kernel(int n, const int* __re
             D.toff[e] = B.c_toff[c] + (long long)s * 3 * B.g_cap[g];
             D.tcap[e] = B.g_cap[g];
             const int bw = B.s_bw[so];
-            if (bw >= 32 && bw <= 1024 && (bw & (bw - 1)) == 0) k = 31 - __clz(bw) - 5;
+            if (bw >= 32 && bw <= 2048 && (bw & (bw - 1)) == 0) k = 31 - __clz(bw) - 5;
         }
         slot = atomicAdd(&hist[k], 1);
     }
@@ -516,6 +516,7 @@ struct glass_hip_fd_batch
     size_t tmpl_bytes = 0;    // template slots
     int maxsrc = kMaxSrcDefault;
     size_t alloc_bytes = 0;   // device memory the plan holds
+    double2* d2048 = nullptr; // BW-2048 derivative scratch, one slice per CU
     DerivItems D{};
     int ncu = 0;
     int tgrid[kNClass] = {0};
@@ -892,6 +893,7 @@ extern "C" int glass_hip_fd_batch_create_ex(glass_hip_fd_batch** plan, const gla
     B.tw = tw;
     B.fmt = fmt;
     if ((rc = setup_grids(P))) return fail(rc);
+    if (P->B.bwcap >= 2048 && (rc = dalloc(P, &P->d2048, (size_t)P->ncu * kDeriv2048Scratch))) return fail(rc);
     if ((rc = rebuild(P, cx->stream)) || (rc = check_err(P, cx->stream))) return fail(rc);
     *plan = P;
     return GLASS_HIP_OK;
@@ -1251,7 +1253,14 @@ extern "C" int glass_hip_fd_batch_derivs(glass_hip_fd_batch* P, int n, const int
     {
         A.list = P->D.list + (size_t)k * P->cls_cap;
         A.count = P->D.cnt + k;
-        if (int rc = launch_derivs(32 << k, A, n, s)) return rc;
+        if (32 << k == 2048)
+        {
+            // BW 2048 only occurs when some window allows it (then d2048 exists)
+            if (P->d2048)
+                if (int rc = launch_derivs_2048(A, std::min(n, P->ncu), P->d2048, s)) return rc;
+        }
+        else if (int rc = launch_derivs(32 << k, A, n, s))
+            return rc;
     }
     return GLASS_HIP_OK;
 }

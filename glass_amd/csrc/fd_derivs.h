@@ -45,4 +45,10 @@ struct DerivArgs
 // deriv_kernel<N> over grid workgroups (N a power of two, 32..1024).
 int launch_derivs(int N, const DerivArgs& A, int grid, hipStream_t s);
 
+// The BW 2048 kernel: grid workgroups striding over A.list (required),
+// each with a scratch slice of kDeriv2048Scratch complex at hs + w slice.
+constexpr int kDeriv2048Threads = 512;
+constexpr size_t kDeriv2048Scratch = 18 * 2048;
+int launch_derivs_2048(const DerivArgs& A, int grid, double2* hs, hipStream_t s);
+
 }  // namespace glass

@@ -13,6 +13,7 @@
 // thread accumulates L, the 8 gradient entries and the 36 distinct Fisher
 // entries of its bins and the workgroup reduces the 45 sums.
 #include <cmath>
+#include <type_traits>
 
 #include "../../include/glass_hip_batch.h"
 #include "fd_derivs.h"
@@ -206,6 +207,232 @@ __global__ __launch_bounds__(NTHR) void deriv_kernel(DerivArgs A)
         if (threadIdx.x < 8) A.grad[(size_t)e * 8 + threadIdx.x] = red[1 + threadIdx.x];
         if (threadIdx.x == 0) A.outL[e] = red[0];
     }
+}
+
+// The window and residual of item e.
+struct DerivItem
+{
+    const double2* R;   // residual, 3 x W complex
+    const double* iC;   // invC, 6 x W
+    const double2* Ha;  // stored template added back (or NULL)
+    int W, imin, hcap;
+};
+
+__device__ __forceinline__ DerivItem deriv_item(const DerivArgs& A, int e)
+{
+    DerivItem it;
+    it.R = A.iroff ? A.resid + A.iroff[e] : A.resid + (size_t)e * (A.rstride / 2);
+    it.iC = A.iioff ? A.invC + A.iioff[e] : A.invC + (size_t)e * A.istride;
+    it.W = A.iW ? A.iW[e] : A.W;
+    it.imin = A.imin[e];
+    it.Ha = A.tadd ? A.tadd + A.taddoff[e] : nullptr;
+    it.hcap = A.tadd ? A.taddcap[e] : 0;
+    return it;
+}
+
+// Adds template bin m's terms to the 45 sums from its six series-set TDI
+// values Hb[set][channel]: h and the eight dh/dtheta (sets 0,1 the plus /
+// cross basis, 2..5 the tangent sets), L, the gradient and the Fisher
+// upper triangle.
+__device__ __forceinline__ void deriv_bin(const UcbConst& c, const UcbDiff& dd, const DerivArgs& A, const DerivItem& it, int m,
+                                 const double2 (&Hb)[6][3], double (&acc)[kNSum])
+{
+    const int W = it.W;
+    const int j = it.imin + m;
+    const int wi = A.mode == 1 ? m : j;  // weight index
+    if (wi < 0 || wi >= W) return;
+    const double2 DP = make_double2(c.DPr, c.DPi), DC = make_double2(c.DCr, c.DCi);
+    const double2 DP4 = make_double2(dd.DPr4, dd.DPi4), DC4 = make_double2(dd.DCr4, dd.DCi4);
+    const double2 DP5 = make_double2(dd.DPr5, dd.DPi5), DC5 = make_double2(dd.DCr5, dd.DCi5);
+    double w[6];
+#pragma unroll
+    for (int q = 0; q < 6; q++) w[q] = it.iC[(size_t)q * W + wi];
+    double2 h[3], dh[8][3], r[3];
+#pragma unroll
+    for (int ch = 0; ch < 3; ch++)
+    {
+        const double2 hp = Hb[0][ch], hc = Hb[1][ch];
+        h[ch] = make_double2(cmulc(DP, hp).x + cmulc(DC, hc).x, cmulc(DP, hp).y + cmulc(DC, hc).y);
+        dh[0][ch] = Hb[2][ch];
+        dh[1][ch] = Hb[3][ch];
+        dh[2][ch] = Hb[4][ch];
+        dh[3][ch] = h[ch];                                  // d/d ln A
+        dh[4][ch] = make_double2(cmulc(DP4, hp).x + cmulc(DC4, hc).x, cmulc(DP4, hp).y + cmulc(DC4, hc).y);
+        dh[5][ch] = make_double2(cmulc(DP5, hp).x + cmulc(DC5, hc).x, cmulc(DP5, hp).y + cmulc(DC5, hc).y);
+        dh[6][ch] = make_double2(h[ch].y, -h[ch].x);        // d/d phi0 = -i h
+        dh[7][ch] = Hb[5][ch];
+        const bool inwin = j >= 0 && j < W;
+        r[ch] = (A.resid && inwin) ? it.R[(size_t)ch * W + j] : make_double2(0.0, 0.0);
+        if (it.Ha && inwin)
+        {
+            // the residual of the whole model plus this source's stored
+            // template: the residual without the source
+            r[ch].x += it.Ha[(size_t)ch * it.hcap + m].x;
+            r[ch].y += it.Ha[(size_t)ch * it.hcap + m].y;
+        }
+    }
+    // likelihood terms only where the template lands in the window
+    if (A.mode == 0 || (j >= 0 && j < W))
+    {
+        double wj[6];
+        if (A.mode == 1)
+        {
+            for (int q = 0; q < 6; q++) wj[q] = (j >= 0 && j < W) ? it.iC[(size_t)q * W + j] : 0.0;
+        }
+        const double* ww = A.mode == 1 ? wj : w;
+        acc[0] += bf3(r, h, ww) - 0.5 * bf3(h, h, ww);
+        double2 rh[3];
+#pragma unroll
+        for (int ch = 0; ch < 3; ch++) rh[ch] = make_double2(r[ch].x - h[ch].x, r[ch].y - h[ch].y);
+#pragma unroll
+        for (int a = 0; a < 8; a++) acc[1 + a] += bf3(rh, dh[a], ww);
+    }
+    int idx = 9;
+#pragma unroll
+    for (int a = 0; a < 8; a++)
+#pragma unroll
+        for (int b = a; b < 8; b++) acc[idx++] += bf3(dh[a], dh[b], w);
+}
+
+// Workgroup reduction of the 45 sums and the outputs of item e.
+template <int NTHR>
+__device__ __forceinline__ void deriv_write(const DerivArgs& A, int e, double (&acc)[kNSum], double* red)
+{
+    block_sum_vec<NTHR>(acc, red);
+    if (threadIdx.x < 64)
+    {
+        const int a = threadIdx.x / 8, b = threadIdx.x % 8;
+        const int lo = a < b ? a : b, hi = a < b ? b : a;
+        const int idx = 9 + lo * 8 - lo * (lo - 1) / 2 + (hi - lo);
+        A.fisher[(size_t)e * 64 + threadIdx.x] = red[idx];
+        if (threadIdx.x < 8) A.grad[(size_t)e * 8 + threadIdx.x] = red[1 + threadIdx.x];
+        if (threadIdx.x == 0) A.outL[e] = red[0];
+    }
+    __syncthreads();
+}
+
+// N = 2048: the six link series of one set take 196 KB, more than a CU's
+// LDS, so each set is built by decimation in time as in fd_link_spectra's
+// N >= 2048 branch (two 1024-point sub-series, six 1024-point link FFTs in
+// 96 KB of LDS, folded in registers), one set per pass; each set's TDI
+// values go to the workgroup's slice of hs (6 sets x 3 channels x N
+// complex) and are read back bin by bin for the sums.  Grid-striding over
+// the class list (A.list / A.count required).
+template <int NTHR>
+__global__ __launch_bounds__(NTHR) void deriv_kernel_2048(DerivArgs A, double2* __restrict__ hs)
+{
+    constexpr int N = 2048, NS = 1024, P = N / NS, U = NS / NTHR, PER = N / NTHR;
+    static_assert(NS % NTHR == 0, "decimated shape");
+    __shared__ double2 lds[6 * NS];
+    __shared__ UcbDiff sdd;
+    __shared__ double red[kNSum];
+    double2* H = hs + (size_t)blockIdx.x * 18 * N;
+    const int cnt = *A.count;
+    const int t = threadIdx.x;
+    const int twstride = kTwN / N;
+    const double inv2N = 1. / (double)(2 * N);
+    for (int w = blockIdx.x; w < cnt; w += gridDim.x)
+    {
+        const int e = A.list[w];
+        const double* p = A.params + (size_t)(A.pidx ? A.pidx[e] : e) * 8;
+        const UcbConst& c = ucb_setup_shared(p, 8, A.T, A.t0, A.L, A.fstar);
+        if (t == 0) ucb_diff_setup(p, A.T, sdd);
+        __syncthreads();
+        const UcbDiff& dd = sdd;
+        auto one_set = [&](auto set_tag) {
+            constexpr int S = decltype(set_tag)::value;
+            double2 X[U][P][6];
+#pragma unroll
+            for (int u = 0; u < U; u++)
+#pragma unroll
+                for (int r = 0; r < P; r++)
+#pragma unroll
+                    for (int l = 0; l < 6; l++) X[u][r][l] = make_double2(0.0, 0.0);
+#pragma unroll 1
+            for (int ph = 0; ph < P; ph++)
+            {
+                for (int j = t; j < NS; j += NTHR)
+                {
+                    const int n = P * j + ph;
+                    const double ts = ucb_sample_time(c.T, c.t0, n, N);
+                    ucb_sample_sets<S, 1>(c, dd, ts, A.pos + (size_t)n * (4096 / N) * 9, lds, NS, j);
+                }
+                __syncthreads();
+                lds_fft<NS, 6, NTHR>(lds, A.tw, twstride * P);
+#pragma unroll
+                for (int u = 0; u < U; u++)
+                {
+                    const int j = t + u * NTHR;
+                    const double2 wj = A.tw[(ph * j) * twstride];
+#pragma unroll
+                    for (int l = 0; l < 6; l++)
+                    {
+                        const double2 v = cmul(lds[l * NS + j], wj);
+#pragma unroll
+                        for (int r = 0; r < P; r++)
+                        {
+                            const double2 q = cmul(v, A.tw[((ph * r) % P) * NS * twstride]);
+                            X[u][r][l].x += q.x;
+                            X[u][r][l].y += q.y;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+            // FFT index s = j + NS r (j = t + u NTHR) is template bin
+            // m = (s + N/2) mod N = t + k NTHR
+#pragma unroll
+            for (int u = 0; u < U; u++)
+#pragma unroll
+                for (int r = 0; r < P; r++)
+                {
+                    const int k = (u + U * r + N / (2 * NTHR)) % PER;
+                    const int m = t + k * NTHR;
+                    double2 d6[6], o5[5];
+#pragma unroll
+                    for (int l = 0; l < 6; l++) d6[l] = make_double2(X[u][r][l].x * inv2N, X[u][r][l].y * inv2N);
+                    fd_tdi<N>(c, m, A.fmt, d6, o5);
+#pragma unroll
+                    for (int ch = 0; ch < 3; ch++) H[(size_t)(S * 3 + ch) * N + m] = o5[ch];
+                }
+        };
+        one_set(std::integral_constant<int, 0>{});
+        one_set(std::integral_constant<int, 1>{});
+        one_set(std::integral_constant<int, 2>{});
+        one_set(std::integral_constant<int, 3>{});
+        one_set(std::integral_constant<int, 4>{});
+        one_set(std::integral_constant<int, 5>{});
+        // each thread reads back only the bins it wrote: no barrier needed
+        DerivItem it = deriv_item(A, e);
+        double acc[kNSum];
+#pragma unroll
+        for (int q = 0; q < kNSum; q++) acc[q] = 0.0;
+#pragma unroll 1
+        for (int k = 0; k < PER; k++)
+        {
+            const int m = t + k * NTHR;
+            double2 Hb[6][3];
+#pragma unroll
+            for (int S = 0; S < 6; S++)
+#pragma unroll
+                for (int ch = 0; ch < 3; ch++) Hb[S][ch] = H[(size_t)(S * 3 + ch) * N + m];
+            deriv_bin(c, dd, A, it, m, Hb, acc);
+        }
+        deriv_write<NTHR>(A, e, acc, red);
+    }
+}
+
+int launch_derivs_2048(const DerivArgs& A, int grid, double2* hs, hipStream_t s)
+{
+    if (grid <= 0) return GLASS_HIP_OK;
+    if (!A.list || !A.count || !hs)
+    {
+        set_error("fd derivs: BW 2048 needs a class list and scratch");
+        return GLASS_HIP_EINVAL;
+    }
+    deriv_kernel_2048<kDeriv2048Threads><<<grid, kDeriv2048Threads, 0, s>>>(A, hs);
+    GH_CHECK(hipGetLastError());
+    return GLASS_HIP_OK;
 }
 
 int launch_derivs(int N, const DerivArgs& A, int grid, hipStream_t s)

@@ -192,7 +192,7 @@ int glass_hip_fd_source_derivs(const glass_hip_orbit *orbit, const char *format,
  * window's invC, as glass_hip_fd_source_derivs mode 0:
  *   d_logL[e] = (R|h) - (h|h)/2,  d_grad[8e..] = (R - h | dh/dtheta),
  *   d_fisher[64e..] = (dh/dtheta_k | dh/dtheta_l).
- * Sources with BW above 1024, or invalid indices, get NaN outputs.  Graph
+ * Sources with BW above 2048, or invalid indices, get NaN outputs.  Graph
  * safe (no allocation); shares the plan's class lists with evals, so it
  * must not overlap an eval of the same plan on another stream. */
 int glass_hip_fd_batch_derivs(glass_hip_fd_batch *plan, int n, const int *d_chain, const int *d_src,
