@@ -653,7 +653,7 @@ def main():
                 "ms_eval_only": round(gl_["ms_eval"], 3), "ms_derivs_only": round(gl_["ms_derivs"], 3),
                 "eval_rate": round(gl_["E"] / (gl_["ms_eval"] * 1e-3), 1),
                 "fisher_rate": round(gl_["nF"] / (gl_["ms_derivs"] * 1e-3), 1),
-                "fisher_items_beyond_bw_1024": gl_["nan_fisher"],
+                "fisher_items_nan": gl_["nan_fisher"],
                 "chains": gl_["chains"], "segments": gl_["segments"], "bins": gl_["bins"],
                 "max_sources_per_chain": gl_["max_src"], "device_GB": round(gl_["device_GB"], 1),
                 "template_GB": round(gl_["template_GB"], 1), "tiles": gl_["tiles"],
