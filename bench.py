@@ -36,8 +36,8 @@ FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per launch of the batched per-class kernels, measured with rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE on this workload and scaled by the same counters on a 256 MiB device copy (fetch x2.00,
 # write x1.00): profiles/eval_kernel_batched_fd_sweep.md.  Keyed by template bins N.
-MEASURED_TRAFFIC = {512: 2.3187e9, 1024: 1.8220e9, 2048: 2.3717e9, 256: 8.1357e8, 4096: 7.1843e8,
-                    128: 2.2562e8, 64: 6.550e7}
+MEASURED_TRAFFIC = {512: 2.3552e9, 1024: 1.8288e9, 2048: 2.3939e9, 256: 8.3085e8, 4096: 7.2205e8,
+                    128: 2.3213e8, 64: 7.1404e7, 32: 1.8041e6}
 
 # HBM bytes per launch of logl_chunk_kernel on the inner-product leg (rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE, same calibration): profiles/logl_chunk_kernel_batched_inner_product.md.

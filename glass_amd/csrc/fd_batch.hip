@@ -637,7 +637,13 @@ static int launch_items(glass_hip_fd_batch* P, int n, const Items& it, int mode,
         const int* cnt = P->cls + k;
         const bool pr = P->prof && mode == 0;
         if (pr) GH_CHECK(hipEventRecord(P->ev[k][0], s));
-        if (mode == 0)
+        if ((32 << k) > P->B.bwcap)
+        {
+            // align_kernel routes BW above every window's cap to the error
+            // count, so this class is empty; its one-workgroup-per-item grid
+            // would only launch empty workgroups
+        }
+        else if (mode == 0)
         {
             if (int rc = launch_eval(P, k, P->tile[k], n, list, cnt, it, logl_out, snr_out, s)) return rc;
         }
