@@ -305,6 +305,114 @@ def inner_leg(gl, synth, n=4096, steps=20, warmup=3, seed=11, parity_n=64):
     return dict(n=n, bins=bins, ms=ms, bytes=144.0 * bins, max_rel=worst, checked=min(parity_n, n))
 
 
+def noise_data(orc_or_none, fstar, N, T, fmin, seed):
+    """Synthetic data for the noise leg: the instrument model at the
+    reference's default link levels plus a smooth foreground-like covariance
+    (kept fixed, as the foreground model is while the instrument model is
+    sampled), drawn per bin from the 3x3 covariance."""
+    from glass_amd import noise
+    rng = np.random.default_rng(seed)
+    f = fmin + np.arange(N) / T
+    sa, so = noise.default_links()
+    C = noise.instrument_noise_model(fstar, f, sa, so)
+    g = 0.5 * C[0, 0] * (1e-3 / f) ** (7.0 / 3.0) * np.exp(-f / 2e-3)
+    Cadd = np.zeros_like(C)
+    for a in range(3):
+        for b in range(3):
+            Cadd[a, b] = g * (1.0 if a == b else -0.5)
+    L = np.linalg.cholesky(np.moveaxis(C + Cadd, -1, 0))
+    w = (rng.standard_normal((N, 3)) + 1j * rng.standard_normal((N, 3))) / np.sqrt(2)
+    d = np.einsum("nab,nb->na", L, w)
+    return f, d[:, 0].copy(), d[:, 1].copy(), d[:, 2].copy(), np.concatenate([sa, so]), Cadd
+
+
+def noise_leg(gl, steps=3, seed=31, pcalls=2, ladders=64):
+    """Instrument-noise sampler on the device (glass_hip_noise_*): the band
+    of the reference's noise_mcmc example (--duration 7864320 --fmin 1e-4
+    --fmax 8e-3: 62,128 bins) with a fixed foreground covariance.  One sweep
+    is the reference's outer iteration: per chain 10 calls of
+    noise_instrument_model_mcmc (10 moves, one likelihood of every bin
+    each), then noise_ptmcmc -- here one hipGraph.  Runs the reference's 12
+    chains, and `ladders` ladders of 12 for throughput; times the fused
+    likelihood kernel alone; checks sampled likelihoods and one sweep of 12
+    chains (pcalls calls each) against the oracle, which is also the
+    single-core CPU baseline."""
+    import torch
+    from glass_amd import noise
+    from oracle import orc
+    T, fmin, fmax, ntemp = 7864320.0, 1e-4, 8e-3, 12
+    N = int(round((fmax - fmin) * T))
+    fstar = gl.analytic_orbit().fstar
+    f, X, Y, Z, p0, Cadd = noise_data(None, fstar, N, T, fmin, seed)
+    rng = np.random.default_rng(seed + 1)
+    temps1 = 1.2 ** np.arange(ntemp)
+    res = dict(N=N, ntemp=ntemp)
+    od = orc.NoiseData(fstar, f, X, Y, Z, Cadd)
+    for nl in (1, ladders):
+        S = nl * ntemp
+        plan = noise.NoisePlan(fstar, f, X, Y, Z, nladder=nl, ntemp=ntemp, Cadd=Cadd)
+        P = np.stack([p0 * rng.uniform(0.9, 1.1, 12) for _ in range(S)])
+        P[:, 6:] = np.repeat(P[:, 6::2], 2, axis=1)
+        seeds = rng.integers(1, 2 ** 32, S, dtype=np.uint64).astype(np.uint32)
+        temps = np.tile(temps1, nl)
+        plan.set_state(P, seeds, temps)
+        if nl == 1:
+            # parity: one sweep of the reference's loop against the oracle
+            st0 = plan.state()
+            plan.mcmc(pcalls, swap=True, graph=False)
+            st = plan.state()
+            x, lg = st0["params"].copy(), st0["logL"].copy()
+            sd, idx = st0["seeds"].astype(np.int64), st0["index"].copy()
+            nacc = np.zeros(S, dtype=np.int64)
+            t0 = time.perf_counter()
+            for s_ in range(S):
+                for _ in range(pcalls):
+                    x[idx[s_]], lg[idx[s_]], sd[s_], na = od.mcmc_call(x[idx[s_]], lg[idx[s_]], sd[s_], temps[s_])
+                    nacc[s_] += na
+            cpu_s = time.perf_counter() - t0
+            i2, s2, _ = orc.noise_ptmcmc(idx, temps, lg, sd)
+            same = (np.array_equal(st["nacc"], nacc) and np.array_equal(st["index"], i2) and
+                    np.array_equal(st["seeds"], s2.astype(np.uint32)))
+            prel = float(np.max(np.abs(st["params"] - x) / np.abs(x)))
+            lrel = float(np.max(np.abs(st["logL"] - lg) / np.abs(lg)))
+            res["parity"] = {"chains": S, "moves_per_chain": 10 * pcalls, "same_decisions": bool(same),
+                             "max_param_rel_err": prel, "max_logL_rel_err": lrel,
+                             "gate": "same accept/swap decisions; params 1e-12, logL 1e-10 relative",
+                             "pass": bool(same and prel <= 1e-12 and lrel <= 1e-10)}
+            res["cpu"] = {"rate": S * 10 * pcalls / cpu_s, "secs": cpu_s, "evals": S * 10 * pcalls}
+        plan.mcmc(10, swap=True, graph=True)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            plan.mcmc(10, swap=True, graph=True)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        # the fused likelihood kernel alone, all slots at once
+        dP = torch.tensor(P, device="cuda")
+        out = plan.logl(dP)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(20):
+            plan.logl(dP, out=out)
+        e1.record()
+        torch.cuda.synchronize()
+        kms = e0.elapsed_time(e1) / 20
+        st = plan.state()
+        moves = 100 * (steps + 1) + (10 * pcalls if nl == 1 else 0)
+        res[nl] = dict(slots=S, ms_sweep=ms, logl_ms=kms, info=plan.info(),
+                       acc_rate=float(st["nacc"].sum() / (S * moves)))
+        if nl == ladders:
+            smp = rng.choice(S, 8, replace=False)
+            got = out.cpu().numpy()[smp]
+            ref = np.array([od.logl(P[i]) for i in smp])
+            res["logl_parity"] = float(np.max(np.abs(got - ref) / np.abs(ref)))
+        plan.close()
+        del plan
+    return res
+
+
 def globalfit_leg(gl, synth, steps=3, seed=4, cpu_seconds=2.0, nthreads=0):
     """Global-fit UCB block at full scale on one GPU: 20,000 binaries (f0
     log-uniform 0.1-25 mHz, A log-uniform 1e-24..1e-21), Tobs 4 yr, the
@@ -463,6 +571,7 @@ def main():
     ap.add_argument("--no-fisher", action="store_true")
     ap.add_argument("--no-wdm", action="store_true")
     ap.add_argument("--no-inner", action="store_true")
+    ap.add_argument("--no-noise", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default tiles (skip the autotuner)")
     ap.add_argument("--globalfit", action="store_true",
                     help="also run the global-fit block at full scale (1,024,000 chains, ~170 GB of HBM)")
@@ -628,6 +737,36 @@ def main():
         log(f"inner: {il['ms']:.4f} ms per launch of {il['n']} items / {il['bins']} bins, {gbs:.1f} GB/s, "
             f"parity {il['max_rel']:.3e}")
 
+    nzl = None
+    if not args.no_noise:
+        nl = noise_leg(gl)
+        big = nl[max(k for k in nl if isinstance(k, int))]
+        one = nl[1]
+        N = nl["N"]
+        nzl = {"value": round(big["slots"] * 100 / (big["ms_sweep"] * 1e-3), 1),
+               "unit": "noise-model likelihoods/s (each over all bins, inside the MCMC sweep)",
+               "bins": N, "chains": big["slots"], "ms_per_sweep": round(big["ms_sweep"], 3),
+               "reference_config": {"chains": one["slots"], "ms_per_sweep": round(one["ms_sweep"], 3),
+                                    "likelihoods_per_s": round(one["slots"] * 100 / (one["ms_sweep"] * 1e-3), 1)},
+               "fused_logl_kernel": {"ms_per_launch": round(big["logl_ms"], 4), "chains": big["slots"],
+                                     "chain_bins_per_s": round(big["slots"] * N / (big["logl_ms"] * 1e-3), 1),
+                                     "hbm_bytes": 128.0 * N * ((big["slots"] + 7) // 8),
+                                     "plan": big["info"]},
+               "acceptance": round(big["acc_rate"], 4),
+               "parity": dict(nl["parity"], logl_sample_max_rel_err=nl.get("logl_parity")),
+               "config": "instrument-noise MCMC, band 0.1-8 mHz at Tobs 7,864,320 s (62,128 bins) with a "
+                         "fixed foreground covariance; sweep = per chain 10 x noise_instrument_model_mcmc "
+                         "(10 moves each) + noise_ptmcmc, one hipGraph; 12 temperatures per ladder"}
+        if "cpu" in nl:
+            nzl["cpu_baseline"] = {"value": round(nl["cpu"]["rate"], 2), "unit": "likelihoods/s", "cores": 1,
+                                   "kind": "port",
+                                   "sample": f"the 12 reference chains, {nl['cpu']['evals']} moves, by "
+                                             f"the oracle restatement, single-threaded, "
+                                             f"{nl['cpu']['secs']:.1f} s"}
+        log(f"noise: {nzl['value']:.4e} likelihoods/s at {big['slots']} chains ({big['ms_sweep']:.3f} ms/sweep); "
+            f"12 chains {one['ms_sweep']:.3f} ms/sweep; fused kernel {big['logl_ms']:.4f} ms; "
+            f"parity {nzl['parity']}")
+
     cpu = None
     parity = None
     if not args.no_cpu_baseline:
@@ -689,6 +828,7 @@ def main():
         "wdm": wdm,
         "inner_product": inner,
         "globalfit": gfit,
+        "noise": nzl,
         "parity": parity,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -24,6 +24,18 @@
  *                           of the same one-arm response (tests/)
  *   orc_ucb_bandwidth    -> closed form by hand at chosen inputs
  *   orc_invert_noise_covariance -> C * invC = I
+ *   orc_instrument_noise_model -> equal-link parameters reproduce the
+ *                           reference's closed forms XYZnoise_FF/XYZcross_FF
+ *                           with get_noise_levels("sangria"); the cyclic
+ *                           relabelling of the spacecraft permutes X,Y,Z;
+ *                           linear in every link parameter
+ *   orc_my_noise_log_likelihood, orc_noise_log_likelihood,
+ *   orc_my_noise_log_likelihood_wavelet -> numpy per-bin inverse and
+ *                           log-determinant on small inputs
+ *   orc_rand_*           -> the C library's rand_r itself
+ *   orc_noise_mcmc_call, orc_noise_ptmcmc -> detailed balance of the swap
+ *                           (exact swap probabilities), and the cold chain of
+ *                           a long run settling on the injected noise levels
  *   orc_ucb_fisher_fd    -> symmetric, and equal to the Gram matrix of the
  *                           finite-difference waveform derivatives
  *   libm (glibc sin/cos/exp/log/pow): unpinned, ulp level only.
@@ -272,6 +284,47 @@ double orc_eval_logl(const orc_orbit *orbit, double T, double t0, const orc_chai
 double orc_eval_batch(const orc_orbit *orbit, double T, double t0, const orc_chain *chains,
                       int n, const int *chain_of, const int *sid_of, const double *params,
                       double *out, int nthreads);
+
+/* ------------------------------------------------------- noise model ---- */
+/* generate_instrument_noise_model, noise/src/glass_noise_model.c:389-545.
+ * f[N] bin frequencies, nch 1/2/3 channels, sacc[6]/soms[6] in the link
+ * order 12,21,23,32,13,31 (:24-44); C[nch][nch][N] (entries the reference
+ * leaves alone -- f = 0 bins, C[1][0] of two channels -- are not written
+ * apart from the final /4 of the latter). */
+void orc_instrument_noise_model(double fstar, const double *f, int N, int nch, const double *sacc,
+                                const double *soms, double *C);
+/* generate_instrument_noise_model_wavelet, :561-610; C planes of stride N;
+ * returns the number of active layers written, -1 if it exceeds N */
+int orc_instrument_noise_model_wavelet(double fstar, const double *f, int N, double df, int ND,
+                                       const double *sacc, const double *soms, double *C);
+/* my_noise_log_likelihood, :1275-1306 (X,Y,Z interleaved complex [2N]) */
+double orc_my_noise_log_likelihood(const double *X, const double *Y, const double *Z, const double *invC,
+                                   const double *logdetC, int N);
+/* noise_log_likelihood, :1104-1134, three channels */
+double orc_noise_log_likelihood(const double *X, const double *Y, const double *Z, const double *invC,
+                                const double *logdetC, int N);
+/* my_noise_log_likelihood_wavelet, :1182-1206 (pixel k = i + j NT, planes
+ * of stride ldc, Ntot = data->N) */
+double orc_my_noise_log_likelihood_wavelet(const double *X, const double *Y, const double *Z,
+                                           const double *invC, const double *logdetC, int NT, int Nlayer,
+                                           int Ntot, int ldc);
+/* rand_r_U_0_1 / rand_r_N_0_1, utils/src/glass_utils.c:19-35 (libc rand_r) */
+double orc_rand_U_0_1(unsigned int *seed);
+double orc_rand_N_0_1(unsigned int *seed);
+/* instrument model of p12 = sacc[6],soms[6] (+ Cadd[9][N] if not NULL),
+ * inverse, my_noise_log_likelihood; work holds 19 N doubles */
+double orc_noise_logl_params(double fstar, const double *f, int N, const double *X, const double *Y,
+                             const double *Z, const double *Cadd, const double *p12, double *work);
+/* one call of noise_instrument_model_mcmc (noise/src/glass_noise_sampler.c:
+ * 395-592) for one chain: x[12] and *logLx updated in place; returns the
+ * number of accepted moves of its 10 */
+int orc_noise_mcmc_call(double fstar, const double *f, int N, const double *X, const double *Y,
+                        const double *Z, const double *Cadd, double temperature, double *x, double *logLx,
+                        unsigned int *seed, double *work);
+/* noise_ptmcmc, noise/src/glass_noise_sampler.c:70-117: one ladder of NC
+ * slots; index[] slot -> state, logL[] per state, seeds[] per slot */
+void orc_noise_ptmcmc(int NC, int *index, const double *temperature, const double *logL, unsigned int *seeds,
+                      int *acceptance);
 
 #ifdef __cplusplus
 }

@@ -93,6 +93,7 @@ def lib():
                                      ctypes.c_int]
         L.orc_eval_batch.restype = ctypes.c_double
         _wdm_signatures(L)
+        _noise_signatures(L)
         _lib = L
     return _lib
 
@@ -532,3 +533,123 @@ def wdm_eval_batch(w, orb, Tobs, params, data, invC9, t0=0.0, nthreads=0):
     secs = lib().orc_wdm_eval_batch(w.ref, len(t), _d(t), _d(coef), float(Tobs), float(t0), n, _d(p),
                                     _d(d), _d(ic), d.shape[1], _d(dl), _d(sn), _i(nw), nthreads)
     return dl, sn, nw, secs
+
+
+# ---------------------------------------------------------- noise model ---
+
+def _noise_signatures(L):
+    _up = ctypes.POINTER(ctypes.c_uint)
+    d, i = ctypes.c_double, ctypes.c_int
+    L.orc_instrument_noise_model.argtypes = [d, _dp, i, i, _dp, _dp, _dp]
+    L.orc_instrument_noise_model_wavelet.argtypes = [d, _dp, i, d, i, _dp, _dp, _dp]
+    L.orc_my_noise_log_likelihood.argtypes = [_dp] * 5 + [i]
+    L.orc_my_noise_log_likelihood.restype = d
+    L.orc_noise_log_likelihood.argtypes = [_dp] * 5 + [i]
+    L.orc_noise_log_likelihood.restype = d
+    L.orc_my_noise_log_likelihood_wavelet.argtypes = [_dp] * 5 + [i] * 4
+    L.orc_my_noise_log_likelihood_wavelet.restype = d
+    L.orc_rand_U_0_1.argtypes = [_up]
+    L.orc_rand_U_0_1.restype = d
+    L.orc_rand_N_0_1.argtypes = [_up]
+    L.orc_rand_N_0_1.restype = d
+    L.orc_noise_logl_params.argtypes = [d, _dp, i, _dp, _dp, _dp, _dp, _dp, _dp]
+    L.orc_noise_logl_params.restype = d
+    L.orc_noise_mcmc_call.argtypes = [d, _dp, i, _dp, _dp, _dp, _dp, d, _dp, _dp, _up, _dp]
+    L.orc_noise_mcmc_call.restype = i
+    L.orc_noise_ptmcmc.argtypes = [i, _ip, _dp, _dp, _up, _ip]
+
+
+def instrument_noise_model(fstar, f, sacc, soms, nch=3, C=None):
+    """generate_instrument_noise_model -> C[nch, nch, N] (in place if C given)."""
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    if C is None:
+        C = np.zeros((nch, nch, f.size))
+    sa = np.ascontiguousarray(sacc, dtype=np.float64)
+    so = np.ascontiguousarray(soms, dtype=np.float64)
+    lib().orc_instrument_noise_model(fstar, _d(f), f.size, nch, _d(sa), _d(so), _d(C))
+    return C
+
+
+def instrument_noise_model_wavelet(fstar, f, df, ND, sacc, soms, C=None):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    if C is None:
+        C = np.zeros((3, 3, f.size))
+    sa = np.ascontiguousarray(sacc, dtype=np.float64)
+    so = np.ascontiguousarray(soms, dtype=np.float64)
+    na = lib().orc_instrument_noise_model_wavelet(fstar, _d(f), f.size, df, ND, _d(sa), _d(so), _d(C))
+    return C, na
+
+
+def _cplx(a):
+    return np.ascontiguousarray(a, dtype=np.complex128).view(np.float64)
+
+
+def my_noise_log_likelihood(X, Y, Z, invC, logdetC):
+    invC = np.ascontiguousarray(invC, dtype=np.float64)
+    ld = np.ascontiguousarray(logdetC, dtype=np.float64)
+    return lib().orc_my_noise_log_likelihood(_d(_cplx(X)), _d(_cplx(Y)), _d(_cplx(Z)), _d(invC), _d(ld), ld.size)
+
+
+def noise_log_likelihood(X, Y, Z, invC, logdetC):
+    invC = np.ascontiguousarray(invC, dtype=np.float64)
+    ld = np.ascontiguousarray(logdetC, dtype=np.float64)
+    return lib().orc_noise_log_likelihood(_d(_cplx(X)), _d(_cplx(Y)), _d(_cplx(Z)), _d(invC), _d(ld), ld.size)
+
+
+def my_noise_log_likelihood_wavelet(X, Y, Z, invC, logdetC, NT, Nlayer, Ntot):
+    X, Y, Z = (np.ascontiguousarray(v, dtype=np.float64) for v in (X, Y, Z))
+    invC = np.ascontiguousarray(invC, dtype=np.float64)
+    ld = np.ascontiguousarray(logdetC, dtype=np.float64)
+    return lib().orc_my_noise_log_likelihood_wavelet(_d(X), _d(Y), _d(Z), _d(invC), _d(ld), NT, Nlayer, Ntot,
+                                                     ld.size)
+
+
+def rand_stream(seed, n):
+    """n rand_r_U_0_1 draws then n rand_r_N_0_1 draws from one rand_r seed."""
+    s = ctypes.c_uint(seed & 0xFFFFFFFF)
+    u = np.array([lib().orc_rand_U_0_1(ctypes.byref(s)) for _ in range(n)])
+    g = np.array([lib().orc_rand_N_0_1(ctypes.byref(s)) for _ in range(n)])
+    return u, g
+
+
+class NoiseData:
+    """One data segment for the noise oracle: f [N], X, Y, Z complex [N],
+    Cadd [3, 3, N] or None."""
+
+    def __init__(self, fstar, f, X, Y, Z, Cadd=None):
+        self.fstar = float(fstar)
+        self.f = np.ascontiguousarray(f, dtype=np.float64)
+        self.N = self.f.size
+        self.X, self.Y, self.Z = _cplx(X), _cplx(Y), _cplx(Z)
+        self.Cadd = None if Cadd is None else np.ascontiguousarray(Cadd, dtype=np.float64)
+        self.work = np.zeros(19 * self.N)
+
+    def _ca(self):
+        return None if self.Cadd is None else _d(self.Cadd)
+
+    def logl(self, p12):
+        p = np.ascontiguousarray(p12, dtype=np.float64)
+        return lib().orc_noise_logl_params(self.fstar, _d(self.f), self.N, _d(self.X), _d(self.Y), _d(self.Z),
+                                           self._ca(), _d(p), _d(self.work))
+
+    def mcmc_call(self, x, logLx, seed, temperature):
+        """One noise_instrument_model_mcmc call; returns (x, logLx, seed, nacc)."""
+        xx = np.array(x, dtype=np.float64)
+        lg = np.array([logLx], dtype=np.float64)
+        s = ctypes.c_uint(int(seed) & 0xFFFFFFFF)
+        na = lib().orc_noise_mcmc_call(self.fstar, _d(self.f), self.N, _d(self.X), _d(self.Y), _d(self.Z),
+                                       self._ca(), float(temperature), _d(xx), _d(lg), ctypes.byref(s),
+                                       _d(self.work))
+        return xx, float(lg[0]), s.value, na
+
+
+def noise_ptmcmc(index, temperature, logL, seeds):
+    """noise_ptmcmc over one ladder; returns (index, seeds, acceptance)."""
+    idx = np.ascontiguousarray(index, dtype=np.int32).copy()
+    t = np.ascontiguousarray(temperature, dtype=np.float64)
+    lg = np.ascontiguousarray(logL, dtype=np.float64)
+    sd = np.ascontiguousarray(seeds, dtype=np.uint32).copy()
+    acc = np.zeros(idx.size, dtype=np.int32)
+    lib().orc_noise_ptmcmc(idx.size, _i(idx), _d(t), _d(lg), sd.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)),
+                           _i(acc))
+    return idx, sd, acc

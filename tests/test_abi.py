@@ -30,11 +30,10 @@ def test_library_exports_declared_symbols():
 
 
 def test_headers_compile_as_c(tmp_path):
-    """The public header is plain C (no torch or HIP types)."""
+    """Every public header is plain C (no torch or HIP types)."""
     import subprocess
     src = tmp_path / "t.c"
-    src.write_text('#include "glass_hip.h"\n#include "glass_hip_batch.h"\nint main(void){return 0;}\n'
-                   if os.path.exists(os.path.join(ROOT, "include", "glass_hip_batch.h")) else
-                   '#include "glass_hip.h"\nint main(void){return 0;}\n')
+    heads = sorted(os.path.basename(h) for h in glob.glob(os.path.join(ROOT, "include", "*.h")))
+    src.write_text("".join(f'#include "{h}"\n' for h in heads) + "int main(void){return 0;}\n")
     subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
                     "-c", str(src), "-o", str(tmp_path / "t.o")], check=True)
