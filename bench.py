@@ -413,6 +413,60 @@ def noise_leg(gl, steps=3, seed=31, pcalls=2, ladders=64):
     return res
 
 
+def sampler_leg(plan, cfg, nladder, ntemp, nsweep=3, nsteps=80, seed=41):
+    """The UCB sampler on the device over the batched plan's chains
+    (glass_hip_ucb_sampler_*): one iteration of the reference's outer loop
+    (apps/src/ucb_mcmc.c:230-265) = per chain 10 * NP = 80 ucb_mcmc steps
+    (one proposal, waveform, likelihood and accept each), the Fisher update
+    of every live source (exact derivatives + Jacobi eigen-decomposition),
+    ptmcmc and adapt_temperature_ladder -- one hipGraph.  Default proposal
+    weights without the F-statistic / GMM / covariance proposals, window
+    priors, snr cut 5."""
+    import torch
+    from glass_amd.sampler import UcbSampler
+    G = len(cfg["group_W"])
+    pr = np.zeros((G, 8, 2))
+    for g in range(G):
+        q, W = float(cfg["group_qmin"][g]), float(cfg["group_W"][g])
+        pr[g] = [[q, q + W], [-1, 1], [0, 2 * np.pi], [-60, -40], [-1, 1], [0, np.pi], [0, 2 * np.pi], [-1, 1]]
+    smp = UcbSampler(plan, nladder, ntemp, pr)
+    rng = np.random.default_rng(seed)
+    C = nladder * ntemp
+    smp.set_state(rng.integers(1, 2 ** 32, C, dtype=np.uint64).astype(np.uint32), np.tile(1.2 ** np.arange(ntemp),
+                                                                                           nladder))
+    smp.fisher()
+    t0 = time.perf_counter()
+    smp.sweep(nsteps, adapt=True, graph=True)  # capture + first replay
+    torch.cuda.synchronize()
+    t_first = time.perf_counter() - t0
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(nsweep):
+        smp.sweep(nsteps, adapt=True, graph=True)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / nsweep
+    # the parts: steps alone, Fisher update alone
+    e0.record()
+    smp.steps(nsteps)
+    e1.record()
+    torch.cuda.synchronize()
+    ms_steps = e0.elapsed_time(e1)
+    e0.record()
+    smp.fisher()
+    e1.record()
+    torch.cuda.synchronize()
+    ms_fisher = e0.elapsed_time(e1)
+    st = smp.state()
+    smp.close()
+    tr, ac = st["trials"].sum(axis=0), st["accepts"].sum(axis=0)
+    return dict(chains=C, nsteps=nsteps, ms=ms, ms_steps_eager=ms_steps, ms_fisher=ms_fisher, first_s=t_first,
+                live=int(np.asarray(cfg["chain_nsrc"]).sum()),
+                acceptance={k: round(float(ac[i] / max(tr[i], 1)), 4) for i, k in
+                            ((0, "uniform_prior"), (4, "fisher"), (5, "fm_shift"), (6, "psi_phi"))},
+                swap_rate=round(float(st["swaps"].sum() / max(1, (nsweep + 2) * nladder * (ntemp - 1))), 4))
+
+
 def globalfit_leg(gl, synth, steps=3, seed=4, cpu_seconds=2.0, nthreads=0):
     """Global-fit UCB block at full scale on one GPU: 20,000 binaries (f0
     log-uniform 0.1-25 mHz, A log-uniform 1e-24..1e-21), Tobs 4 yr, the
@@ -572,6 +626,7 @@ def main():
     ap.add_argument("--no-wdm", action="store_true")
     ap.add_argument("--no-inner", action="store_true")
     ap.add_argument("--no-noise", action="store_true")
+    ap.add_argument("--no-sampler", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default tiles (skip the autotuner)")
     ap.add_argument("--globalfit", action="store_true",
                     help="also run the global-fit block at full scale (1,024,000 chains, ~170 GB of HBM)")
@@ -781,6 +836,22 @@ def main():
                   "pass": bool(leg["max_rel"] <= 1e-10)}
         log(f"cpu baseline {leg['rate']:.1f} evals/s on {nthr} threads; parity max rel {leg['max_rel']:.3e}")
 
+    smpl = None
+    if not args.no_sampler:
+        sl = sampler_leg(plan, cfg, args.groups, args.temps)
+        smpl = {"value": round(sl["chains"] * sl["nsteps"] / (sl["ms"] * 1e-3), 1),
+                "unit": "ucb_mcmc steps/s (proposal + waveform + likelihood + accept per chain step)",
+                "ms_per_iteration": round(sl["ms"], 3), "chains": sl["chains"], "steps_per_iteration": sl["nsteps"],
+                "ms_steps_eager": round(sl["ms_steps_eager"], 3), "ms_fisher_update": round(sl["ms_fisher"], 3),
+                "live_sources": sl["live"], "graph_capture_and_first_run_s": round(sl["first_s"], 2),
+                "acceptance": sl["acceptance"], "pt_swap_rate": sl["swap_rate"],
+                "config": "the batched case's %d windows x %d temperatures; per iteration 80 ucb_mcmc steps per "
+                          "chain, exact Fisher + Jacobi eigen of every live source, ptmcmc + "
+                          "adapt_temperature_ladder, one hipGraph; proposals: uniform prior 0.1, Fisher 0.7, "
+                          "fm shift 0.1, psi-phi 0.1; snr cut 5" % (args.groups, args.temps)}
+        log(f"sampler: {smpl['value']:.4e} steps/s, {sl['ms']:.2f} ms per iteration (steps eager "
+            f"{sl['ms_steps_eager']:.2f} ms, Fisher update {sl['ms_fisher']:.2f} ms), acceptance {sl['acceptance']}")
+
     gfit = None
     if args.globalfit:
         plan.close()
@@ -829,6 +900,7 @@ def main():
         "inner_product": inner,
         "globalfit": gfit,
         "noise": nzl,
+        "sampler": smpl,
         "parity": parity,
         "roofline": roofline,
         "cpu_baseline": cpu,

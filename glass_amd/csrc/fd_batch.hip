@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "../../include/glass_hip_batch.h"
+#include "fd_batch_hooks.h"
 #include "fd_derivs.h"
 #include "fd_waveform.h"
 #include "reduce.h"
@@ -81,6 +82,9 @@ struct Items
     const int* src;
     const double* params;
     const int* sel;
+    // optional device-side item count (items e >= *dn are left out); lets
+    // graph-captured callers size a pass on the device
+    const int* dn = nullptr;
 };
 
 __device__ inline int item_q(const Items& it, int e) { return it.sel ? it.sel[e] : e; }
@@ -114,7 +118,9 @@ __global__ __launch_bounds__(256) void align_kernel(int n, Items it, BatchDev B,
     __syncthreads();
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     int k = -1, slot = 0;
-    if (e < n)
+    // items past the device count, or with a negative source index (a
+    // proposal already rejected), are left out of every class list
+    if (e < n && (!it.dn || e < *it.dn) && item_src(it, e) >= 0)
     {
         const double* p = item_params(it, e);
         int c = item_chain(it, e);
@@ -388,7 +394,7 @@ __global__ __launch_bounds__(256) void chain_logl_kernel(BatchDev B)
 __global__ __launch_bounds__(256) void accept_add_old_kernel(int nacc, Items it, BatchDev B)
 {
     const int a = blockIdx.x;
-    if (a >= nacc) return;
+    if (a >= nacc || (it.dn && a >= *it.dn)) return;
     const int c = item_chain(it, a), s = item_src(it, a), so = c * B.maxsrc + s;
     const int g = B.c_group[c], W = B.g_W[g], cap = B.g_cap[g];
     const int io = B.s_imin[so], bo = B.s_bw[so];
@@ -410,7 +416,7 @@ __global__ __launch_bounds__(256) void accept_sub_new_kernel(int nacc, Items it,
                                                              const double* __restrict__ logl_new)
 {
     const int a = blockIdx.x;
-    if (a >= nacc) return;
+    if (a >= nacc || (it.dn && a >= *it.dn)) return;
     const int c = item_chain(it, a), s = item_src(it, a), so = c * B.maxsrc + s;
     const int g = B.c_group[c], W = B.g_W[g], cap = B.g_cap[g];
     const int in = imin_new[a], bn = bw_new[a];
@@ -1011,6 +1017,55 @@ extern "C" int glass_hip_fd_batch_accept(glass_hip_fd_batch* P, int nacc, const 
     GH_CHECK(hipGetLastError());
     return GLASS_HIP_OK;
 }
+
+// ------------------------------------------- hooks for the device sampler ---
+
+namespace glass {
+
+FdBatchView fd_batch_view(glass_hip_fd_batch* P)
+{
+    FdBatchView v{};
+    v.device = P->device;
+    v.C = P->C;
+    v.G = P->G;
+    v.maxsrc = P->maxsrc;
+    v.ninit = P->ninit;
+    v.cap = P->cls_cap;
+    v.T = P->T;
+    v.c_group = P->B.c_group;
+    v.c_nsrc = P->B.c_nsrc;
+    v.sparams = P->B.sparams;
+    v.logl = P->B.logl;
+    v.i_chain = P->i_chain;
+    v.i_src = P->i_src;
+    return v;
+}
+
+int fd_batch_eval_dev(glass_hip_fd_batch* P, int n, const int* chain, const int* src, const double* params,
+                      double* logl, double* snr, hipStream_t s)
+{
+    Items it{chain, src, params, nullptr};
+    return launch_items(P, n, it, 0, logl, snr, s);
+}
+
+int fd_batch_accept_dev(glass_hip_fd_batch* P, int nmax, const int* d_n, const int* d_idx, const int* chain,
+                        const int* src, const double* params, const double* logl, hipStream_t s)
+{
+    if (nmax <= 0) return GLASS_HIP_OK;
+    if (nmax > P->C || nmax > P->cls_cap)
+    {
+        set_error("batch accept: %d items exceed the plan (%d chains, capacity %d)", nmax, P->C, P->cls_cap);
+        return GLASS_HIP_EINVAL;
+    }
+    Items it{chain, src, params, d_idx, d_n};
+    accept_add_old_kernel<<<nmax, 256, 0, s>>>(nmax, it, P->B);
+    if (int rc = launch_items(P, nmax, it, 1, nullptr, nullptr, s)) return rc;
+    accept_sub_new_kernel<<<nmax, 256, 0, s>>>(nmax, it, P->B, P->p_bw, P->p_imin, logl);
+    GH_CHECK(hipGetLastError());
+    return GLASS_HIP_OK;
+}
+
+}  // namespace glass
 
 extern "C" int glass_hip_fd_batch_refresh(glass_hip_fd_batch* P, void* stream)
 {

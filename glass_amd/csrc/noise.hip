@@ -42,6 +42,7 @@
 
 #include "../../include/glass_hip_noise.h"
 #include "reduce.h"
+#include "rng.h"
 #include "runtime.h"
 
 namespace glass {
@@ -392,46 +393,6 @@ __global__ void noise_finish_kernel(const double* __restrict__ part, int nchunk,
     double s = 0.0;
     for (int k = 0; k < nchunk; k++) s += part[(size_t)k * n + i];
     out[i] = s + cst;
-}
-
-// ---------------------------------------------------- random numbers ---
-
-// rand_r of the GNU C library (three steps of its linear congruential
-// generator, 11 + 10 + 10 bits), RAND_MAX = 2^31 - 1
-__device__ inline int libc_rand_r(unsigned* seed)
-{
-    unsigned next = *seed;
-    int result;
-    next *= 1103515245u;
-    next += 12345u;
-    result = (int)((next / 65536u) % 2048u);
-    next *= 1103515245u;
-    next += 12345u;
-    result <<= 10;
-    result ^= (int)((next / 65536u) % 1024u);
-    next *= 1103515245u;
-    next += 12345u;
-    result <<= 10;
-    result ^= (int)((next / 65536u) % 1024u);
-    *seed = next;
-    return result;
-}
-
-__device__ inline double rand_U(unsigned* s) { return (double)libc_rand_r(s) / 2147483647.0; }
-
-__device__ inline double rand_N(unsigned* s)
-{
-    // rounded products, as the C library build computes them: a fused
-    // u*u + v*v moves q by an ulp, which log(q) magnifies by 1/(1-q)
-#pragma clang fp contract(off)
-    double u, v, q;
-    do
-    {
-        u = 2 * rand_U(s) - 1;
-        v = 2 * rand_U(s) - 1;
-        q = u * u + v * v;
-    } while (q >= 1 || q == 0);
-    return u * sqrt(-2 * log(q) / q);
 }
 
 // a link index; U = 1 exactly would give 6, one past the six links

@@ -33,6 +33,13 @@
  *   orc_my_noise_log_likelihood_wavelet -> numpy per-bin inverse and
  *                           log-determinant on small inputs
  *   orc_rand_*           -> the C library's rand_r itself
+ *   orc_jacobi_eigen8    -> numpy.linalg.eigh (LAPACK) on random and graded
+ *                           matrices, A v = lambda v, orthonormality
+ *   orc_ucb_propose, orc_ucb_prior, orc_ucb_accept, orc_ucb_ptmcmc,
+ *   orc_adapt_temperature_ladder -> closed-form prior values and wraps, the
+ *                           Fisher jump covariance F^-1/8, the fm-shift and
+ *                           psi-phi jump laws, Metropolis rates, a hand-worked
+ *                           ladder update
  *   orc_noise_mcmc_call, orc_noise_ptmcmc -> detailed balance of the swap
  *                           (exact swap probabilities), and the cold chain of
  *                           a long run settling on the injected noise levels
@@ -325,6 +332,35 @@ int orc_noise_mcmc_call(double fstar, const double *f, int N, const double *X, c
  * slots; index[] slot -> state, logL[] per state, seeds[] per slot */
 void orc_noise_ptmcmc(int NC, int *index, const double *temperature, const double *logL, unsigned int *seeds,
                       int *acceptance);
+
+/* ------------------------------------------------------- UCB sampler --- */
+/* Eigen-decomposition of a symmetric 8x8 matrix replacing matrix_eigenstuff
+ * (utils/src/glass_math.c:475-499): cyclic Jacobi; evec[i][j] = component i
+ * of eigenvector j, eigenvalues ascending, largest component of each vector
+ * positive.  Returns the sweeps used. */
+int orc_jacobi_eigen8(const double *F, double *evec, double *eval);
+/* evaluate_ucb_prior, galaxy/update flags off (ucb/src/glass_ucb_prior.c:
+ * 536-646); pr[8][2] = model->prior, lpv[8] its log volumes; wraps p */
+double orc_ucb_prior(double *p, const double *pr, const double *lpv);
+/* model->logPriorVolume (ucb/src/glass_ucb_prior.c:396) */
+void orc_ucb_log_prior_volume(const double *pr, double *lpv);
+/* the draws and proposal of one ucb_mcmc step (ucb/src/glass_ucb_sampler.c:
+ * 75-136, with the RJ coin of apps/src/ucb_mcmc.c:242): chain with nlive
+ * sources sparams[nlive][8] (the chosen source's params wrapped in place by
+ * the prior), their eigen data evec[nlive][64], eval[nlive][8]; returns the
+ * source n, proposal type np, proposed y[8], log priors of x and y */
+void orc_ucb_propose(unsigned int *seed, int nlive, double *sparams, const double *evec, const double *eval,
+                     const double *pr, const double *lpv, const double *w, double T, int fix_sky, int fix_freq,
+                     int fix_fdot, int *n_out, int *np_out, double *y, double *lpx_out, double *lpy_out);
+/* the rest of the step when the prior of y is finite (:138-172): snr cut,
+ * Hastings ratio, accept draw; returns 1 to accept */
+int orc_ucb_accept(unsigned int *seed, double logLy, double logLx, double snr, double snr_min, double temperature,
+                   double lpx, double lpy);
+/* ptmcmc (utils/src/glass_sampler.c:19-59) with model->logLnorm */
+void orc_ucb_ptmcmc(int NC, int *index, const double *temperature, const double *logL, const double *lognorm,
+                    unsigned int *seeds, int *acceptance);
+/* adapt_temperature_ladder (utils/src/glass_sampler.c:61-94) */
+void orc_adapt_temperature_ladder(int NC, double *temperature, const int *acceptance, int mcmc);
 
 #ifdef __cplusplus
 }

@@ -94,6 +94,7 @@ def lib():
         L.orc_eval_batch.restype = ctypes.c_double
         _wdm_signatures(L)
         _noise_signatures(L)
+        _sampler_signatures(L)
         _lib = L
     return _lib
 
@@ -653,3 +654,93 @@ def noise_ptmcmc(index, temperature, logL, seeds):
     lib().orc_noise_ptmcmc(idx.size, _i(idx), _d(t), _d(lg), sd.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)),
                            _i(acc))
     return idx, sd, acc
+
+
+# ---------------------------------------------------------- UCB sampler ---
+
+def _sampler_signatures(L):
+    _up = ctypes.POINTER(ctypes.c_uint)
+    d, i = ctypes.c_double, ctypes.c_int
+    L.orc_jacobi_eigen8.argtypes = [_dp, _dp, _dp]
+    L.orc_jacobi_eigen8.restype = i
+    L.orc_ucb_prior.argtypes = [_dp, _dp, _dp]
+    L.orc_ucb_prior.restype = d
+    L.orc_ucb_log_prior_volume.argtypes = [_dp, _dp]
+    L.orc_ucb_propose.argtypes = [_up, i, _dp, _dp, _dp, _dp, _dp, _dp, d, i, i, i, _ip, _ip, _dp, _dp, _dp]
+    L.orc_ucb_accept.argtypes = [_up, d, d, d, d, d, d, d]
+    L.orc_ucb_accept.restype = i
+    L.orc_ucb_ptmcmc.argtypes = [i, _ip, _dp, _dp, _dp, _up, _ip]
+    L.orc_adapt_temperature_ladder.argtypes = [i, _dp, _ip, i]
+
+
+def jacobi_eigen8(F):
+    """(evec[8,8], eval[8], sweeps) of the symmetric matrix F."""
+    F = np.ascontiguousarray(F, dtype=np.float64)
+    ev, la = np.zeros((8, 8)), np.zeros(8)
+    sw = lib().orc_jacobi_eigen8(_d(F), _d(ev), _d(la))
+    return ev, la, sw
+
+
+def log_prior_volume(pr):
+    pr = np.ascontiguousarray(pr, dtype=np.float64)
+    lpv = np.zeros(8)
+    lib().orc_ucb_log_prior_volume(_d(pr), _d(lpv))
+    return lpv
+
+
+def ucb_prior(p, pr, lpv=None):
+    """(logP, p wrapped) of evaluate_ucb_prior."""
+    p = np.array(p, dtype=np.float64)
+    pr = np.ascontiguousarray(pr, dtype=np.float64)
+    lpv = log_prior_volume(pr) if lpv is None else np.ascontiguousarray(lpv, dtype=np.float64)
+    return lib().orc_ucb_prior(_d(p), _d(pr), _d(lpv)), p
+
+
+class UcbProposer:
+    """Per-window prior box and the proposal weights/flags of the sampler."""
+
+    def __init__(self, w, T, fix_sky=0, fix_freq=0, fix_fdot=0, snr_min=5.0):
+        self.w = np.ascontiguousarray(w, dtype=np.float64)
+        self.T = float(T)
+        self.flags = (int(fix_sky), int(fix_freq), int(fix_fdot))
+        self.snr_min = float(snr_min)
+
+    def propose(self, seed, sparams, evec, eval_, pr):
+        """sparams [nlive, 8] (wrapped in place), evec [nlive, 64], eval
+        [nlive, 8], pr [8, 2] -> (seed, n, np, y, lpx, lpy)."""
+        s = ctypes.c_uint(int(seed) & 0xFFFFFFFF)
+        pr = np.ascontiguousarray(pr, dtype=np.float64)
+        lpv = log_prior_volume(pr)
+        n, npr = ctypes.c_int(), ctypes.c_int()
+        y = np.zeros(8)
+        lpx, lpy = ctypes.c_double(), ctypes.c_double()
+        ev = np.ascontiguousarray(evec, dtype=np.float64)
+        la = np.ascontiguousarray(eval_, dtype=np.float64)
+        lib().orc_ucb_propose(ctypes.byref(s), sparams.shape[0], _d(sparams), _d(ev), _d(la), _d(pr), _d(lpv),
+                              _d(self.w), self.T, *self.flags, ctypes.byref(n), ctypes.byref(npr), _d(y),
+                              ctypes.byref(lpx), ctypes.byref(lpy))
+        return s.value, n.value, npr.value, y, lpx.value, lpy.value
+
+    def accept(self, seed, logLy, logLx, snr, temperature, lpx, lpy):
+        s = ctypes.c_uint(int(seed) & 0xFFFFFFFF)
+        a = lib().orc_ucb_accept(ctypes.byref(s), logLy, logLx, snr, self.snr_min, temperature, lpx, lpy)
+        return s.value, bool(a)
+
+
+def ucb_ptmcmc(index, temperature, logL, lognorm, seeds):
+    idx = np.ascontiguousarray(index, dtype=np.int32).copy()
+    t = np.ascontiguousarray(temperature, dtype=np.float64)
+    lg = np.ascontiguousarray(logL, dtype=np.float64)
+    ln = np.ascontiguousarray(lognorm, dtype=np.float64)
+    sd = np.ascontiguousarray(seeds, dtype=np.uint32).copy()
+    acc = np.zeros(idx.size, dtype=np.int32)
+    lib().orc_ucb_ptmcmc(idx.size, _i(idx), _d(t), _d(lg), _d(ln), sd.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)),
+                         _i(acc))
+    return idx, sd, acc
+
+
+def adapt_temperature_ladder(temperature, acceptance, mcmc):
+    t = np.array(temperature, dtype=np.float64)
+    a = np.ascontiguousarray(acceptance, dtype=np.int32)
+    lib().orc_adapt_temperature_ladder(t.size, _d(t), _i(a), int(mcmc))
+    return t
