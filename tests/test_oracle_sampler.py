@@ -2,8 +2,9 @@
 not retype it:
 
 * the Jacobi eigen-decomposition against numpy.linalg.eigh (LAPACK) on
-  random and strongly graded (Fisher-like) matrices, A v = lambda v,
-  orthonormality, the order and sign conventions;
+  random matrices and against a 50-digit mpmath eigensolve on graded
+  (Fisher-like) ones, A v = lambda v, orthonormality, the order and sign
+  conventions;
 * the prior: its value for in-range parameters is minus the sum of the log
   prior volumes, out-of-range parameters give -inf, angles wrap into range;
 * the proposals: Fisher jumps have covariance F^-1 / 8 (one of the 8
