@@ -467,6 +467,79 @@ def sampler_leg(plan, cfg, nladder, ntemp, nsweep=3, nsteps=80, seed=41):
                 swap_rate=round(float(st["swaps"].sum() / max(1, (nsweep + 2) * nladder * (ntemp - 1))), 4))
 
 
+def fstat_leg(gl, seed=51, NFFT=1024, fq=3e-3, cpu_cells=200):
+    """The F-statistic proposal density of one data segment
+    (glass_hip_fstat_proposal, build_fstatistic_proposal of
+    ucb/src/glass_ucb_proposal.c:1086-1180): NFFT = 1024 bins at 3 mHz,
+    Tobs 2 yr, three binaries in noise; n_f = 1024 frequency cells x 20 x 20
+    sky cells, each an F-statistic of four filter templates, plus the
+    chirp-mass scan of fdot (10 more) for the cells with logL > 1.  The
+    oracle's get_Fstat_logL on a sample of the cells is the CPU baseline."""
+    import torch
+    from glass_amd.fstat import Fstat
+    from oracle import orc
+    T = 62914560.0
+    o = orc.analytic_orbit()
+    rng = np.random.default_rng(seed)
+    qmin0 = int(fq * T)
+    C, _, _ = orc.xyz_noise(T, qmin0 / T, NFFT, orbit=o)
+    C00 = C[0, 0].copy()
+    X, A, E = (np.zeros(NFFT, dtype=np.complex128) for _ in range(3))
+    for _ in range(3):
+        f0 = (qmin0 + rng.uniform(100, NFFT - 100)) / T
+        theta, phi = np.arccos(rng.uniform(-1, 1)), rng.uniform(0, 2 * np.pi)
+        M = orc.ucb_bandwidth(f0, 0.0, np.cos(theta), 1e-22, T, NFFT, orbit=o)
+        p = np.array([f0 * T, np.cos(theta), phi, np.log(2e-22), rng.uniform(-1, 1), rng.uniform(0, np.pi),
+                      rng.uniform(0, 2 * np.pi), 0.0])
+        h = orc.ucb_waveform(p, T, M, NI=2, orbit=o)
+        q = int(p[0])
+        for i in range(M):
+            k = q - qmin0 + i - M // 2
+            if 0 < k < NFFT:
+                X[k] += h[0][i]
+                A[k] += h[3][i]
+                E[k] += h[4][i]
+    s = np.sqrt(C00 / 2)
+    for ch in (X, A, E):
+        ch += s * (rng.standard_normal(NFFT) + 1j * rng.standard_normal(NFFT))
+    fmin, fmax = (qmin0 + 1) / T, (qmin0 + NFFT - 1) / T
+    F = Fstat(gl.analytic_orbit(), "sangria", T, 0.0, qmin0, X, A, E, C00, max_cells=1 << 19)
+    F.proposal(fmin, fmin + 8 / T)  # warm up (kernels, tables)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ten, norm, maxp, nf, df = F.proposal(fmin, fmax)
+    secs = time.perf_counter() - t0
+    ncell = ten.size
+    # the fdot = 0 pass alone, as device time
+    cells = np.zeros((ncell, 4))
+    d_theta, d_phi = 2.0 / 20, 6.283185307179586 / 20
+    ii, jj, kk = np.meshgrid(np.arange(nf), np.arange(20), np.arange(20), indexing="ij")
+    cells[:, 0] = (fmin + ii.reshape(-1) * df) + df / 2
+    cells[:, 2] = np.arccos(-1.0 + jj.reshape(-1) * d_theta + d_theta / 2)
+    cells[:, 3] = kk.reshape(-1) * d_phi + d_phi / 2
+    dc = torch.tensor(cells[:1 << 19], device="cuda")
+    F.eval(dc)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    lx, la, fp = F.eval(dc)
+    e1.record()
+    torch.cuda.synchronize()
+    ms_eval = e0.elapsed_time(e1)
+    n_eval = dc.shape[0]
+    # parity on a sample of cells, and the CPU baseline
+    fd = orc.FstatData(T, qmin0, X, A, E, C00, orbit=o)
+    smp = rng.choice(n_eval, cpu_cells, replace=False)
+    la_h = la.cpu().numpy()
+    t1 = time.perf_counter()
+    ref = [fd.logl(*cells[c])[1] for c in smp]
+    cpu_s = time.perf_counter() - t1
+    worst = float(np.max(np.abs(la_h[smp] - np.array(ref)) / np.abs(ref)))
+    F.close()
+    return dict(ncell=ncell, nf=nf, secs=secs, maxp=maxp, ms_eval=ms_eval, n_eval=n_eval, max_rel=worst,
+                cpu_rate=cpu_cells / cpu_s, cpu_s=cpu_s, cpu_cells=cpu_cells)
+
+
 def globalfit_leg(gl, synth, steps=3, seed=4, cpu_seconds=2.0, nthreads=0):
     """Global-fit UCB block at full scale on one GPU: 20,000 binaries (f0
     log-uniform 0.1-25 mHz, A log-uniform 1e-24..1e-21), Tobs 4 yr, the
@@ -627,6 +700,7 @@ def main():
     ap.add_argument("--no-inner", action="store_true")
     ap.add_argument("--no-noise", action="store_true")
     ap.add_argument("--no-sampler", action="store_true")
+    ap.add_argument("--no-fstat", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default tiles (skip the autotuner)")
     ap.add_argument("--globalfit", action="store_true",
                     help="also run the global-fit block at full scale (1,024,000 chains, ~170 GB of HBM)")
@@ -836,6 +910,25 @@ def main():
                   "pass": bool(leg["max_rel"] <= 1e-10)}
         log(f"cpu baseline {leg['rate']:.1f} evals/s on {nthr} threads; parity max rel {leg['max_rel']:.3e}")
 
+    fst = None
+    if not args.no_fstat:
+        fl = fstat_leg(gl)
+        fst = {"value": round(fl["n_eval"] / (fl["ms_eval"] * 1e-3), 1),
+               "unit": "F-statistic cells/s (4 filter templates + 28 sums + two 4x4 inverses per cell)",
+               "cells_per_launch": fl["n_eval"], "ms_per_launch": round(fl["ms_eval"], 3),
+               "proposal_build_s": round(fl["secs"], 3), "proposal_cells": fl["ncell"], "frequency_cells": fl["nf"],
+               "parity": {"checked": fl["cpu_cells"], "max_logL_AE_rel_err": fl["max_rel"], "gate": 1e-10,
+                          "pass": bool(fl["max_rel"] <= 1e-10)},
+               "cpu_baseline": {"value": round(fl["cpu_rate"], 1), "unit": "cells/s", "cores": 1, "kind": "port",
+                                "sample": f"{fl['cpu_cells']} cells by the oracle's get_Fstat_logL, "
+                                          f"single-threaded, {fl['cpu_s']:.2f} s"},
+               "config": "F-statistic proposal of a 1024-bin segment at 3 mHz, Tobs 2 yr, three binaries in noise: "
+                         "1022 x 20 x 20 cells at fdot = 0 + the chirp-mass fdot scan of cells with logL > 1 "
+                         "(build_fstatistic_proposal)"}
+        log(f"fstat: {fst['value']:.4e} cells/s ({fl['ms_eval']:.2f} ms / {fl['n_eval']} cells), proposal of "
+            f"{fl['ncell']} cells in {fl['secs']:.3f} s, parity {fl['max_rel']:.2e}, oracle {fl['cpu_rate']:.0f} "
+            f"cells/s")
+
     smpl = None
     if not args.no_sampler:
         sl = sampler_leg(plan, cfg, args.groups, args.temps)
@@ -901,6 +994,7 @@ def main():
         "globalfit": gfit,
         "noise": nzl,
         "sampler": smpl,
+        "fstat": fst,
         "parity": parity,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -40,6 +40,11 @@
  *                           Fisher jump covariance F^-1/8, the fm-shift and
  *                           psi-phi jump laws, Metropolis rates, a hand-worked
  *                           ladder update
+ *   orc_get_Fstat_logL   -> the F-statistic identity: on noise-free data of
+ *                           the cell's own intrinsic parameters F = (d|d)/2
+ *                           for any extrinsic parameters, quadratic in the
+ *                           data; the fit recovers cos(iota); the inverse
+ *                           against numpy
  *   orc_noise_mcmc_call, orc_noise_ptmcmc -> detailed balance of the swap
  *                           (exact swap probabilities), and the cold chain of
  *                           a long run settling on the injected noise levels
@@ -361,6 +366,17 @@ void orc_ucb_ptmcmc(int NC, int *index, const double *temperature, const double 
                     unsigned int *seeds, int *acceptance);
 /* adapt_temperature_ladder (utils/src/glass_sampler.c:61-94) */
 void orc_adapt_temperature_ladder(int NC, double *temperature, const int *acceptance, int mcmc);
+
+/* ----------------------------------------------------- F-statistic ---- */
+/* get_Fstat_logL (ucb/src/glass_ucb_fstatistic.c:646-703): X, A, E the
+ * data DFT [2 NFFT] from bin qmin0, C00 [NFFT]; returns the filter length */
+int orc_get_Fstat_logL(const orc_orbit *orbit, const char *format, double T, double t0, int NFFT, int qmin0,
+                       const double *X, const double *A, const double *E, const double *C00, double f0, double fdot,
+                       double theta, double phi, double *logL_X, double *logL_AE, double *Fparams);
+/* the 4x4 inverse used for the F-statistic Grams (row-major in/out) */
+void orc_invert4(const double *in, double *out);
+/* ucb_fdot (ucb/src/glass_ucb_waveform.c:307-314) */
+double orc_ucb_fdot(double Mc, double f0);
 
 #ifdef __cplusplus
 }

@@ -95,6 +95,7 @@ def lib():
         _wdm_signatures(L)
         _noise_signatures(L)
         _sampler_signatures(L)
+        _fstat_signatures(L)
         _lib = L
     return _lib
 
@@ -744,3 +745,91 @@ def adapt_temperature_ladder(temperature, acceptance, mcmc):
     a = np.ascontiguousarray(acceptance, dtype=np.int32)
     lib().orc_adapt_temperature_ladder(t.size, _d(t), _i(a), int(mcmc))
     return t
+
+
+# ---------------------------------------------------------- F-statistic ---
+
+def _fstat_signatures(L):
+    d, i = ctypes.c_double, ctypes.c_int
+    L.orc_get_Fstat_logL.argtypes = [ctypes.POINTER(Orbit), ctypes.c_char_p, d, d, i, i, _dp, _dp, _dp, _dp, d, d, d,
+                                     d, _dp, _dp, _dp]
+    L.orc_get_Fstat_logL.restype = i
+    L.orc_invert4.argtypes = [_dp, _dp]
+    L.orc_ucb_fdot.argtypes = [d, d]
+    L.orc_ucb_fdot.restype = d
+
+
+class FstatData:
+    """A data segment for the F-statistic: X, A, E complex [NFFT] from bin
+    qmin0, C00 [NFFT]."""
+
+    def __init__(self, T, qmin0, X, A, E, C00, t0=0.0, fmt="sangria", orbit=None):
+        self.orbit = orbit or analytic_orbit()
+        self.T, self.t0, self.qmin0, self.fmt = float(T), float(t0), int(qmin0), fmt
+        self.X = np.ascontiguousarray(X, dtype=np.complex128).view(np.float64)
+        self.A = np.ascontiguousarray(A, dtype=np.complex128).view(np.float64)
+        self.E = np.ascontiguousarray(E, dtype=np.complex128).view(np.float64)
+        self.C00 = np.ascontiguousarray(C00, dtype=np.float64)
+        self.NFFT = self.C00.size
+
+    def logl(self, f0, fdot, theta, phi):
+        """(logL_X, logL_AE, Fparams[4], filter length) of get_Fstat_logL."""
+        lx, la = ctypes.c_double(), ctypes.c_double()
+        fp = np.zeros(4)
+        M = lib().orc_get_Fstat_logL(ctypes.byref(self.orbit), self.fmt.encode(), self.T, self.t0, self.NFFT,
+                                     self.qmin0, _d(self.X), _d(self.A), _d(self.E), _d(self.C00), float(f0),
+                                     float(fdot), float(theta), float(phi), ctypes.byref(lx), ctypes.byref(la),
+                                     _d(fp))
+        return lx.value, la.value, fp, M
+
+
+def invert4(m):
+    m = np.ascontiguousarray(m, dtype=np.float64)
+    out = np.zeros((4, 4))
+    lib().orc_invert4(_d(m), _d(out))
+    return out
+
+
+def ucb_fdot(Mc, f0):
+    return lib().orc_ucb_fdot(float(Mc), float(f0))
+
+
+def fstat_proposal(fd, fmin, fmax, n_theta=20, n_phi=20):
+    """build_fstatistic_proposal on the oracle: (tensor [n_f, n_theta,
+    n_phi], norm, maxp, n_f, d_f)."""
+    T = fd.T
+    n_f = int((fmax - fmin) * T)
+    d_f = int((fmax - fmin) * T) / float(n_f) / T
+    d_theta, d_phi = 2.0 / n_theta, 6.283185307179586 / n_phi
+    ten = np.zeros((n_f, n_theta, n_phi))
+    d_fdot = (1.1 - 0.1) / 10
+    for i in range(n_f):
+        f = fmin + i * d_f
+        f += d_f / 2
+        for j in range(n_theta):
+            for k in range(n_phi):
+                theta = np.arccos(-1.0 + j * d_theta + d_theta / 2)
+                phi = k * d_phi + d_phi / 2
+                la = fd.logl(f, 0.0, theta, phi)[1]
+                lmax = la
+                if la > 1:
+                    for n in range(10):
+                        la = fd.logl(f, ucb_fdot(0.1 + n * d_fdot, f), theta, phi)[1]
+                        if la > lmax:
+                            lmax = la
+                        if la < 1:
+                            break
+                ten[i, j, k] = lmax * lmax
+    norm = 0.0
+    for v in ten.reshape(-1):
+        norm += v
+    norm = (n_f * n_theta * n_phi) / norm
+    ten *= norm
+    return ten, norm, float(ten.max()), n_f, d_f
+
+
+def ucb_bandwidth(f, fdot, costheta, A, T, N, orbit=None):
+    """ucb_bandwidth (ucb/src/glass_ucb_waveform.c:698-725)."""
+    orbit = orbit or analytic_orbit()
+    return lib().orc_ucb_bandwidth(orbit.L, orbit.fstar, float(f), float(fdot), float(costheta), float(A),
+                                   float(T), int(N))
