@@ -15,9 +15,10 @@
 //   check_range, evaluate_ucb_prior, evaluate_uniform_priors,
 //   evalaute_sky_location_prior   ucb/src/glass_ucb_prior.c:400-646
 //   ptmcmc, adapt_temperature_ladder  utils/src/glass_sampler.c:19-94
-// Every proposal built here is symmetric or, for the uniform prior draw, has
-// equal densities both ways, so log q(x|y) - log q(y|x) = 0 exactly, as the
-// reference computes it.
+// The F-statistic proposals read a per-window proposal density (the tensor
+// of build_fstatistic_proposal, e.g. from glass_hip_fstat_proposal) set with
+// glass_hip_ucb_sampler_set_fstat; their densities enter the Hastings ratio
+// as in the reference, the other proposals' cancel exactly.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -75,6 +76,13 @@ struct SamplerDev
     int* prop;
     int* acc_list;
     int* nacc;
+    double* lq;  // [S] log q(x|y) - log q(y|x) of the step's proposal
+    // F-statistic proposal densities per window (null: none):
+    // tensor [n_f][n_theta][n_phi], dims (n_f, n_theta, n_phi),
+    // par (d_f, maxp, data qmin)
+    const double* const* fs_ten;
+    const int* fs_dims;
+    const double* fs_par;
     // options
     double w[9];
     int fix_sky, fix_freq, fix_fdot;
@@ -170,6 +178,74 @@ __device__ inline double ucb_prior(double* p, const double* pr, const double* lp
     return logP;
 }
 
+struct FsProp
+{
+    const double* ten;
+    int nf, nt, nph;
+    double d_f, maxp, qmin, d_theta, d_phi;
+};
+
+__device__ inline FsProp fs_prop(const SamplerDev& S, int g)
+{
+    FsProp f;
+    f.ten = S.fs_ten[g];
+    f.nf = S.fs_dims[3 * g];
+    f.nt = S.fs_dims[3 * g + 1];
+    f.nph = S.fs_dims[3 * g + 2];
+    f.d_f = S.fs_par[3 * g];
+    f.maxp = S.fs_par[3 * g + 1];
+    f.qmin = S.fs_par[3 * g + 2];
+    f.d_theta = 2. / (double)f.nt;  // setup_fstatistic_proposal, ucb/src/glass_ucb_proposal.c:939-940
+    f.d_phi = kTwoPi / (double)f.nph;
+    return f;
+}
+
+// evaluate_fstatistic_proposal (ucb/src/glass_ucb_proposal.c:1640-1673);
+// wraps p in place through the priors it calls
+__device__ inline double fstat_density(double* p, const double* pr, const double* lpv, const FsProp& f, double T)
+{
+    double logP = 0.0;
+    logP += sky_prior(p, pr, lpv);
+    logP += uniform_priors(p, pr, lpv);
+    const int i = (int)floor((p[0] - f.qmin) / (f.d_f * T));
+    const int j = (int)floor((p[1] - -1) / f.d_theta);
+    const int k = (int)floor((p[2]) / f.d_phi);
+    if (i < 0 || i >= f.nf) return -INFINITY;
+    if (j < 0 || j >= f.nt) return -INFINITY;
+    if (k < 0 || k >= f.nph) return -INFINITY;
+    logP += log(f.ten[((size_t)i * f.nt + j) * f.nph + k]);
+    return logP;
+}
+
+// the rejection sampling of draw_from_fstatistic / jump_from_fstatistic
+// (ucb/src/glass_ucb_proposal.c:1526-1638); jump: the reference's range
+// check (which a draw of U = 1 fails) ends the proposal with y[0..2] as
+// they were
+__device__ inline void fstat_sample(unsigned* seed, double* y, const FsProp& f, double T, bool jump)
+{
+    double i, j, k, q, costheta, phi, p, alpha;
+    do
+    {
+        i = rand_U(seed) * f.nf;
+        j = rand_U(seed) * f.nt;
+        k = rand_U(seed) * f.nph;
+        q = f.qmin + i * f.d_f * T;
+        costheta = -1. + j * f.d_theta;
+        phi = k * f.d_phi;
+        int ii = (int)i, jj = (int)j, kk = (int)k;
+        if (jump && (ii < 0 || ii > f.nf - 1 || jj < 0 || jj > f.nt - 1 || kk < 0 || kk > f.nph - 1)) return;
+        // draw_from_fstatistic has no such check; U = 1 would index one past
+        ii = ii > f.nf - 1 ? f.nf - 1 : ii;
+        jj = jj > f.nt - 1 ? f.nt - 1 : jj;
+        kk = kk > f.nph - 1 ? f.nph - 1 : kk;
+        p = f.ten[((size_t)ii * f.nt + jj) * f.nph + kk];
+        alpha = rand_U(seed) * f.maxp;
+    } while (p < alpha);
+    y[0] = q;
+    y[1] = costheta;
+    y[2] = phi;
+}
+
 __global__ void ucb_propose_kernel(SamplerDev S)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -199,6 +275,15 @@ __global__ void ucb_propose_kernel(SamplerDev S)
     if (np == 0)
     {
         for (int k = 0; k < 8; k++) y[k] = pr[2 * k] + rand_U(&seed) * (pr[2 * k + 1] - pr[2 * k]);
+    }
+    else if (np == 1 || np == 2)
+    {
+        const FsProp f = fs_prop(S, g);
+        if (np == 1)
+            for (int k = 0; k < 8; k++) y[k] = pr[2 * k] + rand_U(&seed) * (pr[2 * k + 1] - pr[2 * k]);
+        else
+            (void)(rand_U(&seed) < -0.5);  // the fm-shift branch of jump_from_fstatistic, never taken
+        fstat_sample(&seed, y, f, S.T, np == 2);
     }
     else if (np == 4)
     {
@@ -247,6 +332,20 @@ __global__ void ucb_propose_kernel(SamplerDev S)
     }
     if (S.fix_freq) y[0] = x[0];
     if (S.fix_fdot) y[7] = x[7];
+    // proposal densities, y first then x (:125-126); only the F-statistic
+    // and uniform-prior proposals have one, and the latter cancels exactly
+    double lqyx = 0.0, lqxy = 0.0;
+    if (np == 0)
+    {
+        for (int k = 0; k < 8; k++) lqyx -= lpv[k];
+        for (int k = 0; k < 8; k++) lqxy -= lpv[k];
+    }
+    else if (np == 1 || np == 2)
+    {
+        const FsProp f = fs_prop(S, g);
+        lqyx = fstat_density(y, pr, lpv, f, S.T);
+        lqxy = fstat_density(x, pr, lpv, f, S.T);
+    }
     const double lpx = ucb_prior(x, pr, lpv);
     const double lpy = ucb_prior(y, pr, lpv);
     for (int k = 0; k < 8; k++) xs[k] = x[k];  // the prior's in-place wraps of the current state
@@ -255,6 +354,7 @@ __global__ void ucb_propose_kernel(SamplerDev S)
     for (int k = 0; k < 8; k++) S.it_params[(size_t)s * 8 + k] = y[k];
     S.lpx[s] = lpx;
     S.lpy[s] = lpy;
+    S.lq[s] = lqxy - lqyx;
     S.prop[s] = np;
     S.seed[s] = seed;
 }
@@ -270,7 +370,7 @@ __global__ void ucb_accept_kernel(SamplerDev S)
     double logH = 0.0;
     logH += (S.it_logl[s] - S.logl[st]) / S.temp[s];
     logH += lpy - S.lpx[s];
-    logH += 0.0 - 0.0;  // log q(x|y) - log q(y|x)
+    logH += S.lq[s];  // log q(x|y) - log q(y|x)
     const double loga = log(rand_U(&seed));
     if (isfinite(logH) && logH > loga)
     {
@@ -359,6 +459,11 @@ struct glass_hip_ucb_sampler
     int nladder = 0, ntemp = 0, nslot = 0;
     std::vector<void*> allocs;
     SamplerDev D{};
+    // F-statistic proposal densities per window
+    std::vector<double*> fs_buf;
+    const double** d_fs_ten = nullptr;
+    int* d_fs_dims = nullptr;
+    double* d_fs_par = nullptr;
     // Fisher update scratch over the live sources
     double *fL = nullptr, *fgrad = nullptr, *fF = nullptr, *fvec = nullptr, *fval = nullptr;
     hipGraph_t graph = nullptr;
@@ -398,6 +503,8 @@ extern "C" int glass_hip_ucb_sampler_destroy(glass_hip_ucb_sampler* S)
     if (S->graph) (void)hipGraphDestroy(S->graph);
     (void)hipDeviceSynchronize();
     for (void* p : S->allocs) (void)hipFree(p);
+    for (double* p : S->fs_buf)
+        if (p) (void)hipFree(p);
     delete S;
     return GLASS_HIP_OK;
 }
@@ -422,11 +529,11 @@ extern "C" int glass_hip_ucb_sampler_create(glass_hip_ucb_sampler** out, glass_h
     double wsum = 0.0;
     for (int i = 0; i < 9; i++)
     {
-        const bool built = i == 0 || i == 4 || i == 5 || i == 6;
+        const bool built = i == 0 || i == 1 || i == 2 || i == 4 || i == 5 || i == 6;
         if (!(opts->weight[i] >= 0.0) || (!built && opts->weight[i] != 0.0))
         {
-            set_error("ucb_sampler_create: proposal %d has weight %g (only 0 uniform prior, 4 Fisher, 5 fm shift, "
-                      "6 psi-phi jump are built)", i, opts->weight[i]);
+            set_error("ucb_sampler_create: proposal %d has weight %g (only 0 uniform prior, 1/2 F-statistic, "
+                      "4 Fisher, 5 fm shift, 6 psi-phi jump are built)", i, opts->weight[i]);
             return GLASS_HIP_EINVAL;
         }
         wsum += opts->weight[i];
@@ -478,7 +585,9 @@ extern "C" int glass_hip_ucb_sampler_create(glass_hip_ucb_sampler** out, glass_h
         (rc = salloc(S, &D.it_src, N)) || (rc = salloc(S, &D.it_params, 8 * (size_t)N)) ||
         (rc = salloc(S, &D.it_logl, N)) || (rc = salloc(S, &D.it_snr, N)) || (rc = salloc(S, &D.lpx, N)) ||
         (rc = salloc(S, &D.lpy, N)) || (rc = salloc(S, &D.prop, N)) || (rc = salloc(S, &D.acc_list, N)) ||
-        (rc = salloc(S, &D.nacc, 1)) || (rc = salloc(S, &S->fL, M)) || (rc = salloc(S, &S->fgrad, 8 * (size_t)M)) ||
+        (rc = salloc(S, &D.nacc, 1)) || (rc = salloc(S, &D.lq, N)) || (rc = salloc(S, &S->d_fs_ten, v.G)) ||
+        (rc = salloc(S, &S->d_fs_dims, 3 * (size_t)v.G)) || (rc = salloc(S, &S->d_fs_par, 3 * (size_t)v.G)) ||
+        (rc = salloc(S, &S->fL, M)) || (rc = salloc(S, &S->fgrad, 8 * (size_t)M)) ||
         (rc = salloc(S, &S->fF, 64 * (size_t)M)) || (rc = salloc(S, &S->fvec, 64 * (size_t)M)) ||
         (rc = salloc(S, &S->fval, 8 * (size_t)M)))
     {
@@ -500,6 +609,10 @@ extern "C" int glass_hip_ucb_sampler_create(glass_hip_ucb_sampler** out, glass_h
     D.evec = d_evec;
     D.eval = d_eval;
     for (int i = 0; i < 9; i++) D.w[i] = opts->weight[i];
+    D.fs_ten = S->d_fs_ten;
+    D.fs_dims = S->d_fs_dims;
+    D.fs_par = S->d_fs_par;
+    S->fs_buf.assign(v.G, nullptr);
     D.fix_sky = opts->fix_sky;
     D.fix_freq = opts->fix_freq;
     D.fix_fdot = opts->fix_fdot;
@@ -509,7 +622,13 @@ extern "C" int glass_hip_ucb_sampler_create(glass_hip_ucb_sampler** out, glass_h
     for (size_t m = 0; m < (size_t)N * v.maxsrc; m++)
         for (int i = 0; i < 8; i++) ev[m * 64 + i * 8 + i] = 1.0;
     std::vector<double> zero(N, 0.0);
-    if (hipMemcpyAsync(d_prior, prior, sizeof(double) * 16 * v.G, hipMemcpyHostToDevice, st) ||
+    std::vector<const double*> nullt(v.G, nullptr);
+    std::vector<int> zdims(3 * (size_t)v.G, 0);
+    std::vector<double> zpar(3 * (size_t)v.G, 0.0);
+    if (hipMemcpyAsync(S->d_fs_ten, nullt.data(), sizeof(double*) * v.G, hipMemcpyHostToDevice, st) ||
+        hipMemcpyAsync(S->d_fs_dims, zdims.data(), sizeof(int) * 3 * v.G, hipMemcpyHostToDevice, st) ||
+        hipMemcpyAsync(S->d_fs_par, zpar.data(), sizeof(double) * 3 * v.G, hipMemcpyHostToDevice, st) ||
+        hipMemcpyAsync(d_prior, prior, sizeof(double) * 16 * v.G, hipMemcpyHostToDevice, st) ||
         hipMemcpyAsync(d_lpv, lpv.data(), sizeof(double) * 8 * v.G, hipMemcpyHostToDevice, st) ||
         hipMemcpyAsync(d_evec, ev.data(), sizeof(double) * ev.size(), hipMemcpyHostToDevice, st) ||
         hipMemcpyAsync(d_eval, la.data(), sizeof(double) * la.size(), hipMemcpyHostToDevice, st) ||
@@ -554,6 +673,48 @@ extern "C" int glass_hip_ucb_sampler_set_state(glass_hip_ucb_sampler* S, const u
     return GLASS_HIP_OK;
 }
 
+extern "C" int glass_hip_ucb_sampler_set_fstat(glass_hip_ucb_sampler* S, int g, const double* tensor, int n_f,
+                                               double d_f, int n_theta, int n_phi, double maxp, int data_qmin,
+                                               void* stream)
+{
+    if (!S || g < 0 || g >= S->v.G || !tensor || n_f <= 0 || n_theta <= 0 || n_phi <= 0 || !(d_f > 0) ||
+        !(maxp > 0))
+    {
+        set_error("ucb_sampler_set_fstat: bad arguments (window %d)", g);
+        return GLASS_HIP_EINVAL;
+    }
+    GH_CHECK(hipSetDevice(S->v.device));
+    hipStream_t st = (hipStream_t)stream;
+    const size_t n = (size_t)n_f * n_theta * n_phi;
+    if (S->graph && drop_graph(S)) return GLASS_HIP_ERUNTIME;
+    GH_CHECK(hipStreamSynchronize(st));
+    if (S->fs_buf[g]) GH_CHECK(hipFree(S->fs_buf[g]));
+    S->fs_buf[g] = nullptr;
+    GH_CHECK(hipMalloc(&S->fs_buf[g], sizeof(double) * n));
+    const int dims[3] = {n_f, n_theta, n_phi};
+    const double par[3] = {d_f, maxp, (double)data_qmin};
+    const double* ptr = S->fs_buf[g];
+    GH_CHECK(hipMemcpyAsync(S->fs_buf[g], tensor, sizeof(double) * n, hipMemcpyHostToDevice, st));
+    GH_CHECK(hipMemcpyAsync(S->d_fs_ten + g, &ptr, sizeof(ptr), hipMemcpyHostToDevice, st));
+    GH_CHECK(hipMemcpyAsync(S->d_fs_dims + 3 * g, dims, sizeof(dims), hipMemcpyHostToDevice, st));
+    GH_CHECK(hipMemcpyAsync(S->d_fs_par + 3 * g, par, sizeof(par), hipMemcpyHostToDevice, st));
+    GH_CHECK(hipStreamSynchronize(st));
+    return GLASS_HIP_OK;
+}
+
+static int check_fstat(glass_hip_ucb_sampler* S)
+{
+    if (S->D.w[1] == 0.0 && S->D.w[2] == 0.0) return GLASS_HIP_OK;
+    for (int g = 0; g < S->v.G; g++)
+        if (!S->fs_buf[g])
+        {
+            set_error("ucb_sampler: F-statistic proposals weighted but window %d has no proposal density "
+                      "(glass_hip_ucb_sampler_set_fstat)", g);
+            return GLASS_HIP_EINVAL;
+        }
+    return GLASS_HIP_OK;
+}
+
 static int enqueue_fisher(glass_hip_ucb_sampler* S, hipStream_t st)
 {
     const FdBatchView& v = S->v;
@@ -569,6 +730,7 @@ static int enqueue_fisher(glass_hip_ucb_sampler* S, hipStream_t st)
 
 static int enqueue_steps(glass_hip_ucb_sampler* S, int nsteps, hipStream_t st)
 {
+    if (int rc = check_fstat(S)) return rc;
     const int N = S->nslot, g = (N + 255) / 256;
     SamplerDev& D = S->D;
     for (int k = 0; k < nsteps; k++)

@@ -28,6 +28,9 @@ _i = ctypes.c_int
 # proposals (initialize_ucb_proposal: uniform prior 0.1, fm shift 0.1,
 # psi-phi jump 0.1, Fisher the rest).
 DEFAULT_WEIGHTS = (0.1, 0.0, 0.0, 0.0, 0.7, 0.1, 0.1, 0.0, 0.0)
+# ... and with the F-statistic draw and jump (0.2 each), as the reference
+# runs by default (ucb/src/glass_ucb_proposal.c:636-747)
+FSTAT_WEIGHTS = (0.1, 0.2, 0.2, 0.0, 0.3, 0.1, 0.1, 0.0, 0.0)
 
 
 class Opts(ctypes.Structure):
@@ -48,6 +51,7 @@ _SIGS = [
     ("glass_hip_ucb_sampler_ptswap", [_vp, _i, _vp], _i),
     ("glass_hip_ucb_sampler_sweep", [_vp, _i, _i, _i, _vp], _i),
     ("glass_hip_ucb_sampler_get_state", [_vp, _up, _ip, _dp, _ip, _ip, _ip, _vp], _i),
+    ("glass_hip_ucb_sampler_set_fstat", [_vp, _i, _dp, _i, ctypes.c_double, _i, _i, ctypes.c_double, _i, _vp], _i),
 ]
 _declared = False
 
@@ -138,6 +142,13 @@ class UcbSampler:
 
     def fisher(self):
         check(_L().glass_hip_ucb_sampler_fisher(self.h, _stream()), "ucb_sampler_fisher")
+
+    def set_fstat(self, g, tensor, d_f, maxp, data_qmin):
+        """The F-statistic proposal density [n_f, n_theta, n_phi] of window g."""
+        t = np.ascontiguousarray(tensor, dtype=np.float64)
+        check(_L().glass_hip_ucb_sampler_set_fstat(self.h, int(g), _p(t), t.shape[0], float(d_f), t.shape[1],
+                                                   t.shape[2], float(maxp), int(data_qmin), _stream()),
+              "ucb_sampler_set_fstat")
 
     def eigen(self):
         ev = np.zeros((self.nslot, self.max_src, 8, 8))

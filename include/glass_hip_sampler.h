@@ -9,6 +9,8 @@
  *   draw_from_uniform_prior      utils/src/glass_sampler.c:96-107
  *   draw_from_fisher             ucb/src/glass_ucb_proposal.c:348-391
  *   fm_shift, psi_phi_jump       ucb/src/glass_ucb_proposal.c:577-613
+ *   draw_from_fstatistic, jump_from_fstatistic,
+ *   evaluate_fstatistic_proposal ucb/src/glass_ucb_proposal.c:1526-1673
  *   proposal table and weights   ucb/src/glass_ucb_proposal.c:615-760
  *   evaluate_ucb_prior et al.    ucb/src/glass_ucb_prior.c:396-646
  *   ptmcmc, adapt_temperature_ladder  utils/src/glass_sampler.c:19-94
@@ -44,8 +46,9 @@ typedef struct
 {
     /* proposal weights in the reference's table order: 0 uniform prior,
      * 1 F-statistic draw, 2 F-statistic jump, 3 extrinsic prior, 4 Fisher,
-     * 5 fm shift, 6 psi-phi jump, 7 GMM prior, 8 covariance.  Only 0, 4, 5
-     * and 6 are built: the others must be 0. */
+     * 5 fm shift, 6 psi-phi jump, 7 GMM prior, 8 covariance.  3, 7 and 8
+     * are not built and must be 0; 1 and 2 need every window's proposal
+     * density (glass_hip_ucb_sampler_set_fstat). */
     double weight[9];
     int fix_sky, fix_freq, fix_fdot; /* flags->fixSky, fixFreq, fixFdot */
     double snr_min;                  /* 5 in the reference */
@@ -66,6 +69,13 @@ int glass_hip_ucb_sampler_destroy(glass_hip_ucb_sampler *sampler);
  * resets the counters and the slot -> chain map to the identity. */
 int glass_hip_ucb_sampler_set_state(glass_hip_ucb_sampler *sampler, const unsigned *seeds, const double *temperature,
                                     const double *logLnorm, int adapt_iter0, void *stream);
+
+/* The F-statistic proposal density of window g (setup_fstatistic_proposal,
+ * ucb/src/glass_ucb_proposal.c:916-1084): host tensor [n_f][n_theta][n_phi]
+ * (normalised, e.g. from glass_hip_fstat_proposal), cell width d_f in Hz,
+ * its maximum maxp, and the window's first bin data->qmin. */
+int glass_hip_ucb_sampler_set_fstat(glass_hip_ucb_sampler *sampler, int g, const double *tensor, int n_f, double d_f,
+                                    int n_theta, int n_phi, double maxp, int data_qmin, void *stream);
 
 /* Fisher matrices of every live source at its current parameters (the
  * plan's exact derivatives, replacing ucb_fisher) and their eigen-

@@ -354,13 +354,21 @@ void orc_ucb_log_prior_volume(const double *pr, double *lpv);
  * sources sparams[nlive][8] (the chosen source's params wrapped in place by
  * the prior), their eigen data evec[nlive][64], eval[nlive][8]; returns the
  * source n, proposal type np, proposed y[8], log priors of x and y */
+typedef struct
+{
+    const double *ten; /* [nf][nt][nph] normalised F-statistic proposal density */
+    int nf, nt, nph;
+    double d_f, maxp, qmin; /* cell width (Hz), maximum, data->qmin of the window */
+} orc_fstat_prop;
 void orc_ucb_propose(unsigned int *seed, int nlive, double *sparams, const double *evec, const double *eval,
                      const double *pr, const double *lpv, const double *w, double T, int fix_sky, int fix_freq,
-                     int fix_fdot, int *n_out, int *np_out, double *y, double *lpx_out, double *lpy_out);
+                     int fix_fdot, const orc_fstat_prop *fs, int *n_out, int *np_out, double *y, double *lpx_out,
+                     double *lpy_out, double *lq_out);
 /* the rest of the step when the prior of y is finite (:138-172): snr cut,
- * Hastings ratio, accept draw; returns 1 to accept */
+ * Hastings ratio with lq = log q(x|y) - log q(y|x), accept draw; returns 1
+ * to accept */
 int orc_ucb_accept(unsigned int *seed, double logLy, double logLx, double snr, double snr_min, double temperature,
-                   double lpx, double lpy);
+                   double lpx, double lpy, double lq);
 /* ptmcmc (utils/src/glass_sampler.c:19-59) with model->logLnorm */
 void orc_ucb_ptmcmc(int NC, int *index, const double *temperature, const double *logL, const double *lognorm,
                     unsigned int *seeds, int *acceptance);

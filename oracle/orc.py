@@ -667,8 +667,9 @@ def _sampler_signatures(L):
     L.orc_ucb_prior.argtypes = [_dp, _dp, _dp]
     L.orc_ucb_prior.restype = d
     L.orc_ucb_log_prior_volume.argtypes = [_dp, _dp]
-    L.orc_ucb_propose.argtypes = [_up, i, _dp, _dp, _dp, _dp, _dp, _dp, d, i, i, i, _ip, _ip, _dp, _dp, _dp]
-    L.orc_ucb_accept.argtypes = [_up, d, d, d, d, d, d, d]
+    L.orc_ucb_propose.argtypes = [_up, i, _dp, _dp, _dp, _dp, _dp, _dp, d, i, i, i, ctypes.POINTER(FstatProp), _ip,
+                                  _ip, _dp, _dp, _dp, _dp]
+    L.orc_ucb_accept.argtypes = [_up, d, d, d, d, d, d, d, d]
     L.orc_ucb_accept.restype = i
     L.orc_ucb_ptmcmc.argtypes = [i, _ip, _dp, _dp, _dp, _up, _ip]
     L.orc_adapt_temperature_ladder.argtypes = [i, _dp, _ip, i]
@@ -697,6 +698,18 @@ def ucb_prior(p, pr, lpv=None):
     return lib().orc_ucb_prior(_d(p), _d(pr), _d(lpv)), p
 
 
+class FstatProp(ctypes.Structure):
+    _fields_ = [("ten", _dp), ("nf", ctypes.c_int), ("nt", ctypes.c_int), ("nph", ctypes.c_int),
+                ("d_f", ctypes.c_double), ("maxp", ctypes.c_double), ("qmin", ctypes.c_double)]
+
+
+def fstat_prop(tensor, d_f, maxp, qmin):
+    """(FstatProp, keep-alive array) of a proposal density [nf, nt, nph]."""
+    t = np.ascontiguousarray(tensor, dtype=np.float64)
+    return FstatProp(ten=_d(t), nf=t.shape[0], nt=t.shape[1], nph=t.shape[2], d_f=float(d_f), maxp=float(maxp),
+                     qmin=float(qmin)), t
+
+
 class UcbProposer:
     """Per-window prior box and the proposal weights/flags of the sampler."""
 
@@ -706,25 +719,27 @@ class UcbProposer:
         self.flags = (int(fix_sky), int(fix_freq), int(fix_fdot))
         self.snr_min = float(snr_min)
 
-    def propose(self, seed, sparams, evec, eval_, pr):
+    def propose(self, seed, sparams, evec, eval_, pr, fs=None):
         """sparams [nlive, 8] (wrapped in place), evec [nlive, 64], eval
-        [nlive, 8], pr [8, 2] -> (seed, n, np, y, lpx, lpy)."""
+        [nlive, 8], pr [8, 2], fs an FstatProp (or None) -> (seed, n, np, y,
+        lpx, lpy, lq)."""
         s = ctypes.c_uint(int(seed) & 0xFFFFFFFF)
         pr = np.ascontiguousarray(pr, dtype=np.float64)
         lpv = log_prior_volume(pr)
         n, npr = ctypes.c_int(), ctypes.c_int()
         y = np.zeros(8)
-        lpx, lpy = ctypes.c_double(), ctypes.c_double()
+        lpx, lpy, lq = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
         ev = np.ascontiguousarray(evec, dtype=np.float64)
         la = np.ascontiguousarray(eval_, dtype=np.float64)
         lib().orc_ucb_propose(ctypes.byref(s), sparams.shape[0], _d(sparams), _d(ev), _d(la), _d(pr), _d(lpv),
-                              _d(self.w), self.T, *self.flags, ctypes.byref(n), ctypes.byref(npr), _d(y),
-                              ctypes.byref(lpx), ctypes.byref(lpy))
-        return s.value, n.value, npr.value, y, lpx.value, lpy.value
+                              _d(self.w), self.T, *self.flags, None if fs is None else ctypes.byref(fs),
+                              ctypes.byref(n), ctypes.byref(npr), _d(y), ctypes.byref(lpx), ctypes.byref(lpy),
+                              ctypes.byref(lq))
+        return s.value, n.value, npr.value, y, lpx.value, lpy.value, lq.value
 
-    def accept(self, seed, logLy, logLx, snr, temperature, lpx, lpy):
+    def accept(self, seed, logLy, logLx, snr, temperature, lpx, lpy, lq=0.0):
         s = ctypes.c_uint(int(seed) & 0xFFFFFFFF)
-        a = lib().orc_ucb_accept(ctypes.byref(s), logLy, logLx, snr, self.snr_min, temperature, lpx, lpy)
+        a = lib().orc_ucb_accept(ctypes.byref(s), logLy, logLx, snr, self.snr_min, temperature, lpx, lpy, lq)
         return s.value, bool(a)
 
 

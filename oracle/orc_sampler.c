@@ -8,6 +8,8 @@
  *   draw_from_uniform_prior         utils/src/glass_sampler.c:96-107
  *   draw_from_fisher                ucb/src/glass_ucb_proposal.c:348-391
  *   fm_shift, psi_phi_jump          ucb/src/glass_ucb_proposal.c:577-613
+ *   draw_from_fstatistic, jump_from_fstatistic,
+ *   evaluate_fstatistic_proposal    ucb/src/glass_ucb_proposal.c:1526-1673
  *   check_range, evaluate_ucb_prior, evalaute_sky_location_prior,
  *   evaluate_uniform_priors         ucb/src/glass_ucb_prior.c:396-646
  *   ptmcmc, adapt_temperature_ladder  utils/src/glass_sampler.c:19-94
@@ -213,9 +215,55 @@ void orc_ucb_log_prior_volume(const double *pr, double *lpv)
 
 /* ---------------------------------------------------------- one step --- */
 
+/* evaluate_fstatistic_proposal (ucb/src/glass_ucb_proposal.c:1640-1673) */
+static double fstat_density(double *p, const double *pr, const double *lpv, const orc_fstat_prop *f, double T)
+{
+    const double d_theta = 2. / (double)f->nt, d_phi = S_PI2 / (double)f->nph;
+    double logP = 0.0;
+    logP += sky_prior(p, pr, lpv);
+    logP += uniform_priors(p, pr, lpv);
+    int i = (int)floor((p[0] - f->qmin) / (f->d_f * T));
+    int j = (int)floor((p[1] - -1) / d_theta);
+    int k = (int)floor((p[2]) / d_phi);
+    if (i < 0 || i >= f->nf) return -INFINITY;
+    if (j < 0 || j >= f->nt) return -INFINITY;
+    if (k < 0 || k >= f->nph) return -INFINITY;
+    logP += log(f->ten[((size_t)i * f->nt + j) * f->nph + k]);
+    return logP;
+}
+
+/* rejection sampling of draw_from_fstatistic / jump_from_fstatistic
+ * (:1526-1638); the jump's range check (failed only by U = 1) ends the
+ * proposal with y[0..2] unchanged, the draw's missing check is a clamp */
+static void fstat_sample(unsigned int *seed, double *y, const orc_fstat_prop *f, double T, int jump)
+{
+    const double d_theta = 2. / (double)f->nt, d_phi = S_PI2 / (double)f->nph;
+    double i, j, k, q, costheta, phi, p, alpha;
+    do
+    {
+        i = orc_rand_U_0_1(seed) * f->nf;
+        j = orc_rand_U_0_1(seed) * f->nt;
+        k = orc_rand_U_0_1(seed) * f->nph;
+        q = f->qmin + i * f->d_f * T;
+        costheta = -1. + j * d_theta;
+        phi = k * d_phi;
+        int ii = (int)i, jj = (int)j, kk = (int)k;
+        if (jump && (ii < 0 || ii > f->nf - 1 || jj < 0 || jj > f->nt - 1 || kk < 0 || kk > f->nph - 1)) return;
+        if (ii > f->nf - 1) ii = f->nf - 1;
+        if (jj > f->nt - 1) jj = f->nt - 1;
+        if (kk > f->nph - 1) kk = f->nph - 1;
+        p = f->ten[((size_t)ii * f->nt + jj) * f->nph + kk];
+        alpha = orc_rand_U_0_1(seed) * f->maxp;
+    } while (p < alpha);
+    y[0] = q;
+    y[1] = costheta;
+    y[2] = phi;
+}
+
 void orc_ucb_propose(unsigned int *seed, int nlive, double *sparams, const double *evec, const double *eval,
                      const double *pr, const double *lpv, const double *w, double T, int fix_sky, int fix_freq,
-                     int fix_fdot, int *n_out, int *np_out, double *y, double *lpx_out, double *lpy_out)
+                     int fix_fdot, const orc_fstat_prop *fs, int *n_out, int *np_out, double *y, double *lpx_out,
+                     double *lpy_out, double *lq_out)
 {
     (void)orc_rand_U_0_1(seed); /* the step loop's RJ coin, flags->rj off */
     int n = (int)(orc_rand_U_0_1(seed) * (double)nlive);
@@ -234,6 +282,14 @@ void orc_ucb_propose(unsigned int *seed, int nlive, double *sparams, const doubl
     if (np == 0)
     {
         for (int k = 0; k < 8; k++) y[k] = pr[2 * k] + orc_rand_U_0_1(seed) * (pr[2 * k + 1] - pr[2 * k]);
+    }
+    else if (np == 1 || np == 2)
+    {
+        if (np == 1)
+            for (int k = 0; k < 8; k++) y[k] = pr[2 * k] + orc_rand_U_0_1(seed) * (pr[2 * k + 1] - pr[2 * k]);
+        else
+            (void)(orc_rand_U_0_1(seed) < -0.5);
+        fstat_sample(seed, y, fs, T, np == 2);
     }
     else if (np == 4)
     {
@@ -281,6 +337,18 @@ void orc_ucb_propose(unsigned int *seed, int nlive, double *sparams, const doubl
     }
     if (fix_freq) y[0] = x[0];
     if (fix_fdot) y[7] = x[7];
+    double lqyx = 0.0, lqxy = 0.0;
+    if (np == 0)
+    {
+        for (int k = 0; k < 8; k++) lqyx -= lpv[k];
+        for (int k = 0; k < 8; k++) lqxy -= lpv[k];
+    }
+    else if (np == 1 || np == 2)
+    {
+        lqyx = fstat_density(y, pr, lpv, fs, T);
+        lqxy = fstat_density(x, pr, lpv, fs, T);
+    }
+    *lq_out = lqxy - lqyx;
     *lpx_out = orc_ucb_prior(x, pr, lpv);
     *lpy_out = orc_ucb_prior(y, pr, lpv);
     for (int k = 0; k < 8; k++) xs[k] = x[k];
@@ -289,13 +357,13 @@ void orc_ucb_propose(unsigned int *seed, int nlive, double *sparams, const doubl
 }
 
 int orc_ucb_accept(unsigned int *seed, double logLy, double logLx, double snr, double snr_min, double temperature,
-                   double lpx, double lpy)
+                   double lpx, double lpy, double lq)
 {
     if (snr < snr_min) lpy = -INFINITY;
     double logH = 0.0;
     logH += (logLy - logLx) / temperature;
     logH += lpy - lpx;
-    logH += 0.0 - 0.0;
+    logH += lq;
     const double loga = log(orc_rand_U_0_1(seed));
     return isfinite(logH) && logH > loga;
 }

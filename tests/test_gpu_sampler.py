@@ -77,10 +77,10 @@ def setup_sampler(gl, orc, seed=21, ngroup=3, ntemp=4, snr_min=1.0):
     return cfg, plan, smp, pr, seeds, temps
 
 
-def oracle_steps(orc, cfg, pr, seeds, temps, evec, eval_, nsteps, snr_min, logl0):
+def oracle_steps(orc, cfg, pr, seeds, temps, evec, eval_, nsteps, snr_min, logl0, w=W_DEFAULT, fsp=None):
     """Replays nsteps ucb_mcmc steps of every slot (slot s on chain s)."""
     T = cfg["T"]
-    P = orc.UcbProposer(W_DEFAULT, T, snr_min=snr_min)
+    P = orc.UcbProposer(w, T, snr_min=snr_min)
     cs, chains = oracle_chains(orc, cfg, cfg["src_params"])
     C = len(chains)
     out_seeds = np.zeros(C, dtype=np.int64)
@@ -99,14 +99,14 @@ def oracle_steps(orc, cfg, pr, seeds, temps, evec, eval_, nsteps, snr_min, logl0
         la = np.ascontiguousarray(eval_[c, :nlive])
         seed = int(seeds[c])
         for _ in range(nsteps):
-            seed, n, npr, y, lpx, lpy = P.propose(seed, sp, ev, la, pr[g])
+            seed, n, npr, y, lpx, lpy, lq = P.propose(seed, sp, ev, la, pr[g], None if fsp is None else fsp[g][0])
             trials[c, npr] += 1
             if lpy == -np.inf:
                 continue
             ly = orc.lib().orc_eval_logl(ctypes.byref(o), T, 0.0, ctypes.byref(cs.arr[c]), n,
                                          y.ctypes.data_as(orc._dp))
             sn = oracle_snr(orc, cfg, c, y)
-            seed, acc = P.accept(seed, ly, logl[c], sn, temps[c], lpx, lpy)
+            seed, acc = P.accept(seed, ly, logl[c], sn, temps[c], lpx, lpy, lq)
             if acc:
                 accepts[c, npr] += 1
                 b, _, i = orc.ucb_alignment(y, T, W, qmin)
@@ -219,9 +219,53 @@ def test_bad_sampler_arguments(gl, orc):
     plan = new_plan(gl, cfg, 4)
     pr = priors(cfg)
     w = W_DEFAULT.copy()
-    w[1] = 0.2  # F-statistic draw: not built
+    w[3] = 0.2  # extrinsic-prior draw: not built
     with pytest.raises(gl.GlassHipError):
         UcbSampler(plan, 2, 2, pr, weights=w)
     with pytest.raises(gl.GlassHipError):
         UcbSampler(plan, 3, 2, pr)
+    plan.close()
+
+
+def test_steps_with_fstat_proposals_match_oracle(gl, orc):
+    """The reference's default proposal table, F-statistic draw and jump
+    included, with a proposal density per window."""
+    from glass_amd.sampler import FSTAT_WEIGHTS, UcbSampler
+    w = np.array(FSTAT_WEIGHTS)
+    cfg = make_cfg(orc, seed=41, ngroup=3, ntemp=4, nprop=1)
+    C = len(cfg["chain_group"])
+    plan = new_plan(gl, cfg, C)
+    pr = priors(cfg)
+    smp = UcbSampler(plan, 3, 4, pr, weights=w, snr_min=1.0)
+    rng = np.random.default_rng(42)
+    seeds = rng.integers(1, 2 ** 32, C, dtype=np.uint64).astype(np.uint32)
+    temps = np.tile(1.25 ** np.arange(4), 3)
+    smp.set_state(seeds, temps)
+    with pytest.raises(gl.GlassHipError):
+        smp.steps(1)  # F-statistic weights without densities
+    T = cfg["T"]
+    fsp = []
+    for g in range(3):
+        W, q = int(cfg["group_W"][g]), int(cfg["group_qmin"][g])
+        ten = rng.uniform(0.2, 2.0, (W, 20, 20))
+        ten *= ten.size / ten.sum()
+        smp.set_fstat(g, ten, 1.0 / T, ten.max(), q)
+        fsp.append(orc.fstat_prop(ten, 1.0 / T, ten.max(), q))
+    smp.fisher()
+    ev, la = smp.eigen()
+    logl0, _ = plan.state()
+    nsteps = 20
+    smp.steps(nsteps)
+    st = smp.state()
+    logl, sp = plan.state()
+    params, lref, sref, trials, accepts = oracle_steps(orc, cfg, pr, seeds, temps, ev, la, nsteps, 1.0, logl0, w=w,
+                                                       fsp=fsp)
+    print("accepts", st["accepts"].sum(axis=0), "trials", st["trials"].sum(axis=0))
+    assert st["trials"][:, [1, 2]].sum() > 0
+    assert np.array_equal(st["trials"], trials)
+    assert np.array_equal(st["accepts"], accepts)
+    assert np.array_equal(st["seeds"], sref.astype(np.uint32))
+    assert rel(sp, params) < 1e-12
+    assert rel(logl, lref) < 1e-10
+    smp.close()
     plan.close()

@@ -137,7 +137,7 @@ def run_props(w, nrep, eig=None, seed0=1):
     seed = seed0
     for _ in range(nrep):
         sp = x.reshape(1, 8).copy()
-        seed, n, npr, y, lpx, lpy = P.propose(seed, sp, ev, la, pr)
+        seed, n, npr, y, lpx, lpy, lq = P.propose(seed, sp, ev, la, pr)
         out.append((npr, y.copy(), lpx, lpy))
     return x, pr, out
 
@@ -235,3 +235,44 @@ def test_adapt_ladder_hand_case():
     # the minimum ratio 1.1
     out = orc.adapt_temperature_ladder(np.array([1.0, 1.01, 3.0]), np.array([0, 1, 0], dtype=np.int32), 0)
     assert abs(out[1] - 1.1) < 1e-15
+
+
+def test_fstat_draws_follow_the_density():
+    """draw_from_fstatistic / jump_from_fstatistic land in a cell with
+    probability proportional to its density; the frequency/sky of the draw
+    lie in that cell; the proposal densities in the Hastings ratio are the
+    log densities of the two cells."""
+    rng = np.random.default_rng(9)
+    nf, nt, nph = 4, 5, 6
+    ten = rng.uniform(0.1, 3.0, (nf, nt, nph))
+    ten *= ten.size / ten.sum()
+    qmin = 1000.0
+    pr = box(qmin=qmin, W=nf)  # the frequency prior spans the density's band
+    d_f = 1.0 / T2
+    fs, keep = orc.fstat_prop(ten, d_f, ten.max(), qmin)
+    for npw in (1, 2):
+        w = np.zeros(9)
+        w[npw] = 1.0
+        P = orc.UcbProposer(w, T2)
+        x = inside(rng, pr)
+        counts = np.zeros(ten.shape)
+        seed = 77
+        n = 30000
+        for _ in range(n):
+            sp = x.reshape(1, 8).copy()
+            seed, _, npr, y, lpx, lpy, lq = P.propose(seed, sp, np.eye(8).reshape(1, 64), np.ones((1, 8)), pr,
+                                                      fs)
+            assert npr == npw
+            i = int(np.floor((y[0] - qmin) / (d_f * T2)))
+            j = int(np.floor((y[1] + 1) / (2.0 / nt)))
+            k = int(np.floor(y[2] / (2 * np.pi / nph)))
+            counts[i, j, k] += 1
+            ix = int(np.floor((x[0] - qmin) / (d_f * T2)))
+            jx = int(np.floor((x[1] + 1) / (2.0 / nt)))
+            kx = int(np.floor(sp[0, 2] / (2 * np.pi / nph)))
+            assert abs(lq - (np.log(ten[ix, jx, kx]) - np.log(ten[i, j, k]))) < 1e-12
+            if npw == 2:  # the jump keeps the other parameters
+                assert np.array_equal(y[3:], x[3:])
+        expect = n * ten / ten.sum()
+        chi2 = float(np.sum((counts - expect) ** 2 / expect))
+        assert chi2 < ten.size + 6 * np.sqrt(2 * ten.size), chi2
