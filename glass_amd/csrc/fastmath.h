@@ -31,19 +31,32 @@
 
 namespace glass {
 
+// A polynomial coefficient held in a scalar register pair.  gfx9 VALU
+// instructions cannot take a 64-bit literal, so a coefficient used as the
+// addend of a Horner step is otherwise rebuilt in vector registers (two
+// moves per step, as costly as the fma itself); from a scalar pair the step
+// is one v_fma_f64.  The value is unchanged.
+GH_HD inline double kc(double c)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+s"(c));
+#endif
+    return c;
+}
+
 // Taylor coefficients (-1)^n / (2n+1)!  and  (-1)^n / (2n)!
 // sin r = r + r^3 Q(r^2),  Q(z) = -1/3! + z/5! - z^2/7! + ... + z^7/17!
 GH_HD inline double poly_sin(double r)
 {
     const double z = r * r;
     double p = 1.0 / 355687428096000.0;             // 1/17!
-    p = fma(p, z, -1.0 / 1307674368000.0);          // -1/15!
-    p = fma(p, z, 1.0 / 6227020800.0);              // 1/13!
-    p = fma(p, z, -1.0 / 39916800.0);               // -1/11!
-    p = fma(p, z, 1.0 / 362880.0);                  // 1/9!
-    p = fma(p, z, -1.0 / 5040.0);                   // -1/7!
-    p = fma(p, z, 1.0 / 120.0);                     // 1/5!
-    p = fma(p, z, -1.0 / 6.0);                      // -1/3!
+    p = fma(p, z, kc(-1.0 / 1307674368000.0));          // -1/15!
+    p = fma(p, z, kc(1.0 / 6227020800.0));              // 1/13!
+    p = fma(p, z, kc(-1.0 / 39916800.0));               // -1/11!
+    p = fma(p, z, kc(1.0 / 362880.0));                  // 1/9!
+    p = fma(p, z, kc(-1.0 / 5040.0));                   // -1/7!
+    p = fma(p, z, kc(1.0 / 120.0));                     // 1/5!
+    p = fma(p, z, kc(-1.0 / 6.0));                      // -1/3!
     return fma(r * z, p, r);
 }
 
@@ -52,13 +65,13 @@ GH_HD inline double poly_cos(double r)
 {
     const double z = r * r;
     double p = -1.0 / 6402373705728000.0;           // -1/18!
-    p = fma(p, z, 1.0 / 20922789888000.0);          // 1/16!
-    p = fma(p, z, -1.0 / 87178291200.0);            // -1/14!
-    p = fma(p, z, 1.0 / 479001600.0);               // 1/12!
-    p = fma(p, z, -1.0 / 3628800.0);                // -1/10!
-    p = fma(p, z, 1.0 / 40320.0);                   // 1/8!
-    p = fma(p, z, -1.0 / 720.0);                    // -1/6!
-    p = fma(p, z, 1.0 / 24.0);                      // 1/4!
+    p = fma(p, z, kc(1.0 / 20922789888000.0));          // 1/16!
+    p = fma(p, z, kc(-1.0 / 87178291200.0));            // -1/14!
+    p = fma(p, z, kc(1.0 / 479001600.0));               // 1/12!
+    p = fma(p, z, kc(-1.0 / 3628800.0));                // -1/10!
+    p = fma(p, z, kc(1.0 / 40320.0));                   // 1/8!
+    p = fma(p, z, kc(-1.0 / 720.0));                    // -1/6!
+    p = fma(p, z, kc(1.0 / 24.0));                      // 1/4!
     const double h = 0.5 * z;
     const double w = 1.0 - h;
     // (1 - w) - h recovers the rounding of 1 - r^2/2
@@ -364,17 +377,17 @@ GH_HD inline double sinc_ratio(double x)
         return s / x;
     }
     double p = 1.0 / 15511210043330985984000000.0;  // 1/25!
-    p = fma(p, x2, -1.0 / 25852016738884976640000.0); // -1/23!
-    p = fma(p, x2, 1.0 / 51090942171709440000.0);     // 1/21!
-    p = fma(p, x2, -1.0 / 121645100408832000.0);      // -1/19!
-    p = fma(p, x2, 1.0 / 355687428096000.0);          // 1/17!
-    p = fma(p, x2, -1.0 / 1307674368000.0);           // -1/15!
-    p = fma(p, x2, 1.0 / 6227020800.0);               // 1/13!
-    p = fma(p, x2, -1.0 / 39916800.0);                // -1/11!
-    p = fma(p, x2, 1.0 / 362880.0);                   // 1/9!
-    p = fma(p, x2, -1.0 / 5040.0);                    // -1/7!
-    p = fma(p, x2, 1.0 / 120.0);                      // 1/5!
-    p = fma(p, x2, -1.0 / 6.0);                       // -1/3!
+    p = fma(p, x2, kc(-1.0 / 25852016738884976640000.0)); // -1/23!
+    p = fma(p, x2, kc(1.0 / 51090942171709440000.0));     // 1/21!
+    p = fma(p, x2, kc(-1.0 / 121645100408832000.0));      // -1/19!
+    p = fma(p, x2, kc(1.0 / 355687428096000.0));          // 1/17!
+    p = fma(p, x2, kc(-1.0 / 1307674368000.0));           // -1/15!
+    p = fma(p, x2, kc(1.0 / 6227020800.0));               // 1/13!
+    p = fma(p, x2, kc(-1.0 / 39916800.0));                // -1/11!
+    p = fma(p, x2, kc(1.0 / 362880.0));                   // 1/9!
+    p = fma(p, x2, kc(-1.0 / 5040.0));                    // -1/7!
+    p = fma(p, x2, kc(1.0 / 120.0));                      // 1/5!
+    p = fma(p, x2, kc(-1.0 / 6.0));                       // -1/3!
     return fma(p, x2, 1.0);
 }
 
