@@ -197,15 +197,12 @@ __device__ __forceinline__ void eval_item(int w, double2* lds, const int* __rest
                                           BatchDev B, const int* __restrict__ p_imin,
                                           double* __restrict__ logl_out, double* __restrict__ snr_out)
 {
-    using S = FdShape<N, NTHR>;
     {
         const int e = list[w];
         const int c = item_chain(it, e), s = item_src(it, e);
         const int g = B.c_group[c];
         const int W = B.g_W[g];
         const UcbConst& k = ucb_load_shared(B.pc + e);
-        double2 dl[S::PER][6];
-        fd_link_spectra<N, NTHR>(k, lds, opaque(B.pos), kPosN / N, opaque(B.tw), kTwN / N, dl);
 
         const int so = c * B.maxsrc + s;
         const int imin_n = p_imin[e];
@@ -216,33 +213,30 @@ __device__ __forceinline__ void eval_item(int w, double2* lds, const int* __rest
         const double* iC = B.invC + (size_t)B.g_off[g] * 6;
         double acc = 0.0, s2 = 0.0;
         // new-template bins (and the old template where they overlap)
-#pragma unroll
-        for (int kk = 0; kk < S::PER; kk++)
-        {
-            const int m = threadIdx.x + kk * NTHR;
-            if (m >= N) continue;
-            double2 v[5];
-            fd_tdi<N>(k, m, B.fmt, dl[kk], v);
-            double wq[6];
-            load_w(iC, W, m, wq);  // snr: template bin m against window bin m
-            s2 += bform(v, v, wq);
-            const int j = imin_n + m;
-            if (j >= 0 && j < W)
-            {
-                double2 d[3], r[3];
-                const int mo = j - imin_o;
-                const bool inold = mo >= 0 && mo < bw_o;
-#pragma unroll
-                for (int ch = 0; ch < 3; ch++)
+        fd_link_bins<N, NTHR, fd_split(N, NTHR)>(
+            k, lds, opaque(B.pos), kPosN / N, opaque(B.tw), kTwN / N, [&](int m, const double2* dl) {
+                double2 v[5];
+                fd_tdi<N>(k, m, B.fmt, dl, v);
+                double wq[6];
+                load_w(iC, W, m, wq);  // snr: template bin m against window bin m
+                s2 += bform(v, v, wq);
+                const int j = imin_n + m;
+                if (j >= 0 && j < W)
                 {
-                    double2 h0 = inold ? ho[ch * cap + mo] : make_double2(0.0, 0.0);
-                    d[ch] = make_double2(v[ch].x - h0.x, v[ch].y - h0.y);
-                    r[ch] = R[(size_t)ch * W + j];
+                    double2 d[3], r[3];
+                    const int mo = j - imin_o;
+                    const bool inold = mo >= 0 && mo < bw_o;
+#pragma unroll
+                    for (int ch = 0; ch < 3; ch++)
+                    {
+                        double2 h0 = inold ? ho[ch * cap + mo] : make_double2(0.0, 0.0);
+                        d[ch] = make_double2(v[ch].x - h0.x, v[ch].y - h0.y);
+                        r[ch] = R[(size_t)ch * W + j];
+                    }
+                    load_w(iC, W, j, wq);
+                    acc += bform(r, d, wq) - 0.5 * bform(d, d, wq);
                 }
-                load_w(iC, W, j, wq);
-                acc += bform(r, d, wq) - 0.5 * bform(d, d, wq);
-            }
-        }
+            });
         // old-template bins outside the new support: dh = -h_old
         for (int mo = threadIdx.x; mo < bw_o; mo += NTHR)
         {
@@ -281,7 +275,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(MINW, 8)))
                                                     Items it, BatchDev B, const int* __restrict__ p_imin,
                                                     double* __restrict__ logl_out, double* __restrict__ snr_out)
 {
-    __shared__ double2 lds[FdShape<N, NTHR>::LDS];
+    __shared__ double2 lds[FdShape<N, NTHR, fd_split(N, NTHR)>::LDS];
     const int cnt = *count;
     if constexpr (LOOP)
     {
@@ -315,27 +309,21 @@ template <int N, int NTHR>
 __global__ __launch_bounds__(NTHR) void tmpl_kernel(const int* __restrict__ list, const int* __restrict__ count,
                                                     Items it, BatchDev B)
 {
-    using S = FdShape<N, NTHR>;
-    __shared__ double2 lds[S::LDS];
+    __shared__ double2 lds[FdShape<N, NTHR, fd_split(N, NTHR)>::LDS];
     const int cnt = *count;
     GH_ITEM_LOOP
     {
         const int e = list[w];
         const UcbConst& k = ucb_load_shared(B.pc + e);
-        double2 dl[S::PER][6];
-        fd_link_spectra<N, NTHR>(k, lds, opaque(B.pos), kPosN / N, opaque(B.tw), kTwN / N, dl);
         const int c = item_chain(it, e), cap = chain_cap(B, c);
         double2* o = src_tmpl(B, c, item_src(it, e));
+        fd_link_bins<N, NTHR, fd_split(N, NTHR)>(k, lds, opaque(B.pos), kPosN / N, opaque(B.tw), kTwN / N,
+                                                 [&](int m, const double2* dl) {
+                                                     double2 v[5];
+                                                     fd_tdi<N>(k, m, B.fmt, dl, v);
 #pragma unroll
-        for (int kk = 0; kk < S::PER; kk++)
-        {
-            const int m = threadIdx.x + kk * NTHR;
-            if (m >= N) continue;
-            double2 v[5];
-            fd_tdi<N>(k, m, B.fmt, dl[kk], v);
-#pragma unroll
-            for (int ch = 0; ch < 3; ch++) o[ch * cap + m] = v[ch];
-        }
+                                                     for (int ch = 0; ch < 3; ch++) o[ch * cap + m] = v[ch];
+                                                 });
     }
 }
 

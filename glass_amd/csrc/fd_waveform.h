@@ -7,6 +7,12 @@
 //  * N <= 1024: NTHR = max(N, 64) threads, one slow sample each, written
 //    straight into LDS (6 links x N complex = 96 N bytes <= 96 KB), six
 //    FFTs, then every thread reads its bin's six link values.
+//  * N <= 1024 with NTHR = N/2 (fd_link_bins): thread t folds samples t
+//    and t + N/2 by one radix-2 decimation-in-frequency step; the even-bin
+//    series goes to LDS, the odd-bin series waits in registers, and the two
+//    half-length transforms run in turn (48 N bytes of LDS).  Bins are
+//    handed out per pass, one per thread (m = 2t, then 2t + 1), so only
+//    one bin's six link values are live at a time.
 //  * N = P*NS (N >= 2048, NS = 1024): decimation in time.  The samples
 //    n = P*j + p of each phase p form an NS-point series; its six FFTs run
 //    in LDS in turn and each thread folds X_{j+NS r} += w_N^{p j} w_P^{p r}
@@ -33,15 +39,22 @@ namespace glass {
 // 1024 threads for the decimated N >= 2048 path.
 constexpr int fd_threads(int N) { return N < 64 ? 64 : (N <= 512 ? N : (N == 1024 ? GH_NTHR_1024 : GH_NTHR_DECIM)); }
 
-template <int N, int NTHR>
+// SPLIT (fd_link_bins only, N <= 1024 with NTHR = N/2): one radix-2
+// decimation-in-frequency step in registers, then the even- and odd-bin
+// half-length transforms in LDS one after the other (half the LDS).
+template <int N, int NTHR, bool SPLIT = false>
 struct FdShape
 {
+    static_assert(!SPLIT || (N <= 1024 && N >= 8 && 2 * NTHR == N), "split shape: N <= 1024, N/2 threads");
     static constexpr int PER = (N + NTHR - 1) / NTHR;  // bins per thread
-    // FFT length in LDS: the whole template up to 1024 bins, else the
-    // sub-series length of the decimated path
-    static constexpr int NS = N <= 1024 ? N : GH_DECIM_NS;
+    // FFT length in LDS: the whole template up to 1024 bins (half of it
+    // when split), else the sub-series length of the decimated path
+    static constexpr int NS = SPLIT ? N / 2 : (N <= 1024 ? N : GH_DECIM_NS);
     static constexpr int LDS = 6 * NS;                  // double2 elements
 };
+
+// Split launch shapes: two bins per thread up to 1024 bins.
+constexpr bool fd_split(int N, int NTHR) { return N <= 1024 && N >= 8 && 2 * NTHR == N; }
 
 // Move a value that every lane holds identically into a scalar register.
 __device__ inline double uniform(double x)
@@ -189,6 +202,87 @@ __device__ inline void fd_link_spectra(const UcbConst& c, double2* lds, const do
 #pragma unroll
                 for (int l = 0; l < 6; l++) d[k][l] = make_double2(X[u][r][l].x * inv2N, X[u][r][l].y * inv2N);
             }
+    }
+}
+
+// f(m, d) for every output bin m of this thread, d[l] = link l at bin m
+// as fd_link_spectra leaves it.  Ends with a barrier.
+template <int N, int NTHR, bool SPLIT, typename F>
+__device__ inline void fd_link_bins(const UcbConst& c, double2* lds, const double* __restrict__ pos, int posstride,
+                                    const double2* __restrict__ tw, int twstride, F&& f)
+{
+    if constexpr (!SPLIT)
+    {
+        using S = FdShape<N, NTHR>;
+        double2 dl[S::PER][6];
+        fd_link_spectra<N, NTHR>(c, lds, pos, posstride, tw, twstride, dl);
+#pragma unroll
+        for (int k = 0; k < S::PER; k++)
+        {
+            const int m = threadIdx.x + k * NTHR;
+            if (m < N) f(m, dl[k]);
+        }
+    }
+    else
+    {
+        // X[2k] = FFT_H(x[n] + x[n+H]), X[2k+1] = FFT_H((x[n] - x[n+H]) w_N^n),
+        // H = N/2 = NTHR
+        constexpr int H = N / 2;
+        const double inv2N = 1. / (double)(2 * N);
+        const int t = threadIdx.x;
+        ucb_slow_sample(c, ucb_sample_time(c.T, c.t0, t, N), pos + (size_t)t * posstride * 9, lds, H, t);
+        // links 0-2 and 3-5 of the odd-bin series, shifted in as the
+        // spacecraft loop produces them (l < 3 is known per unrolled link)
+        const double2 w = tw[t * twstride];
+        double2 ya[3], yb[3];
+        ucb_slow_sample_to<false>(c, ucb_sample_time(c.T, c.t0, t + H, N), pos + (size_t)(t + H) * posstride * 9,
+                                  [&](int l, double2 x1) {
+                                      const double2 x0 = lds[l * H + t];
+                                      lds[l * H + t] = make_double2(x0.x + x1.x, x0.y + x1.y);
+                                      const double2 dv = cmul(make_double2(x0.x - x1.x, x0.y - x1.y), w);
+                                      if (l < 3)
+                                      {
+                                          ya[0] = ya[1];
+                                          ya[1] = ya[2];
+                                          ya[2] = dv;
+                                      }
+                                      else
+                                      {
+                                          yb[0] = yb[1];
+                                          yb[1] = yb[2];
+                                          yb[2] = dv;
+                                      }
+                                  });
+        __syncthreads();
+#pragma unroll
+        for (int pass = 0; pass < 2; pass++)
+        {
+            lds_fft<H, 6, NTHR>(lds, tw, 2 * twstride);
+            // bin m = 2t + pass is FFT index (m + H) mod N, of parity pass
+            const int m = 2 * t + pass;
+            const int s = ((m + H) & (N - 1)) >> 1;
+            double2 d[6];
+#pragma unroll
+            for (int l = 0; l < 6; l++)
+            {
+                const double2 v = lds[l * H + s];
+                d[l] = make_double2(v.x * inv2N, v.y * inv2N);
+            }
+            __syncthreads();
+            if (!pass)
+            {
+                // the odd-bin series takes the LDS before the even bins'
+                // work, so it is not held in registers across that work
+#pragma unroll
+                for (int l = 0; l < 3; l++)
+                {
+                    lds[l * H + t] = ya[l];
+                    lds[(l + 3) * H + t] = yb[l];
+                }
+            }
+            f(m, d);
+            if (!pass) __syncthreads();
+        }
     }
 }
 

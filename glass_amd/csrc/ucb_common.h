@@ -104,14 +104,15 @@ __device__ inline double quad3(const double* E, double a, double b, double cc)
     return E[0] * a * a + E[1] * b * b + E[2] * cc * cc + 2.0 * (E[3] * a * b + E[4] * a * cc + E[5] * b * cc);
 }
 
-// One slow-series sample of the six links at time t, written to
-// lds[l*stride + j] for link l in the order 12, 21, 31, 13, 23, 32.
+// One slow-series sample of the six links at time t: out(l, value) receives
+// link l in the order 12, 21, 31, 13, 23, 32.
 // pos: spacecraft positions x1,x2,x3,y1,y2,y3,z1,z2,z3 at t (metres).
 // glass_ucb_waveform.c:843-936, LISA_detector_tensor glass_lisa.c:1521-1564.
 // The loop runs over the sending spacecraft i; its two links (i,j) share
-// xi_i, arg2_i and the frequency at spacecraft i.
-__device__ inline void ucb_slow_sample(const UcbConst& c, double t, const double* __restrict__ pos,
-                                       double2* lds, int stride, int j)
+// xi_i, arg2_i and the frequency at spacecraft i.  UNROLL unrolls it so that
+// l is a compile-time index (values may then stay in registers).
+template <bool UNROLL, typename Out>
+__device__ inline void ucb_slow_sample_to(const UcbConst& c, double t, const double* __restrict__ pos, Out&& out)
 {
     const double x1 = pos[0], x2 = pos[1], x3 = pos[2];
     const double y1 = pos[3], y2 = pos[4], y3 = pos[5];
@@ -132,7 +133,7 @@ __device__ inline void ucb_slow_sample(const UcbConst& c, double t, const double
 #pragma clang fp contract(off)
         dft = c.df * t;
     }
-#pragma unroll 1
+#pragma unroll(UNROLL ? 3 : 1)
     for (int i = 0; i < 3; i++)
     {
         const double px = i == 0 ? x1 : (i == 1 ? x2 : x3);
@@ -176,9 +177,17 @@ __device__ inline void ucb_slow_sample(const UcbConst& c, double t, const double
             const double t1i = ae * (dp * c.DPi + dc * c.DCi);
             double s2, c2;
             sincos_red(arg1 - a2, &s2, &c2);
-            lds[(i + 3 * h) * stride + j] = make_double2(sinc * (t1r * c2 + t1i * s2), sinc * (-t1r * s2 + t1i * c2));
+            out(i + 3 * h, make_double2(sinc * (t1r * c2 + t1i * s2), sinc * (-t1r * s2 + t1i * c2)));
         }
+        if constexpr (UNROLL) __builtin_amdgcn_sched_barrier(0);
     }
+}
+
+// The sample written to lds[l*stride + j].
+__device__ inline void ucb_slow_sample(const UcbConst& c, double t, const double* __restrict__ pos,
+                                       double2* lds, int stride, int j)
+{
+    ucb_slow_sample_to<false>(c, t, pos, [&](int l, double2 v) { lds[l * stride + j] = v; });
 }
 
 // Time of slow sample n (0-based) of a BW-sample series:
