@@ -36,8 +36,8 @@ FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per launch of the batched per-class kernels, measured with rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE on this workload and scaled by the same counters on a 256 MiB device copy (fetch x2.00,
 # write x1.00): profiles/eval_kernel_batched_fd_sweep.md.  Keyed by template bins N.
-MEASURED_TRAFFIC = {512: 2.3552e9, 1024: 1.8288e9, 2048: 2.3939e9, 256: 8.3085e8, 4096: 7.2205e8,
-                    128: 2.3213e8, 64: 7.1404e7, 32: 1.8041e6}
+MEASURED_TRAFFIC = {512: 2.3560e9, 1024: 1.8243e9, 2048: 1.5608e9, 256: 8.3319e8, 4096: 3.0456e8,
+                    128: 2.3316e8, 64: 7.1631e7, 32: 1.8182e6}
 
 # HBM bytes per launch of logl_chunk_kernel on the inner-product leg (rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE, same calibration): profiles/logl_chunk_kernel_batched_inner_product.md.
@@ -702,6 +702,8 @@ def main():
     ap.add_argument("--no-sampler", action="store_true")
     ap.add_argument("--no-fstat", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default tiles (skip the autotuner)")
+    ap.add_argument("--serial-classes", action="store_true",
+                    help="run the BW-class kernels of each sweep in turn on one stream (profiling runs)")
     ap.add_argument("--globalfit", action="store_true",
                     help="also run the global-fit block at full scale (1,024,000 chains, ~170 GB of HBM)")
     ap.add_argument("--copy-calibrate", action="store_true",
@@ -725,6 +727,8 @@ def main():
     plan = FdBatch(o, "sangria", cfg["T"], 0.0, cfg["group_qmin"], cfg["data"], cfg["invC"],
                    cfg["chain_group"], cfg["chain_nsrc"], cfg["src_params"], E)
     plan.set_proposals(cfg["p_chain"], cfg["p_src"], cfg["p_params"])
+    if args.serial_classes:
+        plan.set_concurrent(False)
     torch.cuda.synchronize()
     log(f"inputs + plan: {time.perf_counter() - t0:.1f} s, {E} evaluations per step")
 
@@ -752,8 +756,8 @@ def main():
     torch.cuda.synchronize()
     graph_ms = g0.elapsed_time(g1) / max(1, min(args.steps, 5))
 
-    # timed region: K sweeps with per-kernel HIP events on the launch stream
-    plan.profile(True)
+    # timed region: K eager sweeps as a caller issues them (BW classes on
+    # concurrent streams unless --serial-classes)
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
@@ -764,6 +768,12 @@ def main():
     total_ms = e0.elapsed_time(e1)
     ms_step = total_ms / args.steps
     value = E * args.steps / (total_ms * 1e-3)
+    # per-class kernel times (the roofline's launch duration): the same K
+    # sweeps again with HIP events around each class kernel on the launch
+    # stream, the classes in turn
+    plan.profile(True)
+    for _ in range(args.steps):
+        plan.eval(E)
     kms, klaunch, kcount = plan.profile_read()
     plan.profile(False)
     logL = plan.logL[:E].cpu().numpy()
@@ -982,7 +992,7 @@ def main():
                                "per chain, %d proposals per chain, Tobs 2 yr, windows 512-4096 bins" %
                                (args.groups, args.temps, args.props),
                    "evals_per_step": E, "chains": len(cfg["chain_group"]), "format": "sangria",
-                   "autotune": tune,
+                   "autotune": tune, "concurrent_classes": not args.serial_classes,
                    "graph_replay_ms_per_step": round(graph_ms, 4)},
         "plain": None if plain is None else {"value": round(plain, 1), "unit": "evals/s",
                                              "method": "per-call ucb_waveform + gaussian_log_likelihood "

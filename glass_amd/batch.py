@@ -46,6 +46,7 @@ _EXTRA += [
     ("glass_hip_fd_batch_refresh", [_vp, _vp], _i),
     ("glass_hip_fd_batch_state", [_vp, _dp, _dp], _i),
     ("glass_hip_fd_batch_profile", [_vp, _i], _i),
+    ("glass_hip_fd_batch_set_concurrent", [_vp, _i], _i),
     ("glass_hip_fd_batch_profile_read", [_vp, _dp, _ip, _ip], _i),
     ("glass_hip_ucb_bandwidth", [_d] * 7 + [_i], _i),
     ("glass_hip_ucb_alignment", [_vp, _d, _i, _i, _dp, _ip, _ip, _ip], _i),
@@ -192,6 +193,10 @@ class FdBatch:
     def profile(self, on=True):
         """Per-BW-class kernel timing of eager evals (makes them synchronous)."""
         check(_L().glass_hip_fd_batch_profile(self.h, int(bool(on))), "fd_batch_profile")
+
+    def set_concurrent(self, on=True):
+        """BW-class kernels on forked side streams (default) or in turn."""
+        check(_L().glass_hip_fd_batch_set_concurrent(self.h, int(bool(on))), "fd_batch_set_concurrent")
 
     def profile_read(self):
         """-> (ms[8], launches[8], count[8]) per BW class (BW = 32 << k)."""

@@ -290,18 +290,19 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(MINW, 8)))
 
 // Compiled launch shapes of the class kernels (tile candidates), default
 // first: (class k, template bins N, workgroup threads, waves per SIMD).
-// Workgroups of N threads hold one slow sample per thread; fewer threads
-// take several samples and FFT butterflies each (more registers, fewer
-// waves).  Up to 512 bins the default allocation (~98 VGPRs) leaves 4 waves
-// per SIMD while LDS would hold more workgroups: the 5- and 6-wave variants
-// spill some registers to keep more templates in flight across the FFT
-// barriers.  N = 1024 is LDS-bound (one workgroup per CU); the decimated
-// classes (N >= 2048) keep their accumulators in registers only at 512
-// (and 1024 for N = 2048) threads.
+// From 128 to 1024 bins the default is the split shape (N/2 threads, two
+// bins each, half the LDS of one sample per thread): at 126 VGPRs it keeps
+// 4 waves per SIMD, so BW 1024 fits two workgroups per CU instead of one and
+// BW 512 four 256-thread workgroups instead of three.  The N-thread shapes
+// (one sample per thread, ~94 VGPRs) and their 5- and 6-wave variants
+// (spilling some registers for more waves in flight across the FFT
+// barriers) remain candidates.  The decimated classes (N >= 2048) keep
+// their accumulators in registers only at 512 (and 1024 for N = 2048)
+// threads.
 #define GH_EVAL_TILES(X)                                                                                      \
-    X(0, 32, 64, 1) X(1, 64, 64, 1) X(1, 64, 64, 6) X(2, 128, 128, 1) X(2, 128, 128, 6) X(2, 128, 64, 1)     \
-    X(3, 256, 256, 1) X(3, 256, 256, 5) X(3, 256, 256, 6) X(3, 256, 128, 1) X(4, 512, 512, 1)                \
-    X(4, 512, 512, 6) X(4, 512, 256, 1) X(5, 1024, 512, 1) X(5, 1024, 1024, 1) X(5, 1024, 256, 1)            \
+    X(0, 32, 64, 1) X(1, 64, 64, 1) X(1, 64, 64, 6) X(2, 128, 64, 1) X(2, 128, 128, 1) X(2, 128, 128, 6)     \
+    X(3, 256, 128, 1) X(3, 256, 256, 1) X(3, 256, 256, 5) X(3, 256, 256, 6) X(4, 512, 256, 1)                \
+    X(4, 512, 512, 1) X(4, 512, 512, 6) X(5, 1024, 512, 1) X(5, 1024, 1024, 1) X(5, 1024, 256, 1)            \
     X(6, 2048, 512, 1) X(6, 2048, 1024, 1) X(7, 4096, 512, 1)
 
 // Template of each listed item into its source's slot.
@@ -519,6 +520,12 @@ struct glass_hip_fd_batch
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int g_nprop = -1;
+    // class kernels run concurrently: classes k with k % 4 = 1, 2, 3 go to
+    // side streams forked from and joined back into the caller's stream
+    // (so graph capture records them as parallel branches)
+    hipStream_t side[3] = {};
+    hipEvent_t evf = nullptr, evj[3] = {};
+    bool concurrent = true;
     // profiling (eager evals only): events around each class kernel
     bool prof = false;
     hipEvent_t ev[kNClass][2] = {};
@@ -543,7 +550,7 @@ static int dalloc(glass_hip_fd_batch* P, T** p, size_t n)
     return GLASS_HIP_OK;
 }
 
-#define GH_NTHR_(N) (::glass::fd_threads(N))
+#define GH_NTHR_(N) (::glass::fd_batch_threads(N))
 #define GH_NTHR(N) GH_NTHR_(N)
 
 // Workgroups of a class kernel resident per CU, as bounded by LDS (160 KB)
@@ -551,7 +558,7 @@ static int dalloc(glass_hip_fd_batch* P, T** p, size_t n)
 // `waves` when forced higher).
 static int resident_per_cu(int N, int nthr, int waves)
 {
-    const int ns = N <= 1024 ? N : GH_DECIM_NS;
+    const int ns = fd_split(N, nthr) ? N / 2 : (N <= 1024 ? N : GH_DECIM_NS);
     const int lds = (int)sizeof(double2) * 6 * ns + 512;
     int per = (160 * 1024) / lds;
     const int wv = 4 * (waves > 4 ? waves : 4) / ((nthr + 63) / 64);
@@ -565,8 +572,8 @@ static int setup_grids(glass_hip_fd_batch* P)
     for (int k = 0; k < kNClass; k++)
     {
         const int N = 32 << k;
-        P->tgrid[k] = P->ncu * resident_per_cu(N, fd_threads(N), 1);
-        P->tile[k] = glass_hip_tile{fd_threads(N), 0, 1};
+        P->tgrid[k] = P->ncu * resident_per_cu(N, fd_batch_threads(N), 1);
+        P->tile[k] = glass_hip_tile{fd_batch_threads(N), 0, 1};
     }
     return GLASS_HIP_OK;
 }
@@ -625,12 +632,21 @@ static int launch_items(glass_hip_fd_batch* P, int n, const Items& it, int mode,
     zero_int_kernel<<<1, 64, 0, s>>>(P->cls, kNClass + 1);
     align_kernel<<<(n + 255) / 256, 256, 0, s>>>(n, it, P->B, P->p_bw, P->p_imin, P->cls, P->cls_list, P->cls_cap,
                                                  P->cls + kNClass);
+    // the per-class timings of the profiling mode need the classes in turn
+    const bool par = P->concurrent && !(P->prof && mode == 0);
+    if (par)
+    {
+        GH_CHECK(hipEventRecord(P->evf, s));
+        for (int j = 0; j < 3; j++) GH_CHECK(hipStreamWaitEvent(P->side[j], P->evf, 0));
+    }
+    int lrc = GLASS_HIP_OK;
     for (int k = kNClass - 1; k >= 0; k--)
     {
         const int* list = P->cls_list + (size_t)k * P->cls_cap;
         const int* cnt = P->cls + k;
         const bool pr = P->prof && mode == 0;
-        if (pr) GH_CHECK(hipEventRecord(P->ev[k][0], s));
+        const hipStream_t sk = par && (k & 3) ? P->side[(k & 3) - 1] : s;
+        if (pr) GH_CHECK(hipEventRecord(P->ev[k][0], sk));
         if ((32 << k) > P->B.bwcap)
         {
             // align_kernel routes BW above every window's cap to the error
@@ -639,7 +655,7 @@ static int launch_items(glass_hip_fd_batch* P, int n, const Items& it, int mode,
         }
         else if (mode == 0)
         {
-            if (int rc = launch_eval(P, k, P->tile[k], n, list, cnt, it, logl_out, snr_out, s)) return rc;
+            if ((lrc = launch_eval(P, k, P->tile[k], n, list, cnt, it, logl_out, snr_out, sk))) break;
         }
         else
         {
@@ -647,7 +663,7 @@ static int launch_items(glass_hip_fd_batch* P, int n, const Items& it, int mode,
             if (g > n) g = n;
 #define GH_L(kk, N)                                                                               \
     case kk:                                                                                      \
-        tmpl_kernel<N, GH_NTHR(N)><<<g, GH_NTHR(N), 0, s>>>(list, cnt, it, P->B);                \
+        tmpl_kernel<N, GH_NTHR(N)><<<g, GH_NTHR(N), 0, sk>>>(list, cnt, it, P->B);                \
         break;
             switch (k)
             {
@@ -656,8 +672,16 @@ static int launch_items(glass_hip_fd_batch* P, int n, const Items& it, int mode,
             }
 #undef GH_L
         }
-        if (pr) GH_CHECK(hipEventRecord(P->ev[k][1], s));
+        if (pr) GH_CHECK(hipEventRecord(P->ev[k][1], sk));
     }
+    // joined also after a failed launch, so a capture is never left open
+    if (par)
+        for (int j = 0; j < 3; j++)
+        {
+            GH_CHECK(hipEventRecord(P->evj[j], P->side[j]));
+            GH_CHECK(hipStreamWaitEvent(s, P->evj[j], 0));
+        }
+    if (lrc) return lrc;
     GH_CHECK(hipGetLastError());
     if (P->prof && mode == 0)
     {
@@ -893,6 +917,18 @@ extern "C" int glass_hip_fd_batch_create_ex(glass_hip_fd_batch** plan, const gla
     B.tw = tw;
     B.fmt = fmt;
     if ((rc = setup_grids(P))) return fail(rc);
+    for (int j = 0; j < 3; j++)
+        if (hipStreamCreateWithFlags(&P->side[j], hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&P->evj[j], hipEventDisableTiming) != hipSuccess)
+        {
+            set_error("fd_batch_create: side streams");
+            return fail(GLASS_HIP_ERUNTIME);
+        }
+    if (hipEventCreateWithFlags(&P->evf, hipEventDisableTiming) != hipSuccess)
+    {
+        set_error("fd_batch_create: fork event");
+        return fail(GLASS_HIP_ERUNTIME);
+    }
     if (P->B.bwcap >= 2048 && (rc = dalloc(P, &P->d2048, (size_t)P->ncu * kDeriv2048Scratch))) return fail(rc);
     if ((rc = rebuild(P, cx->stream)) || (rc = check_err(P, cx->stream))) return fail(rc);
     *plan = P;
@@ -911,6 +947,12 @@ extern "C" int glass_hip_fd_batch_destroy(glass_hip_fd_batch* P)
     if (P->sgexec) (void)hipGraphExecDestroy(P->sgexec);
     if (P->sgraph) (void)hipGraphDestroy(P->sgraph);
     (void)hipDeviceSynchronize();
+    for (int j = 0; j < 3; j++)
+    {
+        if (P->side[j]) (void)hipStreamDestroy(P->side[j]);
+        if (P->evj[j]) (void)hipEventDestroy(P->evj[j]);
+    }
+    if (P->evf) (void)hipEventDestroy(P->evf);
     for (void* p : P->allocs) (void)hipFree(p);
     delete P;
     return GLASS_HIP_OK;
@@ -1064,6 +1106,18 @@ extern "C" int glass_hip_fd_batch_refresh(glass_hip_fd_batch* P, void* stream)
     }
     GH_CHECK(hipSetDevice(P->device));
     return rebuild(P, (hipStream_t)stream);
+}
+
+extern "C" int glass_hip_fd_batch_set_concurrent(glass_hip_fd_batch* P, int on)
+{
+    if (!P)
+    {
+        set_error("fd_batch_set_concurrent: null plan");
+        return GLASS_HIP_EINVAL;
+    }
+    if (int rc = drop_graph(P)) return rc;
+    P->concurrent = on != 0;
+    return GLASS_HIP_OK;
 }
 
 extern "C" int glass_hip_fd_batch_profile(glass_hip_fd_batch* P, int on)

@@ -39,6 +39,10 @@ namespace glass {
 // 1024 threads for the decimated N >= 2048 path.
 constexpr int fd_threads(int N) { return N < 64 ? 64 : (N <= 512 ? N : (N == 1024 ? GH_NTHR_1024 : GH_NTHR_DECIM)); }
 
+// Default workgroup size of the batched template kernels (fd_link_bins):
+// the split shape, N/2 threads with two bins each, from 128 to 1024 bins.
+constexpr int fd_batch_threads(int N) { return N <= 64 ? 64 : (N <= 1024 ? N / 2 : GH_NTHR_DECIM); }
+
 // SPLIT (fd_link_bins only, N <= 1024 with NTHR = N/2): one radix-2
 // decimation-in-frequency step in registers, then the even- and odd-bin
 // half-length transforms in LDS one after the other (half the LDS).

@@ -118,6 +118,15 @@ int glass_hip_fd_batch_profile(glass_hip_fd_batch *plan, int on);
 int glass_hip_fd_batch_profile_read(glass_hip_fd_batch *plan, double *ms, int *launches,
                                     int *count);
 
+/* Concurrent BW classes (no reference counterpart; on by default): with
+ * on != 0 the class kernels of an eval, an accept or a template rebuild
+ * run on three side streams of the plan forked from and joined back into
+ * the launch stream (parallel branches under graph capture), so small and
+ * latency-bound classes overlap; with on = 0 they run in turn on the
+ * launch stream.  Results are identical either way.  Drops the captured
+ * graph. */
+int glass_hip_fd_batch_set_concurrent(glass_hip_fd_batch *plan, int on);
+
 /* Launch shape ("tile") of one BW class's fused eval kernel (no reference
  * counterpart): nthr threads per workgroup (one template per workgroup;
  * fewer threads than template bins take several bins each), the grid
