@@ -43,13 +43,13 @@ constexpr int fd_threads(int N) { return N < 64 ? 64 : (N <= 512 ? N : (N == 102
 // the split shape, N/2 threads with two bins each, from 128 to 1024 bins.
 constexpr int fd_batch_threads(int N) { return N <= 64 ? 64 : (N <= 1024 ? N / 2 : GH_NTHR_DECIM); }
 
-// SPLIT (fd_link_bins only, N <= 1024 with NTHR = N/2): one radix-2
+// SPLIT (fd_link_bins only, N <= 2048 with NTHR = N/2): one radix-2
 // decimation-in-frequency step in registers, then the even- and odd-bin
 // half-length transforms in LDS one after the other (half the LDS).
 template <int N, int NTHR, bool SPLIT = false>
 struct FdShape
 {
-    static_assert(!SPLIT || (N <= 1024 && N >= 8 && 2 * NTHR == N), "split shape: N <= 1024, N/2 threads");
+    static_assert(!SPLIT || (N <= 2048 && N >= 8 && 2 * NTHR == N), "split shape: N <= 2048, N/2 threads");
     static constexpr int PER = (N + NTHR - 1) / NTHR;  // bins per thread
     // FFT length in LDS: the whole template up to 1024 bins (half of it
     // when split), else the sub-series length of the decimated path
@@ -57,8 +57,8 @@ struct FdShape
     static constexpr int LDS = 6 * NS;                  // double2 elements
 };
 
-// Split launch shapes: two bins per thread up to 1024 bins.
-constexpr bool fd_split(int N, int NTHR) { return N <= 1024 && N >= 8 && 2 * NTHR == N; }
+// Split launch shapes: two bins per thread up to 2048 bins.
+constexpr bool fd_split(int N, int NTHR) { return N <= 2048 && N >= 8 && 2 * NTHR == N; }
 
 // Move a value that every lane holds identically into a scalar register.
 __device__ inline double uniform(double x)
