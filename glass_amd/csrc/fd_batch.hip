@@ -85,6 +85,9 @@ struct Items
     // optional device-side item count (items e >= *dn are left out); lets
     // graph-captured callers size a pass on the device
     const int* dn = nullptr;
+    // optional template staging of an eval: item e's X, Y, Z bins go to
+    // stg[e * 3 bwcap + ch * N + m] (N its template bins)
+    double2* stg = nullptr;
 };
 
 __device__ inline int item_q(const Items& it, int e) { return it.sel ? it.sel[e] : e; }
@@ -217,6 +220,9 @@ __device__ __forceinline__ void eval_item(int w, double2* lds, const int* __rest
             k, lds, opaque(B.pos), kPosN / N, opaque(B.tw), kTwN / N, [&](int m, const double2* dl) {
                 double2 v[5];
                 fd_tdi<N>(k, m, B.fmt, dl, v);
+                if (it.stg)
+#pragma unroll
+                    for (int ch = 0; ch < 3; ch++) it.stg[(size_t)e * 3 * B.bwcap + ch * N + m] = v[ch];
                 double wq[6];
                 load_w(iC, W, m, wq);  // snr: template bin m against window bin m
                 s2 += bform(v, v, wq);
@@ -1048,6 +1054,56 @@ extern "C" int glass_hip_fd_batch_accept(glass_hip_fd_batch* P, int nacc, const 
     return GLASS_HIP_OK;
 }
 
+// Accept with templates staged by the eval (Items::stg of that eval): per
+// accepted item, residual += h_old over the old support, the staged
+// template into the source's slot, residual -= h_new over the new support
+// (per bin r + h_old - h_new, as the three-pass accept), new parameters.
+// bw_e / imin_e / logl_new are indexed by the eval's item (item_q).  At most
+// one accepted item per chain.  One workgroup per accepted item.
+__global__ __launch_bounds__(256) void accept_staged_kernel(int nacc, Items it, BatchDev B,
+                                                            const double2* __restrict__ stg,
+                                                            const int* __restrict__ bw_e,
+                                                            const int* __restrict__ imin_e,
+                                                            const double* __restrict__ logl_new)
+{
+    const int a = blockIdx.x;
+    if (a >= nacc || (it.dn && a >= *it.dn)) return;
+    const int q = item_q(it, a);
+    const int c = it.chain[q], s = it.src[q], so = c * B.maxsrc + s;
+    const int g = B.c_group[c], W = B.g_W[g], cap = B.g_cap[g];
+    const int io = B.s_imin[so], bo = B.s_bw[so];
+    const int in = imin_e[q], bn = bw_e[q];
+    double2* R = B.resid + (size_t)B.c_roff[c] * 3;
+    double2* h = src_tmpl(B, c, s);
+    const double2* hs = stg + (size_t)q * 3 * B.bwcap;
+    for (int j = max(0, io) + threadIdx.x; j < min(W, io + bo); j += blockDim.x)
+        for (int ch = 0; ch < 3; ch++)
+        {
+            double2 v = R[(size_t)ch * W + j];
+            v.x += h[ch * cap + j - io].x;
+            v.y += h[ch * cap + j - io].y;
+            R[(size_t)ch * W + j] = v;
+        }
+    __syncthreads();  // the old template is read before its slot is overwritten
+    for (int m = threadIdx.x; m < bn; m += blockDim.x)
+        for (int ch = 0; ch < 3; ch++) h[ch * cap + m] = hs[ch * bn + m];
+    for (int j = max(0, in) + threadIdx.x; j < min(W, in + bn); j += blockDim.x)
+        for (int ch = 0; ch < 3; ch++)
+        {
+            double2 v = R[(size_t)ch * W + j];
+            v.x -= hs[ch * bn + j - in].x;
+            v.y -= hs[ch * bn + j - in].y;
+            R[(size_t)ch * W + j] = v;
+        }
+    if (threadIdx.x < 8) B.sparams[(size_t)so * 8 + threadIdx.x] = it.params[(size_t)q * 8 + threadIdx.x];
+    if (threadIdx.x == 0)
+    {
+        B.s_bw[so] = bn;
+        B.s_imin[so] = in;
+        B.logl[c] = logl_new[q];
+    }
+}
+
 // ------------------------------------------- hooks for the device sampler ---
 
 namespace glass {
@@ -1072,25 +1128,28 @@ FdBatchView fd_batch_view(glass_hip_fd_batch* P)
 }
 
 int fd_batch_eval_dev(glass_hip_fd_batch* P, int n, const int* chain, const int* src, const double* params,
-                      double* logl, double* snr, hipStream_t s)
+                      double* logl, double* snr, hipStream_t s, double2* stage)
 {
     Items it{chain, src, params, nullptr};
+    it.stg = stage;
     return launch_items(P, n, it, 0, logl, snr, s);
 }
 
-int fd_batch_accept_dev(glass_hip_fd_batch* P, int nmax, const int* d_n, const int* d_idx, const int* chain,
-                        const int* src, const double* params, const double* logl, hipStream_t s)
+size_t fd_batch_stage_bytes(glass_hip_fd_batch* P, int n) { return sizeof(double2) * (size_t)n * 3 * P->B.bwcap; }
+
+int fd_batch_accept_staged_dev(glass_hip_fd_batch* P, int nmax, const int* d_n, const int* d_idx, const int* chain,
+                               const int* src, const double* params, const double* logl, const double2* stage,
+                               hipStream_t s)
 {
     if (nmax <= 0) return GLASS_HIP_OK;
-    if (nmax > P->C || nmax > P->cls_cap)
+    if (nmax > P->C || nmax > P->cls_cap || !stage)
     {
-        set_error("batch accept: %d items exceed the plan (%d chains, capacity %d)", nmax, P->C, P->cls_cap);
+        set_error("batch accept: %d items exceed the plan (%d chains, capacity %d) or no staged templates", nmax,
+                  P->C, P->cls_cap);
         return GLASS_HIP_EINVAL;
     }
     Items it{chain, src, params, d_idx, d_n};
-    accept_add_old_kernel<<<nmax, 256, 0, s>>>(nmax, it, P->B);
-    if (int rc = launch_items(P, nmax, it, 1, nullptr, nullptr, s)) return rc;
-    accept_sub_new_kernel<<<nmax, 256, 0, s>>>(nmax, it, P->B, P->p_bw, P->p_imin, logl);
+    accept_staged_kernel<<<nmax, 256, 0, s>>>(nmax, it, P->B, stage, P->p_bw, P->p_imin, logl);
     GH_CHECK(hipGetLastError());
     return GLASS_HIP_OK;
 }

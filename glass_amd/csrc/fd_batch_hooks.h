@@ -22,14 +22,20 @@ struct FdBatchView
 FdBatchView fd_batch_view(glass_hip_fd_batch* P);
 
 // eval of n items (chain, src, params[8]); items with src < 0 are skipped
-// and their outputs left as they were
+// and their outputs left as they were.  stage (optional, of
+// fd_batch_stage_bytes(P, n)): the items' templates are kept there for
+// fd_batch_accept_staged_dev.
 int fd_batch_eval_dev(glass_hip_fd_batch* P, int n, const int* chain, const int* src, const double* params,
-                      double* logl, double* snr, hipStream_t s);
+                      double* logl, double* snr, hipStream_t s, double2* stage = nullptr);
 
-// accept the items d_idx[0 .. *d_n) (entries of chain/src/params/logl),
-// at most nmax of them, one per chain
-int fd_batch_accept_dev(glass_hip_fd_batch* P, int nmax, const int* d_n, const int* d_idx, const int* chain,
-                        const int* src, const double* params, const double* logl, hipStream_t s);
+size_t fd_batch_stage_bytes(glass_hip_fd_batch* P, int n);
+
+// accept the items d_idx[0 .. *d_n) (entries of chain/src/params/logl of
+// an eval with stage), at most nmax of them, one per chain, taking their
+// templates from that stage (no template rebuild)
+int fd_batch_accept_staged_dev(glass_hip_fd_batch* P, int nmax, const int* d_n, const int* d_idx, const int* chain,
+                               const int* src, const double* params, const double* logl, const double2* stage,
+                               hipStream_t s);
 
 // eigen-decomposition of n symmetric 8x8 matrices (eigen.hip)
 int fisher_eigen_launch(int n, const double* F, double* evec, double* eval, int* sweeps, hipStream_t s);

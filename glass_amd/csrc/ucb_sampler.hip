@@ -469,6 +469,8 @@ struct glass_hip_ucb_sampler
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int g_steps = -1, g_adapt = -1;
+    // each step's proposal templates, kept by the eval for the accept
+    double2* stage = nullptr;
 };
 
 template <typename T>
@@ -589,7 +591,8 @@ extern "C" int glass_hip_ucb_sampler_create(glass_hip_ucb_sampler** out, glass_h
         (rc = salloc(S, &S->d_fs_dims, 3 * (size_t)v.G)) || (rc = salloc(S, &S->d_fs_par, 3 * (size_t)v.G)) ||
         (rc = salloc(S, &S->fL, M)) || (rc = salloc(S, &S->fgrad, 8 * (size_t)M)) ||
         (rc = salloc(S, &S->fF, 64 * (size_t)M)) || (rc = salloc(S, &S->fvec, 64 * (size_t)M)) ||
-        (rc = salloc(S, &S->fval, 8 * (size_t)M)))
+        (rc = salloc(S, &S->fval, 8 * (size_t)M)) ||
+        (rc = salloc(S, &S->stage, fd_batch_stage_bytes(plan, N) / sizeof(double2))))
     {
         glass_hip_ucb_sampler_destroy(S);
         return rc;
@@ -737,12 +740,13 @@ static int enqueue_steps(glass_hip_ucb_sampler* S, int nsteps, hipStream_t st)
     {
         ucb_propose_kernel<<<g, 256, 0, st>>>(D);
         GH_CHECK(hipGetLastError());
-        if (int rc = fd_batch_eval_dev(S->plan, N, D.it_chain, D.it_src, D.it_params, D.it_logl, D.it_snr, st))
+        if (int rc = fd_batch_eval_dev(S->plan, N, D.it_chain, D.it_src, D.it_params, D.it_logl, D.it_snr, st,
+                                       S->stage))
             return rc;
         ucb_accept_kernel<<<g, 256, 0, st>>>(D);
         GH_CHECK(hipGetLastError());
-        if (int rc = fd_batch_accept_dev(S->plan, N, D.nacc, D.acc_list, D.it_chain, D.it_src, D.it_params,
-                                         D.it_logl, st))
+        if (int rc = fd_batch_accept_staged_dev(S->plan, N, D.nacc, D.acc_list, D.it_chain, D.it_src, D.it_params,
+                                                D.it_logl, S->stage, st))
             return rc;
     }
     return GLASS_HIP_OK;
