@@ -1076,25 +1076,33 @@ __global__ __launch_bounds__(256) void accept_staged_kernel(int nacc, Items it, 
     double2* R = B.resid + (size_t)B.c_roff[c] * 3;
     double2* h = src_tmpl(B, c, s);
     const double2* hs = stg + (size_t)q * 3 * B.bwcap;
-    for (int j = max(0, io) + threadIdx.x; j < min(W, io + bo); j += blockDim.x)
+    // one pass over the union of both supports: (r + h_old) - h_new per bin
+    const int j0 = max(0, min(io, in)), j1 = min(W, max(io + bo, in + bn));
+    for (int j = j0 + threadIdx.x; j < j1; j += blockDim.x)
+    {
+        const bool o = j >= io && j < io + bo, nw = j >= in && j < in + bn;
+        if (!o && !nw) continue;
+#pragma unroll
         for (int ch = 0; ch < 3; ch++)
         {
             double2 v = R[(size_t)ch * W + j];
-            v.x += h[ch * cap + j - io].x;
-            v.y += h[ch * cap + j - io].y;
+            if (o)
+            {
+                v.x += h[ch * cap + j - io].x;
+                v.y += h[ch * cap + j - io].y;
+            }
+            if (nw)
+            {
+                v.x -= hs[ch * bn + j - in].x;
+                v.y -= hs[ch * bn + j - in].y;
+            }
             R[(size_t)ch * W + j] = v;
         }
+    }
     __syncthreads();  // the old template is read before its slot is overwritten
     for (int m = threadIdx.x; m < bn; m += blockDim.x)
+#pragma unroll
         for (int ch = 0; ch < 3; ch++) h[ch * cap + m] = hs[ch * bn + m];
-    for (int j = max(0, in) + threadIdx.x; j < min(W, in + bn); j += blockDim.x)
-        for (int ch = 0; ch < 3; ch++)
-        {
-            double2 v = R[(size_t)ch * W + j];
-            v.x -= hs[ch * bn + j - in].x;
-            v.y -= hs[ch * bn + j - in].y;
-            R[(size_t)ch * W + j] = v;
-        }
     if (threadIdx.x < 8) B.sparams[(size_t)so * 8 + threadIdx.x] = it.params[(size_t)q * 8 + threadIdx.x];
     if (threadIdx.x == 0)
     {
