@@ -1419,20 +1419,34 @@ extern "C" int glass_hip_fd_batch_derivs(glass_hip_fd_batch* P, int n, const int
     A.outL = d_logL;
     A.grad = d_grad;
     A.fisher = d_fisher;
-    for (int k = 0; k < kNDClass; k++)
+    // the classes on the plan's side streams as in launch_items (k % 4)
+    const bool par = P->concurrent;
+    if (par)
     {
+        GH_CHECK(hipEventRecord(P->evf, s));
+        for (int j = 0; j < 3; j++) GH_CHECK(hipStreamWaitEvent(P->side[j], P->evf, 0));
+    }
+    int lrc = GLASS_HIP_OK;
+    for (int k = kNDClass - 1; k >= 0 && !lrc; k--)
+    {
+        const hipStream_t sk = par && (k & 3) ? P->side[(k & 3) - 1] : s;
         A.list = P->D.list + (size_t)k * P->cls_cap;
         A.count = P->D.cnt + k;
         if (32 << k == 2048)
         {
             // BW 2048 only occurs when some window allows it (then d2048 exists)
-            if (P->d2048)
-                if (int rc = launch_derivs_2048(A, std::min(n, P->ncu), P->d2048, s)) return rc;
+            if (P->d2048) lrc = launch_derivs_2048(A, std::min(n, P->ncu), P->d2048, sk);
         }
-        else if (int rc = launch_derivs(32 << k, A, n, s))
-            return rc;
+        else
+            lrc = launch_derivs(32 << k, A, n, sk);
     }
-    return GLASS_HIP_OK;
+    if (par)
+        for (int j = 0; j < 3; j++)
+        {
+            GH_CHECK(hipEventRecord(P->evj[j], P->side[j]));
+            GH_CHECK(hipStreamWaitEvent(s, P->evj[j], 0));
+        }
+    return lrc;
 }
 
 extern "C" int glass_hip_fd_batch_sweep_graph(glass_hip_fd_batch* P, int nprop, const int* d_chain, const int* d_src,

@@ -119,8 +119,8 @@ int glass_hip_fd_batch_profile_read(glass_hip_fd_batch *plan, double *ms, int *l
                                     int *count);
 
 /* Concurrent BW classes (no reference counterpart; on by default): with
- * on != 0 the class kernels of an eval, an accept or a template rebuild
- * run on three side streams of the plan forked from and joined back into
+ * on != 0 the class kernels of an eval, an accept, a template rebuild or
+ * the exact derivatives run on three side streams of the plan forked from and joined back into
  * the launch stream (parallel branches under graph capture), so small and
  * latency-bound classes overlap; with on = 0 they run in turn on the
  * launch stream.  Results are identical either way.  Drops the captured
