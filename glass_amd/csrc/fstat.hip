@@ -132,22 +132,32 @@ __device__ inline void fstat_params(const double M[4][4], const double* N, doubl
     out[3] = atan2(c * (Ap * a[3] - Ac * a[0]), -c * (Ac * a[2] + Ap * a[1]));
 }
 
-// filter length of each cell and the per-class lists
-__global__ void fstat_align_kernel(int n, const double* __restrict__ cell, FstatDev D, int* __restrict__ cls,
-                                   int* __restrict__ list, int cap, int* __restrict__ err)
+// filter length of each cell and the per-class lists.  Slots are reserved
+// per block (an LDS histogram, then one global atomic per class per
+// block): the cells of a segment mostly share one filter length, and one
+// global atomic per cell on that class's counter serialises the launch.
+__global__ __launch_bounds__(256) void fstat_align_kernel(int n, const double* __restrict__ cell, FstatDev D,
+                                                          int* __restrict__ cls, int* __restrict__ list, int cap,
+                                                          int* __restrict__ err)
 {
+    __shared__ int hist[kFsClass + 1], base[kFsClass + 1];
+    if (threadIdx.x <= kFsClass) hist[threadIdx.x] = 0;
+    __syncthreads();
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
-    const double f0 = cell[(size_t)e * 4], fdot = cell[(size_t)e * 4 + 1], theta = cell[(size_t)e * 4 + 2];
-    const int N = ucb_bandwidth(D.L, D.fstar, f0, fdot, cos(theta), 1.e-22, D.T, D.NFFT);
-    int k = 31 - __clz(N) - 4;  // log2(N) - 4
-    if (k < 0 || k >= kFsClass || (N & (N - 1)))
+    int k = -1, slot = 0;
+    if (e < n)
     {
-        atomicAdd(err, 1);
-        return;
+        const double f0 = cell[(size_t)e * 4], fdot = cell[(size_t)e * 4 + 1], theta = cell[(size_t)e * 4 + 2];
+        const int N = ucb_bandwidth(D.L, D.fstar, f0, fdot, cos(theta), 1.e-22, D.T, D.NFFT);
+        k = 31 - __clz(N) - 4;  // log2(N) - 4
+        if (k < 0 || k >= kFsClass || (N & (N - 1))) k = kFsClass;
+        slot = atomicAdd(&hist[k], 1);
     }
-    const int slot = atomicAdd(&cls[k], 1);
-    list[(size_t)k * cap + slot] = e;
+    __syncthreads();
+    if (threadIdx.x < kFsClass && hist[threadIdx.x]) base[threadIdx.x] = atomicAdd(&cls[threadIdx.x], hist[threadIdx.x]);
+    if (threadIdx.x == kFsClass && hist[kFsClass]) atomicAdd(err, hist[kFsClass]);
+    __syncthreads();
+    if (k >= 0 && k < kFsClass) list[(size_t)k * cap + base[k] + slot] = e;
 }
 
 template <int N, int NTHR>
@@ -380,10 +390,17 @@ static int fstat_launch(glass_hip_fstat* F, int n, const double* cells, double* 
     fs_nan_kernel<<<(n + 255) / 256, 256, 0, s>>>(n, lx, lae, fpar);
     fstat_align_kernel<<<(n + 255) / 256, 256, 0, s>>>(n, cells, F->D, F->cls, F->list, F->cap,
                                                        F->cls + kFsClass);
+    // the class counts decide the grids: only non-empty classes launch, one
+    // workgroup per cell (an empty class's full grid of immediately
+    // exiting workgroups costs 80-160 us)
+    int cnt[kFsClass + 1];
+    GH_CHECK(hipMemcpyAsync(cnt, F->cls, sizeof(cnt), hipMemcpyDeviceToHost, s));
+    GH_CHECK(hipStreamSynchronize(s));
     Cells C{cells, lx, lae, fpar};
-#define FS_L(kk, NN)                                                                                       \
-    fstat_kernel<NN, fd_threads(NN)><<<n, fd_threads(NN), 0, s>>>(F->list + (size_t)kk * F->cap, F->cls + kk, \
-                                                                  F->D, C);
+#define FS_L(kk, NN)                                                                                            \
+    if (cnt[kk] > 0)                                                                                            \
+        fstat_kernel<NN, fd_threads(NN)><<<cnt[kk], fd_threads(NN), 0, s>>>(F->list + (size_t)kk * F->cap,     \
+                                                                            F->cls + kk, F->D, C);
     FS_L(0, 16) FS_L(1, 32) FS_L(2, 64) FS_L(3, 128) FS_L(4, 256) FS_L(5, 512) FS_L(6, 1024) FS_L(7, 2048)
 #undef FS_L
     GH_CHECK(hipGetLastError());

@@ -17,10 +17,20 @@ import sys
 
 
 def _short(name):
-    m = re.search(r"glass::(?:wdm::)?(\w+(?:<[^()]*?>)?)", name)
-    if m:
-        return m.group(1).replace(" ", "")
-    return name.split("(")[0].replace("void ", "")[:60]
+    """Kernel name without return type, namespaces and argument list,
+    template arguments kept: 'eval_kernel<512,256,false,1>'."""
+    n = re.sub(r"^void\s+", "", name.replace("(anonymous namespace)::", ""))
+    depth = 0
+    for i, ch in enumerate(n):
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == "(" and depth == 0:
+            n = n[:i]
+            break
+    n = re.sub(r"^(glass::)?(wdm::)?", "", n)
+    return n.replace(" ", "")[:80]
 
 
 def counters(path):

@@ -39,7 +39,9 @@ int glass_hip_fstat_create(glass_hip_fstat **fstat, const glass_hip_orbit *orbit
 int glass_hip_fstat_destroy(glass_hip_fstat *fstat);
 /* n <= max_cells cells, device arrays: cells[n][4] = f0, fdot, theta, phi;
  * logL_X[n], logL_AE[n], Fparams[n][4] out (NaN for a cell whose filter
- * length ucb_bandwidth(f0, fdot, cos theta, 1e-22) falls outside 16..2048) */
+ * length ucb_bandwidth(f0, fdot, cos theta, 1e-22) falls outside 16..2048).
+ * Synchronises the stream once (after bucketing the cells by filter length,
+ * so that only the non-empty lengths launch); not for graph capture. */
 int glass_hip_fstat_eval(glass_hip_fstat *fstat, int n, const double *d_cells, double *d_logL_X, double *d_logL_AE,
                          double *d_Fparams, void *stream);
 /* build_fstatistic_proposal over the band [fmin, fmax): n_f = (int)((fmax -
