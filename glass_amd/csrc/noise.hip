@@ -385,14 +385,19 @@ __global__ __launch_bounds__(kNThr) void noise_logl_kernel(int N, int chunk, con
     }
 }
 
-__global__ void noise_finish_kernel(const double* __restrict__ part, int nchunk, int n, double cst,
-                                    double* __restrict__ out)
+// One wave per chain: lane j sums the chunks k = j, j + 64, ... in order,
+// then a fixed shuffle tree adds the lanes (reproducible run to run; the
+// chunk partials of a small plan number in the hundreds, which a serial
+// sum per chain would read one dependent load at a time).
+__global__ __launch_bounds__(256) void noise_finish_kernel(const double* __restrict__ part, int nchunk, int n,
+                                                           double cst, double* __restrict__ out)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (i >= n) return;  // uniform over the wave
     double s = 0.0;
-    for (int k = 0; k < nchunk; k++) s += part[(size_t)k * n + i];
-    out[i] = s + cst;
+    for (int k = lane; k < nchunk; k += 64) s += part[(size_t)k * n + i];
+    s = wave_sum(s);
+    if (lane == 0) out[i] = s + cst;
 }
 
 // a link index; U = 1 exactly would give 6, one past the six links
@@ -828,7 +833,7 @@ static int launch_logl(glass_hip_noise_plan* P, const double* params, int n, dou
 {
     dim3 grid(P->nchunk, (n + kTile - 1) / kTile);
     noise_logl_kernel<<<grid, kNThr, 0, s>>>(P->N, P->chunk, P->basis, P->prod, P->cadd, params, n, P->part);
-    noise_finish_kernel<<<(n + 255) / 256, 256, 0, s>>>(P->part, P->nchunk, n, -3.0 * P->N * log(2 * M_PI), out);
+    noise_finish_kernel<<<(n + 3) / 4, 256, 0, s>>>(P->part, P->nchunk, n, -3.0 * P->N * log(2 * M_PI), out);
     GH_CHECK(hipGetLastError());
     return GLASS_HIP_OK;
 }

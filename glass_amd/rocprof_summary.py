@@ -3,12 +3,14 @@ measurements).
 
     python -m glass_amd.rocprof_summary counters run_counter_collection.csv
     python -m glass_amd.rocprof_summary stats run_kernel_stats.csv
+    python -m glass_amd.rocprof_summary trace run_kernel_trace.csv
 
 counters: per kernel (template arguments kept, grid size in blocks shown so
 batched launches stand apart from per-call ones) the number of dispatches,
 the mean dispatch duration and the mean of every counter per dispatch
 (summed over the counter's instances).  stats: the kernel-trace summary
-(calls, mean and total duration) in a short table.
+(calls, mean and total duration) in a short table.  trace: the same per
+kernel and grid size from the per-dispatch kernel trace.
 """
 import collections
 import csv
@@ -54,6 +56,31 @@ def counters(path):
         print(f"{key[0]} | {key[1]} | {n} | {dur[key] / n / 1e6:.4f} | {vals}")
 
 
+def _blocks(r):
+    """Workgroups of a dispatch from a kernel-trace or counter row."""
+    if "Grid_Size" in r:
+        g, w = int(r["Grid_Size"]), int(r.get("Workgroup_Size", 1) or 1)
+    else:
+        g = int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"])
+        w = int(r["Workgroup_Size_X"]) * int(r["Workgroup_Size_Y"]) * int(r["Workgroup_Size_Z"])
+    return g // max(w, 1)
+
+
+def trace(path):
+    """Per (kernel, grid size in blocks) dispatch count and mean duration
+    from run_kernel_trace.csv, so launches of one kernel at different sizes
+    stand apart."""
+    n = collections.Counter()
+    dur = collections.defaultdict(float)
+    for r in csv.DictReader(open(path)):
+        key = (_short(r["Kernel_Name"]), _blocks(r))
+        n[key] += 1
+        dur[key] += int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+    print("kernel | blocks | dispatches | mean_us | total_ms")
+    for key in sorted(n, key=lambda k: -dur[k]):
+        print(f"{key[0]} | {key[1]} | {n[key]} | {dur[key] / n[key] / 1e3:.2f} | {dur[key] / 1e6:.3f}")
+
+
 def stats(path):
     print("kernel | calls | mean_us | total_ms | percent")
     for r in csv.DictReader(open(path)):
@@ -62,4 +89,4 @@ def stats(path):
 
 
 if __name__ == "__main__":
-    {"counters": counters, "stats": stats}[sys.argv[1]](sys.argv[2])
+    {"counters": counters, "stats": stats, "trace": trace}[sys.argv[1]](sys.argv[2])
