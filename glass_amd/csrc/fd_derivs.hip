@@ -58,6 +58,11 @@ __device__ inline void block_sum_vec(double (&v)[kNSum], double* out)
     __syncthreads();
 }
 
+// LDS bytes for the series sets of one pass: sets per pass = this / 96 N
+// (at least one).  48 KB up to 256 bins (two or more workgroups per CU at
+// ~250 VGPRs), 96 KB above (one workgroup per CU either way, fewer passes).
+constexpr int deriv_lds_budget(int N) { return N <= 256 ? 49152 : 98304; }
+
 // One pass of series sets [S0, S0+NS): samples -> LDS -> FFT -> TDI into H.
 template <int N, int NTHR, int S0, int NS>
 __device__ inline void deriv_pass(const UcbConst& c, const UcbDiff& dd, double2* lds, const DerivArgs& A,
@@ -100,7 +105,7 @@ template <int N, int NTHR, int S0>
 __device__ inline void deriv_passes(const UcbConst& c, const UcbDiff& dd, double2* lds, const DerivArgs& A,
                                     double2 (&H)[FdShape<N, NTHR>::PER][6][3])
 {
-    constexpr int CAP = (98304 / (96 * N)) < 1 ? 1 : (98304 / (96 * N));
+    constexpr int CAP = (deriv_lds_budget(N) / (96 * N)) < 1 ? 1 : (deriv_lds_budget(N) / (96 * N));
     constexpr int NS = (6 - S0) < CAP ? (6 - S0) : CAP;
     deriv_pass<N, NTHR, S0, NS>(c, dd, lds, A, H);
     if constexpr (S0 + NS < 6) deriv_passes<N, NTHR, S0 + NS>(c, dd, lds, A, H);
@@ -110,7 +115,7 @@ template <int N, int NTHR>
 __global__ __launch_bounds__(NTHR) void deriv_kernel(DerivArgs A)
 {
     using S = FdShape<N, NTHR>;
-    constexpr int CAP = (98304 / (96 * N)) < 1 ? 1 : (98304 / (96 * N));
+    constexpr int CAP = (deriv_lds_budget(N) / (96 * N)) < 1 ? 1 : (deriv_lds_budget(N) / (96 * N));
     constexpr int NSMAX = CAP < 6 ? CAP : 6;
     __shared__ double2 lds[6 * NSMAX * N];
     __shared__ UcbDiff sdd;
