@@ -540,7 +540,7 @@ def fstat_leg(gl, seed=51, NFFT=1024, fq=3e-3, cpu_cells=200):
                 cpu_rate=cpu_cells / cpu_s, cpu_s=cpu_s, cpu_cells=cpu_cells)
 
 
-def globalfit_leg(gl, synth, steps=3, seed=4, cpu_seconds=2.0, nthreads=0):
+def globalfit_leg(gl, synth, steps=3, seed=4, cpu_seconds=2.0, nthreads=0, serial=False):
     """Global-fit UCB block at full scale on one GPU: 20,000 binaries (f0
     log-uniform 0.1-25 mHz, A log-uniform 1e-24..1e-21), Tobs 4 yr, the
     band in 4,000 density-weighted segments of 64-2048 bins, 8 temperatures
@@ -560,6 +560,8 @@ def globalfit_leg(gl, synth, steps=3, seed=4, cpu_seconds=2.0, nthreads=0):
     t0 = time.perf_counter()
     plan = FdBatch(o, "sangria", cfg["T"], 0.0, (cfg["group_qmin"], cfg["group_W"]), cfg["data_flat"],
                    cfg["invC_flat"], cfg["chain_group"], cfg["chain_nsrc"], cfg["src_params"], E, flat=True)
+    if serial:
+        plan.set_concurrent(False)
     torch.cuda.synchronize()
     t_plan = time.perf_counter() - t0
     tb, total = plan.memory()
@@ -960,7 +962,7 @@ def main():
         plan.close()
         torch.cuda.empty_cache()
         nthr = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
-        gl_ = globalfit_leg(gl, synth, nthreads=nthr)
+        gl_ = globalfit_leg(gl, synth, nthreads=nthr, serial=args.serial_classes)
         gfit = {"value": round(gl_["E"] / (gl_["ms"] * 1e-3), 1), "unit": "evals/s (sweep incl. Fisher items)",
                 "ms_per_sweep": round(gl_["ms"], 3), "evals_per_sweep": gl_["E"], "fisher_items": gl_["nF"],
                 "ms_eval_only": round(gl_["ms_eval"], 3), "ms_derivs_only": round(gl_["ms_derivs"], 3),
