@@ -149,20 +149,11 @@ __global__ __launch_bounds__(256) void align_kernel(int n, Items it, BatchDev B,
     if (k >= 0 && k < kNClass) cls_list[k * cls_stride + base[k] + slot] = e;
 }
 
-// Work distribution of the class kernels.  Persistent: a grid sized to the
-// resident capacity loops over the class list.  Otherwise: one workgroup
-// per proposal slot (grid = proposal count), workgroups beyond the class
-// count exit at once; the kernel body then has no loop, so nothing is
-// hoisted across templates and the register footprint stays that of one
-// template.
-#ifndef GH_PERSISTENT
-#define GH_PERSISTENT 0
-#endif
-#if GH_PERSISTENT
-#define GH_ITEM_LOOP for (int w = blockIdx.x; w < cnt; w += gridDim.x)
-#else
-#define GH_ITEM_LOOP if (const int w = blockIdx.x; w < cnt)
-#endif
+// Work distribution of the class kernels: one workgroup per proposal slot
+// (grid = proposal count; workgroups beyond the class count exit at once,
+// and the body has no loop, so nothing is hoisted across templates and the
+// register footprint stays that of one template), or a grid sized to a
+// multiple of the resident capacity striding over the class list.
 
 // The persistent loops rebuild a template per iteration from tables that do
 // not change between iterations (FFT twiddles, spacecraft positions).  Left
@@ -311,26 +302,38 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(MINW, 8)))
     X(4, 512, 512, 1) X(4, 512, 512, 6) X(5, 1024, 512, 1) X(5, 1024, 1024, 1) X(5, 1024, 256, 1)            \
     X(6, 2048, 512, 1) X(6, 2048, 1024, 1) X(7, 4096, 512, 1)
 
-// Template of each listed item into its source's slot.
+// Template of each listed item into its source's slot.  LOOP: the grid
+// strides over the class list (large rebuilds, where one workgroup per
+// item would mostly launch workgroups that exit at once).
 template <int N, int NTHR>
+__device__ __forceinline__ void tmpl_item(int w, double2* lds, const int* __restrict__ list, Items it, BatchDev B)
+{
+    const int e = list[w];
+    const UcbConst& k = ucb_load_shared(B.pc + e);
+    const int c = item_chain(it, e), cap = chain_cap(B, c);
+    double2* o = src_tmpl(B, c, item_src(it, e));
+    fd_link_bins<N, NTHR, fd_split(N, NTHR)>(k, lds, opaque(B.pos), kPosN / N, opaque(B.tw), kTwN / N,
+                                             [&](int m, const double2* dl) {
+                                                 double2 v[5];
+                                                 fd_tdi<N>(k, m, B.fmt, dl, v);
+#pragma unroll
+                                                 for (int ch = 0; ch < 3; ch++) o[ch * cap + m] = v[ch];
+                                             });
+}
+
+template <int N, int NTHR, bool LOOP>
 __global__ __launch_bounds__(NTHR) void tmpl_kernel(const int* __restrict__ list, const int* __restrict__ count,
                                                     Items it, BatchDev B)
 {
     __shared__ double2 lds[FdShape<N, NTHR, fd_split(N, NTHR)>::LDS];
     const int cnt = *count;
-    GH_ITEM_LOOP
+    if constexpr (LOOP)
     {
-        const int e = list[w];
-        const UcbConst& k = ucb_load_shared(B.pc + e);
-        const int c = item_chain(it, e), cap = chain_cap(B, c);
-        double2* o = src_tmpl(B, c, item_src(it, e));
-        fd_link_bins<N, NTHR, fd_split(N, NTHR)>(k, lds, opaque(B.pos), kPosN / N, opaque(B.tw), kTwN / N,
-                                                 [&](int m, const double2* dl) {
-                                                     double2 v[5];
-                                                     fd_tdi<N>(k, m, B.fmt, dl, v);
-#pragma unroll
-                                                     for (int ch = 0; ch < 3; ch++) o[ch * cap + m] = v[ch];
-                                                 });
+        for (int w = blockIdx.x; w < cnt; w += gridDim.x) tmpl_item<N, NTHR>(w, lds, list, it, B);
+    }
+    else
+    {
+        if ((int)blockIdx.x < cnt) tmpl_item<N, NTHR>(blockIdx.x, lds, list, it, B);
     }
 }
 
@@ -665,11 +668,16 @@ static int launch_items(glass_hip_fd_batch* P, int n, const Items& it, int mode,
         }
         else
         {
-            int g = GH_PERSISTENT ? P->tgrid[k] : n;
-            if (g > n) g = n;
+            // one workgroup per item, or a striding grid of 4x the resident
+            // capacity when the items are many more
+            const bool loop = n > 4 * P->tgrid[k];
+            const int g = loop ? 4 * P->tgrid[k] : n;
 #define GH_L(kk, N)                                                                               \
     case kk:                                                                                      \
-        tmpl_kernel<N, GH_NTHR(N)><<<g, GH_NTHR(N), 0, sk>>>(list, cnt, it, P->B);                \
+        if (loop)                                                                                 \
+            tmpl_kernel<N, GH_NTHR(N), true><<<g, GH_NTHR(N), 0, sk>>>(list, cnt, it, P->B);      \
+        else                                                                                      \
+            tmpl_kernel<N, GH_NTHR(N), false><<<g, GH_NTHR(N), 0, sk>>>(list, cnt, it, P->B);     \
         break;
             switch (k)
             {
