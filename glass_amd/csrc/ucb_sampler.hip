@@ -734,16 +734,18 @@ static int enqueue_fisher(glass_hip_ucb_sampler* S, hipStream_t st)
 static int enqueue_steps(glass_hip_ucb_sampler* S, int nsteps, hipStream_t st)
 {
     if (int rc = check_fstat(S)) return rc;
-    const int N = S->nslot, g = (N + 255) / 256;
+    // one thread per slot; one-wave workgroups spread the slots' long
+    // serial proposal over as many CUs as there are waves
+    const int N = S->nslot, g = (N + 63) / 64;
     SamplerDev& D = S->D;
     for (int k = 0; k < nsteps; k++)
     {
-        ucb_propose_kernel<<<g, 256, 0, st>>>(D);
+        ucb_propose_kernel<<<g, 64, 0, st>>>(D);
         GH_CHECK(hipGetLastError());
         if (int rc = fd_batch_eval_dev(S->plan, N, D.it_chain, D.it_src, D.it_params, D.it_logl, D.it_snr, st,
                                        S->stage))
             return rc;
-        ucb_accept_kernel<<<g, 256, 0, st>>>(D);
+        ucb_accept_kernel<<<g, 64, 0, st>>>(D);
         GH_CHECK(hipGetLastError());
         if (int rc = fd_batch_accept_staged_dev(S->plan, N, D.nacc, D.acc_list, D.it_chain, D.it_src, D.it_params,
                                                 D.it_logl, S->stage, st))
