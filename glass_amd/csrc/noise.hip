@@ -530,13 +530,21 @@ __global__ void noise_propose_kernel(NoiseState S)
     S.seed[s] = seed;
 }
 
-// Hastings ratio and accept (:569-585).  logH and logPy carry over the ten
-// moves of one call, as in the reference, where both are declared outside
-// its loop.
-__global__ void noise_accept_kernel(NoiseState S)
+// A move's likelihood sums, Hastings ratio and accept (:569-585) in one
+// launch: one wave per slot adds the slot's chunk partials as
+// noise_finish_kernel does, then its lane 0 takes the accept step.  logH and
+// logPy carry over the ten moves of one call, as in the reference, where
+// both are declared outside its loop.
+__global__ __launch_bounds__(256) void noise_finish_accept_kernel(const double* __restrict__ part, int nchunk,
+                                                                  double cst, NoiseState S)
 {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= S.nslot) return;
+    const int s = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (s >= S.nslot) return;  // uniform over the wave
+    double v = 0.0;
+    for (int k = lane; k < nchunk; k += 64) v += part[(size_t)k * S.nslot + s];
+    v = wave_sum(v);
+    if (lane != 0) return;
+    S.ylogl[s] = v + cst;
     unsigned seed = S.seed[s];
     const int st = S.index[s];
     const double logPy = S.logPy[s];
@@ -1004,8 +1012,11 @@ static int enqueue_sweep(glass_hip_noise_plan* P, int ncalls, int swap, hipStrea
         for (int mc = 0; mc < 10; mc++)
         {
             noise_propose_kernel<<<g, 256, 0, s>>>(S);
-            if (int rc = launch_logl(P, P->y, P->nslot, P->ylogl, s)) return rc;
-            noise_accept_kernel<<<g, 256, 0, s>>>(S);
+            dim3 grid(P->nchunk, (P->nslot + kTile - 1) / kTile);
+            noise_logl_kernel<<<grid, kNThr, 0, s>>>(P->N, P->chunk, P->basis, P->prod, P->cadd, P->y, P->nslot,
+                                                     P->part);
+            noise_finish_accept_kernel<<<(P->nslot + 3) / 4, 256, 0, s>>>(P->part, P->nchunk,
+                                                                          -3.0 * P->N * log(2 * M_PI), S);
         }
     }
     if (swap) noise_ptswap_kernel<<<(P->nladder + 63) / 64, 64, 0, s>>>(S, P->nladder, P->ntemp);
