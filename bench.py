@@ -413,6 +413,42 @@ def noise_leg(gl, steps=3, seed=31, pcalls=2, ladders=64):
     return res
 
 
+def formats_leg(plan, cfg):
+    """The reference's checkpoint (save_chain_state / restore_chain_state,
+    ucb/src/glass_ucb_io.c:418-490) of every chain of the batched plan: the
+    device state copied and written as chain_state_<c>.dat, then every file
+    read back; the read-back parameters must print as the plan's do."""
+    import shutil
+    import tempfile
+    import torch
+    from glass_amd import files
+    d = tempfile.mkdtemp(prefix="glass_ckpt_")
+    try:
+        logl, sp = plan.state()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        files.save_chain_states(plan, d, step=1, logLmax=float(logl.max()), t0=0.0, eta=[1.0, 1.0, 1.0])
+        t_write = time.perf_counter() - t0
+        nchain = len(cfg["chain_group"])
+        t0 = time.perf_counter()
+        back = [files.restore_chain_state(os.path.join(d, f"chain_state_{c}.dat"), cfg["T"], 3,
+                                          maxsrc=sp.shape[1]) for c in range(nchain)]
+        t_read = time.perf_counter() - t0
+        nsrc = int(sum(b["nlive"] for b in back))
+        same = all(files.print_source_params(b["params"][k], cfg["T"]) ==
+                   files.print_source_params(sp[c, k], cfg["T"])
+                   for c, b in enumerate(back[:64]) for k in range(b["nlive"]))
+        if not same:
+            raise SystemExit("formats: checkpoint read-back differs from the plan's state")
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    return {"chains": nchain, "sources": nsrc, "write_s": t_write, "read_s": t_read,
+            "sources_per_s": 2 * nsrc / (t_write + t_read), "parity": "read-back parameters print as the plan's "
+            "(first 64 chains)",
+            "config": "save_chain_state / restore_chain_state of the batched case's chains, one file per chain, "
+                      "host formatting after one device-to-host copy"}
+
+
 def sampler_leg(plan, cfg, nladder, ntemp, nsweep=3, nsteps=80, seed=41):
     """The UCB sampler on the device over the batched plan's chains
     (glass_hip_ucb_sampler_*): one iteration of the reference's outer loop
@@ -703,6 +739,7 @@ def main():
     ap.add_argument("--no-noise", action="store_true")
     ap.add_argument("--no-sampler", action="store_true")
     ap.add_argument("--no-fstat", action="store_true")
+    ap.add_argument("--no-formats", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default tiles (skip the autotuner)")
     ap.add_argument("--serial-classes", action="store_true",
                     help="run the BW-class kernels of each sweep in turn on one stream (profiling runs)")
@@ -957,6 +994,12 @@ def main():
         log(f"sampler: {smpl['value']:.4e} steps/s, {sl['ms']:.2f} ms per iteration (steps eager "
             f"{sl['ms_steps_eager']:.2f} ms, Fisher update {sl['ms_fisher']:.2f} ms), acceptance {sl['acceptance']}")
 
+    fmts = None
+    if not args.no_formats:
+        fmts = formats_leg(plan, cfg)
+        log(f"formats: checkpoint of {fmts['chains']} chains / {fmts['sources']} sources written in "
+            f"{fmts['write_s']:.3f} s, read back in {fmts['read_s']:.3f} s ({fmts['sources_per_s']:.3e} sources/s)")
+
     gfit = None
     if args.globalfit:
         plan.close()
@@ -1006,6 +1049,7 @@ def main():
         "globalfit": gfit,
         "noise": nzl,
         "sampler": smpl,
+        "formats": fmts,
         "fstat": fst,
         "parity": parity,
         "roofline": roofline,
