@@ -105,20 +105,22 @@ bool psd_state_text(int step, double logL, double logLnorm, const double* eta, i
     return put(buf, cap, len, "\n");
 }
 
-// scan_source_params on a stream, then map_ucb_params_to_array
+// scan_source_params on a stream, then map_ucb_params_to_array.  Every
+// conversion must succeed (the reference fails only when none does, so a
+// truncated file would pass it with values never read).
 bool scan_source(FILE* f, double T, double* params)
 {
     Named s{};
     int check = 0;
-    check += fscanf(f, "%lg", &s.f0);
-    check += fscanf(f, "%lg", &s.dfdt);
-    check += fscanf(f, "%lg", &s.amp);
-    check += fscanf(f, "%lg", &s.phi);
-    check += fscanf(f, "%lg", &s.costheta);
-    check += fscanf(f, "%lg", &s.cosi);
-    check += fscanf(f, "%lg", &s.psi);
-    check += fscanf(f, "%lg", &s.phiref);
-    if (!check) return false;
+    check += fscanf(f, "%lg", &s.f0) == 1;
+    check += fscanf(f, "%lg", &s.dfdt) == 1;
+    check += fscanf(f, "%lg", &s.amp) == 1;
+    check += fscanf(f, "%lg", &s.phi) == 1;
+    check += fscanf(f, "%lg", &s.costheta) == 1;
+    check += fscanf(f, "%lg", &s.cosi) == 1;
+    check += fscanf(f, "%lg", &s.psi) == 1;
+    check += fscanf(f, "%lg", &s.phiref) == 1;
+    if (check != kNP) return false;
     to_array(s, T, params);
     return true;
 }
@@ -271,7 +273,8 @@ extern "C" int glass_hip_read_injection_file(const char* path, double T, int max
     N--;
     for (int k = 0; k < N; k++)
     {
-        if (!fscanf(f, "%lg %lg %lg %lg %lg %lg %lg %lg", &f0, &dfdt, &theta, &phi, &amp, &iota, &psi, &phi0))
+        // each counted source must convert all eight fields
+        if (fscanf(f, "%lg %lg %lg %lg %lg %lg %lg %lg", &f0, &dfdt, &theta, &phi, &amp, &iota, &psi, &phi0) != 8)
         {
             fclose(f);
             set_error("read_injection_file: error reading %s", path);
@@ -347,21 +350,22 @@ extern "C" int glass_hip_read_chain_state_file(const char* path, int maxsrc, int
         set_error("read_chain_state_file: error reading the %s of %s", what, path);
         return GLASS_HIP_EINVAL;
     };
-    if (!fscanf(f, "%lg", logLmax)) return fail("maximum logL");
+    // the reference's reads, each required to convert all its fields
+    if (fscanf(f, "%lg", logLmax) != 1) return fail("maximum logL");
     // scan_chain_state
     int check = 0;
-    check += fscanf(f, "%i", step);
-    check += fscanf(f, "%i", nlive);
-    check += fscanf(f, "%lg", logL);
-    check += fscanf(f, "%lg", logLnorm);
-    check += fscanf(f, "%lg", t0);
-    if (!check) return fail("chain state");
+    check += fscanf(f, "%i", step) == 1;
+    check += fscanf(f, "%i", nlive) == 1;
+    check += fscanf(f, "%lg", logL) == 1;
+    check += fscanf(f, "%lg", logLnorm) == 1;
+    check += fscanf(f, "%lg", t0) == 1;
+    if (check != 5) return fail("chain state");
     // scan_psd_state
     check = 0;
-    check += fscanf(f, "%i ", step);
-    check += fscanf(f, "%lg %lg ", logL, logLnorm);
-    for (int n = 0; n < neta; n++) check += fscanf(f, "%lg", &eta[n]);
-    if (!check) return fail("noise state");
+    check += fscanf(f, "%i ", step) == 1;
+    check += fscanf(f, "%lg %lg ", logL, logLnorm) == 2;
+    for (int n = 0; n < neta; n++) check += fscanf(f, "%lg", &eta[n]) == 1;
+    if (check != 2 + neta) return fail("noise state");
     if (*nlive < 0 || *nlive > maxsrc)
     {
         fclose(f);

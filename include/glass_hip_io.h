@@ -72,7 +72,9 @@ int glass_hip_write_chain_state_file(const char *path, double logLmax, int step,
  * chain-state file: logLmax, scan_chain_state (step, Nlive, logL, logLnorm,
  * t0), scan_psd_state (step, logL, logLnorm, eta[neta]), then Nlive
  * sources (at most maxsrc) into params[maxsrc][8].  The psd state's step
- * and likelihoods overwrite the chain state's, as in the reference. */
+ * and likelihoods overwrite the chain state's, as in the reference.  Every
+ * field must convert (the reference accepts a file as long as some field of
+ * each read does, so it takes truncated files). */
 int glass_hip_read_chain_state_file(const char *path, int maxsrc, int neta, double T, double *logLmax, int *step,
                                     int *nlive, double *logL, double *logLnorm, double *t0, double *eta,
                                     double *params);

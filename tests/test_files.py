@@ -173,3 +173,8 @@ def test_bad_files(tmp_path):
     files.save_chain_state(path, 0.0, 1, 0.0, 0.0, 0.0, [1.0], random_params(np.random.default_rng(6), 3), T)
     with pytest.raises(GlassHipError):
         files.restore_chain_state(path, T, 1, maxsrc=2)
+    # a checkpoint cut inside its last source line
+    text = path.read_text()
+    path.write_text(text[:text.rstrip().rfind(" ")])
+    with pytest.raises(GlassHipError):
+        files.restore_chain_state(path, T, 1)
