@@ -312,6 +312,7 @@ def _wdm_signatures(L):
                                                    _dp, _dp, _dp]
     L.orc_ucb_waveform_wavelet_compact.restype = c_int
     L.orc_ucb_wavelet_series.argtypes = [c_int, _dp, _dp, c_d, c_d, _dp, c_int, c_d, c_d, _dp]
+    L.orc_ucb_barycenter_waveform.argtypes = [_dp, c_int, _dp, _dp, _dp, c_d]
     L.orc_set_cr_trig.argtypes = [c_int]
     L.orc_ucb_wavelet_response.argtypes = [c_int, _dp, _dp, c_d, c_d, _dp, _dp, _dp, _dp, _dp]
     L.orc_snr_wavelet.argtypes = [_dp, _dp, _dp, _dp, c_int, _ip, c_int]
@@ -478,6 +479,15 @@ def ucb_wavelet_series(orb, Tobs, params, N, dt_s, fhet, t0=0.0):
     lib().orc_ucb_wavelet_series(len(t), _d(t), _d(coef), float(Tobs), float(t0), _d(p), int(N),
                                  float(dt_s), float(fhet), _d(out))
     return out
+
+
+def ucb_barycenter_waveform(params, times, T):
+    """(phase, amp) of ucb_barycenter_waveform at the given SSB times."""
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    tt = np.ascontiguousarray(times, dtype=np.float64)
+    ph, am = np.zeros(len(tt)), np.zeros(len(tt))
+    lib().orc_ucb_barycenter_waveform(_d(p), len(tt), _d(tt), _d(ph), _d(am), float(T))
+    return ph, am
 
 
 def ucb_wavelet_response(orb, Tobs, params, t0=0.0):

@@ -653,6 +653,18 @@ static double ucb_amplitude(double t, const double *p, double T)
     return A0 * (1.0 + 2.0 / 3.0 * fdot / f0 * t);
 }
 
+/* ucb_barycenter_waveform (ucb/src/glass_ucb_waveform.c:355-362): the SSB
+ * phase and amplitude of ucb_phase / ucb_amplitude (:332-353) at N times. */
+void orc_ucb_barycenter_waveform(const double *params, int N, const double *times, double *phase,
+                                 double *amp, double T)
+{
+    for (int n = 0; n < N; n++)
+    {
+        phase[n] = ucb_phase(times[n], params, T);
+        amp[n] = ucb_amplitude(times[n], params, T);
+    }
+}
+
 /* The interpolated TDI time series of ucb_waveform_wavelet
  * (ucb/src/glass_ucb_waveform.c:1064-1178): SSB phase/amplitude splines on
  * the orbit knots, spacecraft-1 phase, TDI amplitude/phase on Norb samples
