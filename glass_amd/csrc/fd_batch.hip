@@ -114,7 +114,9 @@ __global__ void zero_int_kernel(int* p, int n)
 // the classes are counted in *err and left out.
 __global__ __launch_bounds__(256) void align_kernel(int n, Items it, BatchDev B, int* __restrict__ bw_out,
                                                     int* __restrict__ imin_out, int* __restrict__ cls_count,
-                                                    int* __restrict__ cls_list, int cls_stride, int* __restrict__ err)
+                                                    int* __restrict__ cls_list, int cls_stride, int* __restrict__ err,
+                                                    double* __restrict__ logl_out = nullptr,
+                                                    double* __restrict__ snr_out = nullptr)
 {
     __shared__ int hist[kNClass + 1], base[kNClass + 1];
     if (threadIdx.x <= kNClass) hist[threadIdx.x] = 0;
@@ -139,7 +141,14 @@ __global__ __launch_bounds__(256) void align_kernel(int n, Items it, BatchDev B,
         // the template constants of the class kernels, once per item
         ucb_setup(p, 8, B.T, B.t0, B.L, B.fstar, B.pc[e]);
         k = 31 - __clz(BW) - 5;  // log2(BW) - 5
-        if (BW > B.g_cap[g] || k < 0 || k >= kNClass || (BW & (BW - 1))) k = kNClass;
+        if (BW > B.g_cap[g] || k < 0 || k >= kNClass || (BW & (BW - 1)))
+        {
+            k = kNClass;
+            // no class kernel evaluates this item: a likelihood of -inf
+            // (and snr 0) keeps a sampler from ever accepting it
+            if (logl_out) logl_out[e] = -INFINITY;
+            if (snr_out) snr_out[e] = 0.0;
+        }
         slot = atomicAdd(&hist[k], 1);
     }
     __syncthreads();
@@ -640,7 +649,8 @@ static int launch_items(glass_hip_fd_batch* P, int n, const Items& it, int mode,
     }
     zero_int_kernel<<<1, 64, 0, s>>>(P->cls, kNClass + 1);
     align_kernel<<<(n + 255) / 256, 256, 0, s>>>(n, it, P->B, P->p_bw, P->p_imin, P->cls, P->cls_list, P->cls_cap,
-                                                 P->cls + kNClass);
+                                                 P->cls + kNClass, mode == 0 ? logl_out : nullptr,
+                                                 mode == 0 ? snr_out : nullptr);
     // the per-class timings of the profiling mode need the classes in turn
     const bool par = P->concurrent && !(P->prof && mode == 0);
     if (par)
@@ -801,9 +811,11 @@ extern "C" int glass_hip_fd_batch_create_ex(glass_hip_fd_batch** plan, const gla
     int Wmax = 0;
     for (int g = 0; g < ngroup; g++)
     {
-        if (group_W[g] < 32)
+        // ucb_bandwidth allows 2 * 2^floor(log2(W/2)) bins: at most the
+        // 4096-bin template slots (and positions table) for W < 8192
+        if (group_W[g] < 32 || group_W[g] >= 2 * kPosN)
         {
-            set_error("fd_batch_create: window %d has W=%d < 32", g, group_W[g]);
+            set_error("fd_batch_create: window %d has W=%d outside 32..%d", g, group_W[g], 2 * kPosN - 1);
             return GLASS_HIP_EINVAL;
         }
         goff[g + 1] = goff[g] + group_W[g];
@@ -1081,6 +1093,9 @@ __global__ __launch_bounds__(256) void accept_staged_kernel(int nacc, Items it, 
     const int g = B.c_group[c], W = B.g_W[g], cap = B.g_cap[g];
     const int io = B.s_imin[so], bo = B.s_bw[so];
     const int in = imin_e[q], bn = bw_e[q];
+    // an item the alignment left out of every class (BW above the window's
+    // slot) has no staged template: never written into the slot
+    if (bn > cap || bn > B.bwcap) return;
     double2* R = B.resid + (size_t)B.c_roff[c] * 3;
     double2* h = src_tmpl(B, c, s);
     const double2* hs = stg + (size_t)q * 3 * B.bwcap;

@@ -25,6 +25,7 @@
 #include <cstring>
 #include <vector>
 
+#include "../../include/glass_hip_io.h"
 #include "../../include/glass_hip_sampler.h"
 #include "fd_batch_hooks.h"
 #include "rng.h"
@@ -38,6 +39,7 @@ constexpr double kPi = 3.14159265358979323846;
 constexpr double kTwoPi = 6.283185307179586;        // PI2 of the reference
 constexpr double kYear = 3.15581497632e7;           // YEAR of the reference
 constexpr int kWrapMax = 64;                        // periods stepped one at a time
+constexpr int kFsMaxTries = 1 << 24;                // rejection tries of one F-statistic draw
 
 struct SamplerDev
 {
@@ -83,6 +85,7 @@ struct SamplerDev
     const double* const* fs_ten;
     const int* fs_dims;
     const double* fs_par;
+    int* fs_overrun;  // draws that gave up after kFsMaxTries rejections
     // options
     double w[9];
     int fix_sky, fix_freq, fix_fdot;
@@ -220,12 +223,22 @@ __device__ inline double fstat_density(double* p, const double* pr, const double
 // the rejection sampling of draw_from_fstatistic / jump_from_fstatistic
 // (ucb/src/glass_ucb_proposal.c:1526-1638); jump: the reference's range
 // check (which a draw of U = 1 fails) ends the proposal with y[0..2] as
-// they were
-__device__ inline void fstat_sample(unsigned* seed, double* y, const FsProp& f, double T, bool jump)
+// they were.  The reference loops until a cell is accepted; here a draw
+// gives up after kFsMaxTries tries (about maxp x cells / sum of the density
+// tries are expected, and set_fstat refuses tables that make that
+// unbounded), counts the overrun and leaves y[0..2] unchanged.
+__device__ inline void fstat_sample(unsigned* seed, double* y, const FsProp& f, double T, bool jump,
+                                    int* overrun)
 {
     double i, j, k, q, costheta, phi, p, alpha;
+    int tries = 0;
     do
     {
+        if (++tries > kFsMaxTries)
+        {
+            atomicAdd(overrun, 1);
+            return;
+        }
         i = rand_U(seed) * f.nf;
         j = rand_U(seed) * f.nt;
         k = rand_U(seed) * f.nph;
@@ -283,7 +296,7 @@ __global__ void ucb_propose_kernel(SamplerDev S)
             for (int k = 0; k < 8; k++) y[k] = pr[2 * k] + rand_U(&seed) * (pr[2 * k + 1] - pr[2 * k]);
         else
             (void)(rand_U(&seed) < -0.5);  // the fm-shift branch of jump_from_fstatistic, never taken
-        fstat_sample(&seed, y, f, S.T, np == 2);
+        fstat_sample(&seed, y, f, S.T, np == 2, S.fs_overrun);
     }
     else if (np == 4)
     {
@@ -587,7 +600,7 @@ extern "C" int glass_hip_ucb_sampler_create(glass_hip_ucb_sampler** out, glass_h
         (rc = salloc(S, &D.it_src, N)) || (rc = salloc(S, &D.it_params, 8 * (size_t)N)) ||
         (rc = salloc(S, &D.it_logl, N)) || (rc = salloc(S, &D.it_snr, N)) || (rc = salloc(S, &D.lpx, N)) ||
         (rc = salloc(S, &D.lpy, N)) || (rc = salloc(S, &D.prop, N)) || (rc = salloc(S, &D.acc_list, N)) ||
-        (rc = salloc(S, &D.nacc, 1)) || (rc = salloc(S, &D.lq, N)) || (rc = salloc(S, &S->d_fs_ten, v.G)) ||
+        (rc = salloc(S, &D.nacc, 1)) || (rc = salloc(S, &D.lq, N)) || (rc = salloc(S, &D.fs_overrun, 1)) || (rc = salloc(S, &S->d_fs_ten, v.G)) ||
         (rc = salloc(S, &S->d_fs_dims, 3 * (size_t)v.G)) || (rc = salloc(S, &S->d_fs_par, 3 * (size_t)v.G)) ||
         (rc = salloc(S, &S->fL, M)) || (rc = salloc(S, &S->fgrad, 8 * (size_t)M)) ||
         (rc = salloc(S, &S->fF, 64 * (size_t)M)) || (rc = salloc(S, &S->fvec, 64 * (size_t)M)) ||
@@ -672,7 +685,32 @@ extern "C" int glass_hip_ucb_sampler_set_state(glass_hip_ucb_sampler* S, const u
     GH_CHECK(hipMemcpyAsync(D.swaps, zero.data(), sizeof(int) * N, hipMemcpyHostToDevice, st));
     GH_CHECK(hipMemcpyAsync(D.pacc, zero.data(), sizeof(int) * N, hipMemcpyHostToDevice, st));
     GH_CHECK(hipMemcpyAsync(D.iter, &adapt_iter0, sizeof(int), hipMemcpyHostToDevice, st));
+    GH_CHECK(hipMemcpyAsync(D.fs_overrun, zero.data(), sizeof(int), hipMemcpyHostToDevice, st));
     GH_CHECK(hipStreamSynchronize(st));
+    return GLASS_HIP_OK;
+}
+
+// A proposal density the device's rejection sampler can draw from: finite,
+// non-negative cells, a positive maximum, and maxp no larger than it (the
+// reference sets maxp to the table's maximum, ucb/src/glass_ucb_proposal.c:1172-1177;
+// a larger maxp only adds rejections, an all-zero table never accepts).
+static int fstat_table_check(const double* t, size_t n, double maxp)
+{
+    double mx = 0.0;
+    for (size_t i = 0; i < n; i++)
+    {
+        if (!(t[i] >= 0.0) || !std::isfinite(t[i]))
+        {
+            set_error("ucb_sampler_set_fstat: density cell %zu is %g (must be finite and >= 0)", i, t[i]);
+            return GLASS_HIP_EINVAL;
+        }
+        mx = t[i] > mx ? t[i] : mx;
+    }
+    if (!(mx > 0.0) || !(maxp <= mx * (1.0 + 1e-12)))
+    {
+        set_error("ucb_sampler_set_fstat: maxp %g must be > 0 and at most the density's maximum %g", maxp, mx);
+        return GLASS_HIP_EINVAL;
+    }
     return GLASS_HIP_OK;
 }
 
@@ -680,15 +718,21 @@ extern "C" int glass_hip_ucb_sampler_set_fstat(glass_hip_ucb_sampler* S, int g, 
                                                double d_f, int n_theta, int n_phi, double maxp, int data_qmin,
                                                void* stream)
 {
-    if (!S || g < 0 || g >= S->v.G || !tensor || n_f <= 0 || n_theta <= 0 || n_phi <= 0 || !(d_f > 0) ||
-        !(maxp > 0))
+    if (!tensor || n_f <= 0 || n_theta <= 0 || n_phi <= 0 || !(d_f > 0) || !(maxp > 0))
+    {
+        set_error("ucb_sampler_set_fstat: bad arguments (window %d)", g);
+        return GLASS_HIP_EINVAL;
+    }
+    // the table is checked first, on the host
+    const size_t n = (size_t)n_f * n_theta * n_phi;
+    if (int rc = fstat_table_check(tensor, n, maxp)) return rc;
+    if (!S || g < 0 || g >= S->v.G)
     {
         set_error("ucb_sampler_set_fstat: bad arguments (window %d)", g);
         return GLASS_HIP_EINVAL;
     }
     GH_CHECK(hipSetDevice(S->v.device));
     hipStream_t st = (hipStream_t)stream;
-    const size_t n = (size_t)n_f * n_theta * n_phi;
     if (S->graph && drop_graph(S)) return GLASS_HIP_ERUNTIME;
     GH_CHECK(hipStreamSynchronize(st));
     if (S->fs_buf[g]) GH_CHECK(hipFree(S->fs_buf[g]));
@@ -889,6 +933,63 @@ extern "C" int glass_hip_ucb_sampler_get_state(glass_hip_ucb_sampler* S, unsigne
     if (trials) GH_CHECK(hipMemcpyAsync(trials, D.trials, sizeof(int) * 9 * N, hipMemcpyDeviceToHost, st));
     if (accepts) GH_CHECK(hipMemcpyAsync(accepts, D.accepts, sizeof(int) * 9 * N, hipMemcpyDeviceToHost, st));
     if (swaps) GH_CHECK(hipMemcpyAsync(swaps, D.swaps, sizeof(int) * N, hipMemcpyDeviceToHost, st));
+    int overrun = 0;
+    GH_CHECK(hipMemcpyAsync(&overrun, D.fs_overrun, sizeof(int), hipMemcpyDeviceToHost, st));
     GH_CHECK(hipStreamSynchronize(st));
+    if (overrun)
+    {
+        set_error("ucb_sampler: %d F-statistic draws gave up after %d rejections (proposal left unchanged)",
+                  overrun, kFsMaxTries);
+        return GLASS_HIP_ERUNTIME;
+    }
+    return GLASS_HIP_OK;
+}
+
+extern "C" int glass_hip_ucb_sampler_save_chain_states(glass_hip_ucb_sampler* S, const char* const* dirs, int step,
+                                                       const double* logLmax, const double* logLnorm, double t0,
+                                                       const double* eta, int neta, void* stream)
+{
+    if (!S || !dirs || neta < 0 || (neta && !eta))
+    {
+        set_error("ucb_sampler_save_chain_states: bad arguments");
+        return GLASS_HIP_EINVAL;
+    }
+    for (int l = 0; l < S->nladder; l++)
+        if (!dirs[l])
+        {
+            set_error("ucb_sampler_save_chain_states: no directory for ladder %d", l);
+            return GLASS_HIP_EINVAL;
+        }
+    GH_CHECK(hipSetDevice(S->v.device));
+    hipStream_t st = (hipStream_t)stream;
+    const FdBatchView& v = S->v;
+    std::vector<int> index(S->nslot), nsrc(v.C);
+    GH_CHECK(hipMemcpyAsync(index.data(), S->D.index, sizeof(int) * S->nslot, hipMemcpyDeviceToHost, st));
+    GH_CHECK(hipMemcpyAsync(nsrc.data(), v.c_nsrc, sizeof(int) * v.C, hipMemcpyDeviceToHost, st));
+    GH_CHECK(hipStreamSynchronize(st));
+    std::vector<double> logl(v.C), sp((size_t)v.C * v.maxsrc * 8);
+    if (int rc = glass_hip_fd_batch_state(S->plan, logl.data(), sp.data())) return rc;
+    std::vector<char> path(4096);
+    // save_chain_state (ucb/src/glass_ucb_io.c:418-446): file ic of a run
+    // holds model[chain->index[ic]], the chain now at temperature ic
+    for (int l = 0; l < S->nladder; l++)
+        for (int t = 0; t < S->ntemp; t++)
+        {
+            const int c = index[(size_t)l * S->ntemp + t];
+            if (c < 0 || c >= v.C)
+            {
+                set_error("ucb_sampler_save_chain_states: slot %d maps to chain %d", l * S->ntemp + t, c);
+                return GLASS_HIP_ERUNTIME;
+            }
+            if (snprintf(path.data(), path.size(), "%s/chain_state_%i.dat", dirs[l], t) >= (int)path.size())
+            {
+                set_error("ucb_sampler_save_chain_states: directory name too long");
+                return GLASS_HIP_EINVAL;
+            }
+            if (int rc = glass_hip_write_chain_state_file(path.data(), logLmax ? logLmax[l] : 0.0, step, nsrc[c],
+                                                          logl[c], logLnorm ? logLnorm[c] : 0.0, t0, eta, neta,
+                                                          sp.data() + (size_t)c * v.maxsrc * 8, v.T))
+                return rc;
+        }
     return GLASS_HIP_OK;
 }

@@ -13,6 +13,7 @@ once:
   matrix_eigenstuff's LAPACKE_dgeev (utils/src/glass_math.c:475-499).
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -52,6 +53,8 @@ _SIGS = [
     ("glass_hip_ucb_sampler_sweep", [_vp, _i, _i, _i, _vp], _i),
     ("glass_hip_ucb_sampler_get_state", [_vp, _up, _ip, _dp, _ip, _ip, _ip, _vp], _i),
     ("glass_hip_ucb_sampler_set_fstat", [_vp, _i, _dp, _i, ctypes.c_double, _i, _i, ctypes.c_double, _i, _vp], _i),
+    ("glass_hip_ucb_sampler_save_chain_states", [_vp, ctypes.POINTER(ctypes.c_char_p), _i, _dp, _dp,
+                                                 ctypes.c_double, _dp, _i, _vp], _i),
 ]
 _declared = False
 
@@ -184,3 +187,17 @@ class UcbSampler:
                                                    tr.ctypes.data_as(_ip), ac.ctypes.data_as(_ip),
                                                    sw.ctypes.data_as(_ip), _stream()), "ucb_sampler_get_state")
         return {"seeds": sd, "index": idx, "temperature": t, "trials": tr, "accepts": ac, "swaps": sw}
+
+    def save_chain_states(self, dirs, step, logLmax=None, lognorm=None, t0=0.0, eta=None):
+        """save_chain_state of every ladder: dirs[l]/chain_state_<t>.dat holds
+        the chain now at temperature t of ladder l."""
+        assert len(dirs) == self.nladder
+        arr = (ctypes.c_char_p * self.nladder)(*[os.fsencode(d) for d in dirs])
+        lm = None if logLmax is None else np.ascontiguousarray(logLmax, dtype=np.float64).reshape(self.nladder)
+        ln = None if lognorm is None else np.ascontiguousarray(lognorm, dtype=np.float64).reshape(self.nslot)
+        et = None if eta is None else np.ascontiguousarray(eta, dtype=np.float64).ravel()
+        check(_L().glass_hip_ucb_sampler_save_chain_states(self.h, arr, int(step), None if lm is None else _p(lm),
+                                                           None if ln is None else _p(ln), float(t0),
+                                                           None if et is None else _p(et),
+                                                           0 if et is None else len(et), _stream()),
+              "ucb_sampler_save_chain_states")

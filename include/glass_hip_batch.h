@@ -53,8 +53,12 @@ typedef struct glass_hip_fd_batch glass_hip_fd_batch;
 
 /* Creates a plan and uploads the data (host arrays, copied).  format:
  * "sangria" | "phase" | "frequency".  max_prop: largest proposal count a
- * single eval will be given.  Computes every source's template, the
- * residuals and logL_c on the device before returning. */
+ * single eval will be given.  Windows hold 32 <= W_g <= 8191 bins (the
+ * largest template ucb_bandwidth allows there is 4096 bins); wider windows
+ * are refused with GLASS_HIP_EINVAL.  Computes every source's template, the
+ * residuals and logL_c on the device before returning.  An evaluated item
+ * whose template fits no class gets logL -inf and snr 0 (and is counted in
+ * the plan's error count), so a sampler never accepts it. */
 int glass_hip_fd_batch_create(glass_hip_fd_batch **plan, const glass_hip_orbit *orbit,
                               const char *format, double T, double t0, int ngroup,
                               const int *group_qmin, const int *group_W, const double *data,

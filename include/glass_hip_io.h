@@ -83,7 +83,10 @@ int glass_hip_read_chain_state_file(const char *path, int maxsrc, int neta, doub
  * counterpart as one call): copies the plan's chain logL and live source
  * parameters from the device and writes dir/chain_state_<c>.dat for chain
  * c = 0..nchain-1, with logLnorm[c] (NULL: 0), the common t0 and eta
- * (neta values, NULL: none).  Synchronises the device. */
+ * (neta values, NULL: none).  Synchronises the device.  Files are named by
+ * plan chain: once a sampler's temperature swaps have permuted which chain
+ * sits at which temperature, checkpoint with
+ * glass_hip_ucb_sampler_save_chain_states instead (files by temperature). */
 int glass_hip_fd_batch_save_chain_states(glass_hip_fd_batch *plan, const char *dir, int step, double logLmax,
                                          const double *logLnorm, double t0, const double *eta, int neta);
 

@@ -73,7 +73,12 @@ int glass_hip_ucb_sampler_set_state(glass_hip_ucb_sampler *sampler, const unsign
 /* The F-statistic proposal density of window g (setup_fstatistic_proposal,
  * ucb/src/glass_ucb_proposal.c:916-1084): host tensor [n_f][n_theta][n_phi]
  * (normalised, e.g. from glass_hip_fstat_proposal), cell width d_f in Hz,
- * its maximum maxp, and the window's first bin data->qmin. */
+ * its maximum maxp, and the window's first bin data->qmin.  Refused
+ * (GLASS_HIP_EINVAL, checked on the host) unless every cell is finite and
+ * >= 0, the maximum is > 0 and maxp is at most that maximum: the device's
+ * rejection sampler could otherwise never accept.  A draw that still finds
+ * no cell in 2^24 tries leaves the proposal's f0 and sky unchanged and is
+ * reported by glass_hip_ucb_sampler_get_state (GLASS_HIP_ERUNTIME). */
 int glass_hip_ucb_sampler_set_fstat(glass_hip_ucb_sampler *sampler, int g, const double *tensor, int n_f, double d_f,
                                     int n_theta, int n_phi, double maxp, int data_qmin, void *stream);
 
@@ -99,9 +104,21 @@ int glass_hip_ucb_sampler_sweep(glass_hip_ucb_sampler *sampler, int nsteps, int 
 
 /* Any pointer may be NULL.  Per slot: seeds, index (slot -> chain),
  * temperature, trials[S][9] / accepts[S][9] per proposal type, swaps
- * (accepted ptmcmc swaps with the slot above). */
+ * (accepted ptmcmc swaps with the slot above).  Returns GLASS_HIP_ERUNTIME
+ * (after copying) if an F-statistic draw gave up since set_state. */
 int glass_hip_ucb_sampler_get_state(glass_hip_ucb_sampler *sampler, unsigned *seeds, int *index, double *temperature,
                                     int *trials, int *accepts, int *swaps, void *stream);
+
+/* save_chain_state (ucb/src/glass_ucb_io.c:418-446) of every ladder, each
+ * ladder being one run of the reference: dirs[l]/chain_state_<t>.dat holds
+ * the chain now at temperature t of ladder l (slot l*ntemp + t through the
+ * slot -> chain map that ptmcmc swaps permute), with logLmax[l] (NULL: 0),
+ * that chain's logLnorm (NULL: 0), t0 and eta[neta].  The reference's
+ * restore reads file t back into temperature t with the map reset to the
+ * identity, which glass_hip_ucb_sampler_set_state does.  Synchronises. */
+int glass_hip_ucb_sampler_save_chain_states(glass_hip_ucb_sampler *sampler, const char *const *dirs, int step,
+                                            const double *logLmax, const double *logLnorm, double t0,
+                                            const double *eta, int neta, void *stream);
 
 #ifdef __cplusplus
 }

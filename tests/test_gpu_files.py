@@ -46,3 +46,38 @@ def test_save_and_restore_chain_states(gl, orc, tmp_path):
     rel = np.max(np.abs(logl2 - logl) / np.maximum(np.abs(logl), 1.0))
     print("restored logL rel", rel)
     assert rel < 1e-6
+
+
+def test_sampler_checkpoint_files_follow_temperature(gl, orc, tmp_path):
+    """After ptmcmc swaps have permuted the slot -> chain map, the sampler's
+    checkpoint writes file t of ladder l from the chain now at temperature t
+    (save_chain_state writes model[chain->index[ic]], glass_ucb_io.c:418-446)."""
+    from glass_amd import files
+    from test_gpu_sampler import setup_sampler
+    cfg, plan, smp, pr, seeds, temps = setup_sampler(gl, orc, seed=61, ngroup=3, ntemp=4)
+    smp.fisher()
+    K = smp.ntemp
+    for _ in range(12):
+        smp.sweep(nsteps=4, adapt=False)
+        st = smp.state()
+        if not np.array_equal(st["index"], np.arange(smp.nslot)):
+            break
+    idx = st["index"]
+    assert not np.array_equal(idx, np.arange(smp.nslot)), "no swap happened"
+    logl, sp = plan.state()
+    dirs = [tmp_path / f"ladder{l}" for l in range(smp.nladder)]
+    for d in dirs:
+        d.mkdir()
+    lmax = np.arange(smp.nladder, dtype=float)
+    smp.save_chain_states([str(d) for d in dirs], step=3, logLmax=lmax, t0=0.0, eta=[1.0, 1.0, 1.0])
+    T = cfg["T"]
+    for l in range(smp.nladder):
+        for t in range(K):
+            c = int(idx[l * K + t])
+            r = files.restore_chain_state(dirs[l] / f"chain_state_{t}.dat", T, 3, maxsrc=sp.shape[1])
+            n = int(cfg["chain_nsrc"][c])
+            assert r["nlive"] == n and r["logLmax"] == lmax[l]
+            assert r["logL"] == float("%g" % logl[c])
+            assert np.all(np.abs(r["params"][:, 0] - sp[c, :n, 0]) <= 1e-15 * np.abs(sp[c, :n, 0]))
+    smp.close()
+    plan.close()
