@@ -29,6 +29,14 @@ class Orbit(ctypes.Structure):
     _fields_ = [("L", ctypes.c_double), ("fstar", ctypes.c_double), ("R", ctypes.c_double)]
 
 
+class NumericOrbit(ctypes.Structure):
+    """glass_hip_numeric_orbit: an orbit read from a file (knots t [Norb],
+    positions x, y, z [3, Norb] each, average arm length L, fstar)."""
+    _fields_ = [("Norb", ctypes.c_int), ("t", ctypes.POINTER(ctypes.c_double)),
+                ("x", ctypes.POINTER(ctypes.c_double)), ("y", ctypes.POINTER(ctypes.c_double)),
+                ("z", ctypes.POINTER(ctypes.c_double)), ("L", ctypes.c_double), ("fstar", ctypes.c_double)]
+
+
 _lib = None
 
 
@@ -40,6 +48,9 @@ def _declare(L):
     L.glass_hip_last_error.restype = ctypes.c_char_p
     L.glass_hip_ucb_waveform.argtypes = [ctypes.POINTER(Orbit), ctypes.c_char_p, d, d, _dp, i,
                                          _dp, _dp, _dp, _dp, _dp, i, i]
+    L.glass_hip_ucb_waveform_numeric.argtypes = [ctypes.POINTER(NumericOrbit), ctypes.c_char_p, d, d, _dp, i,
+                                                 _dp, _dp, _dp, _dp, _dp, i, i]
+    L.glass_hip_numeric_orbit_armlength.argtypes = [i, _dp, _dp, _dp, _dp, _dp]
     L.glass_hip_fourier_nwip.argtypes = [_dp, _dp, _dp, i]
     L.glass_hip_fourier_nwip.restype = d
     L.glass_hip_wavelet_nwip.argtypes = [_dp, _dp, _dp, _ip, i, i]
@@ -133,6 +144,33 @@ def ucb_waveform(orbit, format, T, t0, params, NParams, BW, NI=3):
                                       _d(p), int(NParams), *[_d(out[c]) for c in range(5)],
                                       int(BW), int(NI))
     check(rc, "ucb_waveform")
+    return out.view(np.complex128)
+
+
+def numeric_orbit(t, x, y, z):
+    """A NumericOrbit over host arrays (kept alive on the returned object):
+    knots t [Norb], positions x, y, z [3, Norb] (row i = spacecraft i); L
+    and fstar as initialize_numeric_orbit computes them."""
+    t, x, y, z = (np.ascontiguousarray(a, dtype=np.float64) for a in (t, x, y, z))
+    N = len(t)
+    if x.shape != (3, N) or y.shape != (3, N) or z.shape != (3, N):
+        raise ValueError("x, y, z must be [3, Norb]")
+    Lv, fs = ctypes.c_double(), ctypes.c_double()
+    check(lib().glass_hip_numeric_orbit_armlength(N, _d(x), _d(y), _d(z), ctypes.byref(Lv), ctypes.byref(fs)),
+          "numeric_orbit_armlength")
+    o = NumericOrbit(N, _d(t), _d(x), _d(y), _d(z), Lv.value, fs.value)
+    o._keep = (t, x, y, z)
+    return o
+
+
+def ucb_waveform_numeric(orbit, format, T, t0, params, NParams, BW, NI=3):
+    """ucb_waveform with a numeric orbit (interpolate_orbits); as ucb_waveform."""
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    out = np.zeros((5, 2 * BW))
+    rc = lib().glass_hip_ucb_waveform_numeric(ctypes.byref(orbit), format.encode(), float(T), float(t0),
+                                              _d(p), int(NParams), *[_d(out[c]) for c in range(5)],
+                                              int(BW), int(NI))
+    check(rc, "ucb_waveform_numeric")
     return out.view(np.complex128)
 
 

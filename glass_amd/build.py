@@ -12,7 +12,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libglass_hip.so")
-SOURCES = ["runtime.hip", "fd_plain.hip", "nwip.hip", "fd_batch.hip", "fisher_fd.hip", "fd_derivs.hip",
+SOURCES = ["runtime.hip", "fd_plain.hip", "nwip.hip", "fd_batch.hip", "fisher_fd.hip", "fd_derivs.hip", "fd_adjoint.hip",
            "wdm_ucb.hip", "wdm_transform.hip", "logl_batch.hip", "noise.hip",
            "eigen.hip", "ucb_sampler.hip", "fstat.hip", "ucb_io.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

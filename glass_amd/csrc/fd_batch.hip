@@ -452,9 +452,9 @@ using namespace glass;
 
 // Exact derivatives of chain sources at their current parameters: per-item
 // arguments of the derivative kernels (fd_derivs.h) and BW-class lists
-// (BW 32..2048, classes 0..6).  Items outside those (another BW, a chain or
+// (BW 32..4096, classes 0..7).  Items outside those (another BW, a chain or
 // source index out of range) get NaN outputs.
-constexpr int kNDClass = 7;
+constexpr int kNDClass = 8;
 
 struct DerivItems
 {
@@ -489,7 +489,7 @@ __global__ __launch_bounds__(256) void derivs_prep_kernel(int n, const int* __re
             D.toff[e] = B.c_toff[c] + (long long)s * 3 * B.g_cap[g];
             D.tcap[e] = B.g_cap[g];
             const int bw = B.s_bw[so];
-            if (bw >= 32 && bw <= 2048 && (bw & (bw - 1)) == 0) k = 31 - __clz(bw) - 5;
+            if (bw >= 32 && bw <= 4096 && (bw & (bw - 1)) == 0) k = 31 - __clz(bw) - 5;
         }
         slot = atomicAdd(&hist[k], 1);
     }
@@ -530,6 +530,7 @@ struct glass_hip_fd_batch
     int maxsrc = kMaxSrcDefault;
     size_t alloc_bytes = 0;   // device memory the plan holds
     double2* d2048 = nullptr; // BW-2048 derivative scratch, one slice per CU
+    double2* d4096 = nullptr; // BW-4096 derivative scratch, one slice per CU
     DerivItems D{};
     int ncu = 0;
     int tgrid[kNClass] = {0};
@@ -956,6 +957,7 @@ extern "C" int glass_hip_fd_batch_create_ex(glass_hip_fd_batch** plan, const gla
         return fail(GLASS_HIP_ERUNTIME);
     }
     if (P->B.bwcap >= 2048 && (rc = dalloc(P, &P->d2048, (size_t)P->ncu * kDeriv2048Scratch))) return fail(rc);
+    if (P->B.bwcap >= 4096 && (rc = dalloc(P, &P->d4096, (size_t)P->ncu * kDeriv4096Scratch))) return fail(rc);
     if ((rc = rebuild(P, cx->stream)) || (rc = check_err(P, cx->stream))) return fail(rc);
     *plan = P;
     return GLASS_HIP_OK;
@@ -1459,6 +1461,10 @@ extern "C" int glass_hip_fd_batch_derivs(glass_hip_fd_batch* P, int n, const int
         {
             // BW 2048 only occurs when some window allows it (then d2048 exists)
             if (P->d2048) lrc = launch_derivs_2048(A, std::min(n, P->ncu), P->d2048, sk);
+        }
+        else if (32 << k == 4096)
+        {
+            if (P->d4096) lrc = launch_derivs_4096(A, std::min(n, P->ncu), P->d4096, sk);
         }
         else
             lrc = launch_derivs(32 << k, A, n, sk);

@@ -30,8 +30,10 @@ struct DerivArgs
     const long long* taddoff;
     const int* taddcap;
     // optional class list: workgroup w takes item list[w] while w < *count
+    // (without a list the grid-striding kernels take items w < nitems)
     const int* list;
     const int* count;
+    int nitems;
     int mode;           // 0: window-aligned weights; 1: template-local (reference)
     double T, t0, L, fstar;
     const double* pos;
@@ -45,10 +47,14 @@ struct DerivArgs
 // deriv_kernel<N> over grid workgroups (N a power of two, 32..1024).
 int launch_derivs(int N, const DerivArgs& A, int grid, hipStream_t s);
 
-// The BW 2048 kernel: grid workgroups striding over A.list (required),
-// each with a scratch slice of kDeriv2048Scratch complex at hs + w slice.
+// The BW 2048 and 4096 kernels: grid workgroups striding over A.list (or
+// over items 0..A.nitems-1 without a list), workgroup w with a scratch
+// slice of deriv_scratch(N) complex at hs + w slice.
 constexpr int kDeriv2048Threads = 512;
 constexpr size_t kDeriv2048Scratch = 18 * 2048;
+constexpr size_t kDeriv4096Scratch = 24 * 4096;
+constexpr size_t deriv_scratch(int N) { return N == 2048 ? kDeriv2048Scratch : (N == 4096 ? kDeriv4096Scratch : 0); }
 int launch_derivs_2048(const DerivArgs& A, int grid, double2* hs, hipStream_t s);
+int launch_derivs_4096(const DerivArgs& A, int grid, double2* hs, hipStream_t s);
 
 }  // namespace glass

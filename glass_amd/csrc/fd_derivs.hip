@@ -332,13 +332,13 @@ __global__ __launch_bounds__(NTHR) void deriv_kernel_2048(DerivArgs A, double2* 
     __shared__ UcbDiff sdd;
     __shared__ double red[kNSum];
     double2* H = hs + (size_t)blockIdx.x * 18 * N;
-    const int cnt = *A.count;
+    const int cnt = A.list ? *A.count : A.nitems;
     const int t = threadIdx.x;
     const int twstride = kTwN / N;
     const double inv2N = 1. / (double)(2 * N);
     for (int w = blockIdx.x; w < cnt; w += gridDim.x)
     {
-        const int e = A.list[w];
+        const int e = A.list ? A.list[w] : w;
         const double* p = A.params + (size_t)(A.pidx ? A.pidx[e] : e) * 8;
         const UcbConst& c = ucb_setup_shared(p, 8, A.T, A.t0, A.L, A.fstar);
         if (t == 0) ucb_diff_setup(p, A.T, sdd);
@@ -430,12 +430,135 @@ __global__ __launch_bounds__(NTHR) void deriv_kernel_2048(DerivArgs A, double2* 
 int launch_derivs_2048(const DerivArgs& A, int grid, double2* hs, hipStream_t s)
 {
     if (grid <= 0) return GLASS_HIP_OK;
-    if (!A.list || !A.count || !hs)
+    if ((!A.list && A.nitems <= 0) || (A.list && !A.count) || !hs)
     {
-        set_error("fd derivs: BW 2048 needs a class list and scratch");
+        set_error("fd derivs: BW 2048 needs a class list or an item count, and scratch");
         return GLASS_HIP_EINVAL;
     }
     deriv_kernel_2048<kDeriv2048Threads><<<grid, kDeriv2048Threads, 0, s>>>(A, hs);
+    GH_CHECK(hipGetLastError());
+    return GLASS_HIP_OK;
+}
+
+// N = 4096: the 2048 kernel's decimation in time with P = 4 phases of
+// 1024-point sub-series, but the P x 6 link accumulators of a thread's
+// sub-bins (four times the 2048 kernel's, past any register budget) live in
+// the workgroup's global scratch slice: acc[l][s] (6 x N complex), summed
+// over the phases in order, then TDI'd into H[set][ch][m] (18 x N).  Each
+// thread reads back only the accumulators and bins it wrote (FFT index
+// s = j + NS r and bin m = (s + N/2) mod N keep j mod NTHR), so the slice
+// needs no barrier.
+template <int NTHR>
+__global__ __launch_bounds__(NTHR) void deriv_kernel_4096(DerivArgs A, double2* __restrict__ hs)
+{
+    constexpr int N = 4096, NS = 1024, P = N / NS, U = NS / NTHR, PER = N / NTHR;
+    static_assert(NS % NTHR == 0 && (N / 2) % NTHR == 0, "decimated shape");
+    __shared__ double2 lds[6 * NS];
+    __shared__ UcbDiff sdd;
+    __shared__ double red[kNSum];
+    double2* Acc = hs + (size_t)blockIdx.x * kDeriv4096Scratch;
+    double2* H = Acc + 6 * N;
+    const int cnt = A.list ? *A.count : A.nitems;
+    const int t = threadIdx.x;
+    const int twstride = kTwN / N;
+    const double inv2N = 1. / (double)(2 * N);
+    for (int w = blockIdx.x; w < cnt; w += gridDim.x)
+    {
+        const int e = A.list ? A.list[w] : w;
+        const double* p = A.params + (size_t)(A.pidx ? A.pidx[e] : e) * 8;
+        const UcbConst& c = ucb_setup_shared(p, 8, A.T, A.t0, A.L, A.fstar);
+        if (t == 0) ucb_diff_setup(p, A.T, sdd);
+        __syncthreads();
+        const UcbDiff& dd = sdd;
+        auto one_set = [&](auto set_tag) {
+            constexpr int S = decltype(set_tag)::value;
+#pragma unroll 1
+            for (int ph = 0; ph < P; ph++)
+            {
+                for (int j = t; j < NS; j += NTHR)
+                {
+                    const int n = P * j + ph;
+                    const double ts = ucb_sample_time(c.T, c.t0, n, N);
+                    ucb_sample_sets<S, 1>(c, dd, ts, A.pos + (size_t)n * (4096 / N) * 9, lds, NS, j);
+                }
+                __syncthreads();
+                lds_fft<NS, 6, NTHR>(lds, A.tw, twstride * P);
+#pragma unroll
+                for (int u = 0; u < U; u++)
+                {
+                    const int j = t + u * NTHR;
+                    const double2 wj = A.tw[(ph * j) * twstride];
+#pragma unroll
+                    for (int l = 0; l < 6; l++)
+                    {
+                        const double2 v = cmul(lds[l * NS + j], wj);
+#pragma unroll
+                        for (int r = 0; r < P; r++)
+                        {
+                            const double2 q = cmul(v, A.tw[((ph * r) % P) * NS * twstride]);
+                            double2* a = Acc + (size_t)l * N + j + NS * r;
+                            if (ph == 0)
+                                *a = q;
+                            else
+                                *a = make_double2(a->x + q.x, a->y + q.y);
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+#pragma unroll 1
+            for (int u = 0; u < U; u++)
+#pragma unroll 1
+                for (int r = 0; r < P; r++)
+                {
+                    const int sidx = t + u * NTHR + NS * r;
+                    const int m = (sidx + N / 2) & (N - 1);
+                    double2 d6[6], o5[5];
+#pragma unroll
+                    for (int l = 0; l < 6; l++)
+                    {
+                        const double2 a = Acc[(size_t)l * N + sidx];
+                        d6[l] = make_double2(a.x * inv2N, a.y * inv2N);
+                    }
+                    fd_tdi<N>(c, m, A.fmt, d6, o5);
+#pragma unroll
+                    for (int ch = 0; ch < 3; ch++) H[(size_t)(S * 3 + ch) * N + m] = o5[ch];
+                }
+        };
+        one_set(std::integral_constant<int, 0>{});
+        one_set(std::integral_constant<int, 1>{});
+        one_set(std::integral_constant<int, 2>{});
+        one_set(std::integral_constant<int, 3>{});
+        one_set(std::integral_constant<int, 4>{});
+        one_set(std::integral_constant<int, 5>{});
+        DerivItem it = deriv_item(A, e);
+        double acc[kNSum];
+#pragma unroll
+        for (int q = 0; q < kNSum; q++) acc[q] = 0.0;
+#pragma unroll 1
+        for (int k = 0; k < PER; k++)
+        {
+            const int m = t + k * NTHR;
+            double2 Hb[6][3];
+#pragma unroll
+            for (int S = 0; S < 6; S++)
+#pragma unroll
+                for (int ch = 0; ch < 3; ch++) Hb[S][ch] = H[(size_t)(S * 3 + ch) * N + m];
+            deriv_bin(c, dd, A, it, m, Hb, acc);
+        }
+        deriv_write<NTHR>(A, e, acc, red);
+    }
+}
+
+int launch_derivs_4096(const DerivArgs& A, int grid, double2* hs, hipStream_t s)
+{
+    if (grid <= 0) return GLASS_HIP_OK;
+    if ((!A.list && A.nitems <= 0) || (A.list && !A.count) || !hs)
+    {
+        set_error("fd derivs: BW 4096 needs a class list or an item count, and scratch");
+        return GLASS_HIP_EINVAL;
+    }
+    deriv_kernel_4096<kDeriv2048Threads><<<grid, kDeriv2048Threads, 0, s>>>(A, hs);
     GH_CHECK(hipGetLastError());
     return GLASS_HIP_OK;
 }
@@ -503,5 +626,22 @@ extern "C" int glass_hip_fd_source_derivs(const glass_hip_orbit* orbit, const ch
     A.outL = d_logL;
     A.grad = d_grad;
     A.fisher = d_fisher;
-    return launch_derivs(BW, A, n, (hipStream_t)stream);
+    if (BW <= 1024) return launch_derivs(BW, A, n, (hipStream_t)stream);
+    if (BW != 2048 && BW != 4096)
+    {
+        set_error("fd derivs: BW=%d unsupported (power of two 32..4096)", BW);
+        return GLASS_HIP_EINVAL;
+    }
+    // the decimated kernels stride over the items with a scratch slice per
+    // workgroup (one workgroup per CU at most), held by the calling thread
+    ThreadCtx* cx;
+    if ((rc = ctx_get(&cx))) return rc;
+    int ncu = 0;
+    GH_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, cx->device));
+    const int grid = n < ncu ? n : ncu;
+    if ((rc = cx->dscr.ensure(sizeof(double2) * deriv_scratch(BW) * (size_t)grid))) return rc;
+    A.nitems = n;
+    double2* hs = (double2*)cx->dscr.p;
+    return BW == 2048 ? launch_derivs_2048(A, grid, hs, (hipStream_t)stream)
+                      : launch_derivs_4096(A, grid, hs, (hipStream_t)stream);
 }

@@ -37,7 +37,12 @@ __host__ __device__ inline void ucb_diff_setup(const double* p, double T, UcbDif
     const double costh = p[1], phi = p[2], amp = exp(p[3]), cosi = p[4], psi = p[5];
     const double sinth = sqrt(1.0 - costh * costh);
     const double cosph = cos(phi), sinph = sin(phi);
-    const double cot = costh / sinth;  // d sin(theta)/d cos(theta) = -cot
+    // d sin(theta)/d cos(theta) = -cot diverges at the poles (|cos theta| =
+    // 1, where clipped proposals land); there the tangent is taken as at
+    // |cos theta| = 1 - 1e-6, the reference's Fisher step
+    // (ucb/src/glass_ucb_waveform.c:369-415), so the Fisher matrix stays
+    // finite (sin theta >= sqrt(2e-6) in the cotangent only)
+    const double cot = costh / fmax(sinth, 1.4142135623730951e-3);
     d.u[0] = costh * cosph;  d.u[1] = costh * sinph;  d.u[2] = -sinth;
     d.v[0] = sinph;          d.v[1] = -cosph;         d.v[2] = 0.0;
     d.du1[0] = cosph;        d.du1[1] = sinph;        d.du1[2] = cot;
@@ -88,11 +93,13 @@ __device__ inline void sinc_ratio_d(double x, double* f, double* df)
 
 __device__ inline double2 cmulc(double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 
-// Series sets [S0, S0+NS) of one slow sample at time t into
-// lds[(set - S0)*6*stride + link*stride + j].
-template <int S0, int NS>
-__device__ inline void ucb_sample_sets(const UcbConst& c, const UcbDiff& dd, double t, const double* __restrict__ pos,
-                                       double2* lds, int stride, int j)
+// Series sets [S0, S0+NS) of one slow sample at time t: out(set, link,
+// value) for every set of the range and link 0..5 (12, 21, 31, 13, 23, 32).
+// Sets 0, 1: the plus / cross basis (the sample is DP s0 + DC s1, complex
+// products); 2..5: d/d(f0 T), d/dcos(theta), d/dphi, d/d(fdot T^2).
+template <int S0, int NS, typename Out>
+__device__ inline void ucb_sample_sets_to(const UcbConst& c, const UcbDiff& dd, double t,
+                                          const double* __restrict__ pos, Out&& out)
 {
     constexpr bool W01 = S0 <= 1 && S0 + NS > 0;  // any basis set
     constexpr bool WT = S0 + NS > 2;              // any tangent set
@@ -177,7 +184,7 @@ __device__ inline void ucb_sample_sets(const UcbConst& c, const UcbDiff& dd, dou
                 {
                     if (q < S0 || q >= S0 + NS) continue;
                     const double w = S * ae * (q == 0 ? dp : dc);
-                    lds[(q - S0) * 6 * stride + link * stride + j] = make_double2(w * E.x, w * E.y);
+                    out(q, link, make_double2(w * E.x, w * E.y));
                 }
             }
             if constexpr (WT)
@@ -204,12 +211,20 @@ __device__ inline void ucb_sample_sets(const UcbConst& c, const UcbDiff& dd, dou
                                                      dae[v] * (dp * DP.y + dc * DC.y) + ae * (ddp * DP.y + ddc * DC.y));
                     // E (dS T1 + S dT1) - i dPhi S T1 E
                     const double2 g = cmulc(make_double2(dS * T1.x + S * dT1.x, dS * T1.y + S * dT1.y), E);
-                    lds[(q - S0) * 6 * stride + link * stride + j] =
-                        make_double2(g.x + dPhi * ST1E.y, g.y - dPhi * ST1E.x);
+                    out(q, link, make_double2(g.x + dPhi * ST1E.y, g.y - dPhi * ST1E.x));
                 }
             }
         }
     }
+}
+
+// The sets [S0, S0+NS) of one sample into lds[(set - S0)*6*stride + link*stride + j].
+template <int S0, int NS>
+__device__ inline void ucb_sample_sets(const UcbConst& c, const UcbDiff& dd, double t, const double* __restrict__ pos,
+                                       double2* lds, int stride, int j)
+{
+    ucb_sample_sets_to<S0, NS>(c, dd, t, pos,
+                               [&](int q, int link, double2 v) { lds[(q - S0) * 6 * stride + link * stride + j] = v; });
 }
 
 }  // namespace glass

@@ -11,7 +11,7 @@
 // suffices (96*N bytes for the six link series of a template).
 //
 // Twiddles come from a table tw[k] = exp(-2 pi i k / NTW) of NTW entries
-// (NTW a multiple of N: 4096 for N = 2^a, 12288 for N = 3*2^a) built once
+// (NTW a multiple of N: 4096 or 16384 for N = 2^a, 12288 for N = 3*2^a) built once
 // per device with the phase formed as -2*pi*k/NTW, the kiss_fft twiddle
 // formula.
 #pragma once
@@ -22,6 +22,43 @@ namespace glass {
 __device__ inline double2 cmul(double2 a, double2 b)
 {
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// Radix-R butterfly (R = 2, 3, 4) of twiddled inputs v into y:
+// y[m] = sum_q v[q] exp(-2 pi i q m / R).
+template <int R>
+__device__ inline void radix_butterfly(const double2 (&v)[R], double2 (&y)[R])
+{
+    if constexpr (R == 3)
+    {
+        // w = exp(-2 pi i/3): y1 = a0 - s/2 - i(sqrt3/2) d, y2 = conj rotation
+        const double h3 = 0.86602540378443864676;
+        double2 a0 = v[0], a1 = v[1], a2 = v[2];
+        double2 s12 = make_double2(a1.x + a2.x, a1.y + a2.y);
+        double2 d12 = make_double2(a1.x - a2.x, a1.y - a2.y);
+        double2 t = make_double2(a0.x - 0.5 * s12.x, a0.y - 0.5 * s12.y);
+        y[0] = make_double2(a0.x + s12.x, a0.y + s12.y);
+        y[1] = make_double2(t.x + h3 * d12.y, t.y - h3 * d12.x);
+        y[2] = make_double2(t.x - h3 * d12.y, t.y + h3 * d12.x);
+    }
+    else if constexpr (R == 4)
+    {
+        double2 a0 = v[0], a1 = v[1], a2 = v[2], a3 = v[3];
+        double2 s02 = make_double2(a0.x + a2.x, a0.y + a2.y);
+        double2 d02 = make_double2(a0.x - a2.x, a0.y - a2.y);
+        double2 s13 = make_double2(a1.x + a3.x, a1.y + a3.y);
+        double2 d13 = make_double2(a1.x - a3.x, a1.y - a3.y);
+        y[0] = make_double2(s02.x + s13.x, s02.y + s13.y);
+        y[2] = make_double2(s02.x - s13.x, s02.y - s13.y);
+        // -i * d13 = (d13.y, -d13.x)
+        y[1] = make_double2(d02.x + d13.y, d02.y - d13.x);
+        y[3] = make_double2(d02.x - d13.y, d02.y + d13.x);
+    }
+    else
+    {
+        y[0] = make_double2(v[0].x + v[1].x, v[0].y + v[1].y);
+        y[1] = make_double2(v[0].x - v[1].x, v[0].y - v[1].y);
+    }
 }
 
 // One Stockham stage of radix R over NL transforms; butterflies are spread
@@ -54,41 +91,15 @@ __device__ inline void stockham_stage(double2* buf, int Ns, const double2* __res
         {
             int l = b / (N / R), j = b % (N / R);
             int k = j % Ns;
-            // twiddle exp(-2 pi i m k /(R Ns)) = tw[m*k*(NTW/(R*Ns))]
+            // twiddle exp(-2 pi i m k /(R Ns)) = tw[m*k*(NTW/(R*Ns))]; the
+            // first stage (Ns = 1, a constant once inlined) has k = 0, all
+            // twiddles 1, and skips the loads and products
             int step = k * twstride * (N / (R * Ns));
+            if (Ns > 1)
 #pragma unroll
-            for (int m = 1; m < R; m++) v[p][m] = cmul(v[p][m], tw[m * step]);
+                for (int m = 1; m < R; m++) v[p][m] = cmul(v[p][m], tw[m * step]);
             double2 y[R];
-            if constexpr (R == 3)
-            {
-                // w = exp(-2 pi i/3): y1 = a0 - s/2 - i(sqrt3/2) d, y2 = conj rotation
-                const double h3 = 0.86602540378443864676;
-                double2 a0 = v[p][0], a1 = v[p][1], a2 = v[p][2];
-                double2 s12 = make_double2(a1.x + a2.x, a1.y + a2.y);
-                double2 d12 = make_double2(a1.x - a2.x, a1.y - a2.y);
-                double2 t = make_double2(a0.x - 0.5 * s12.x, a0.y - 0.5 * s12.y);
-                y[0] = make_double2(a0.x + s12.x, a0.y + s12.y);
-                y[1] = make_double2(t.x + h3 * d12.y, t.y - h3 * d12.x);
-                y[2] = make_double2(t.x - h3 * d12.y, t.y + h3 * d12.x);
-            }
-            else if constexpr (R == 4)
-            {
-                double2 a0 = v[p][0], a1 = v[p][1], a2 = v[p][2], a3 = v[p][3];
-                double2 s02 = make_double2(a0.x + a2.x, a0.y + a2.y);
-                double2 d02 = make_double2(a0.x - a2.x, a0.y - a2.y);
-                double2 s13 = make_double2(a1.x + a3.x, a1.y + a3.y);
-                double2 d13 = make_double2(a1.x - a3.x, a1.y - a3.y);
-                y[0] = make_double2(s02.x + s13.x, s02.y + s13.y);
-                y[2] = make_double2(s02.x - s13.x, s02.y - s13.y);
-                // -i * d13 = (d13.y, -d13.x)
-                y[1] = make_double2(d02.x + d13.y, d02.y - d13.x);
-                y[3] = make_double2(d02.x - d13.y, d02.y + d13.x);
-            }
-            else
-            {
-                y[0] = make_double2(v[p][0].x + v[p][1].x, v[p][0].y + v[p][1].y);
-                y[1] = make_double2(v[p][0].x - v[p][1].x, v[p][0].y - v[p][1].y);
-            }
+            radix_butterfly<R>(v[p], y);
             double2* dst = buf + l * N;
             int base = (j / Ns) * Ns * R + k;
 #pragma unroll
@@ -123,6 +134,67 @@ __device__ inline void lds_fft(double2* buf, const double2* __restrict__ tw, int
         stockham_stage<N, NL, 4, NTHR>(buf, Ns, tw, twstride);
         Ns *= 4;
     }
+}
+
+// One Stockham stage of radix R from src to dst (two buffers, any memory:
+// transforms too long for LDS run through a workgroup's global scratch).
+// Same butterflies, twiddles and order as stockham_stage, so the results
+// are those of lds_fft.  Ends with a barrier (the workgroup's writes are
+// then visible to all of its threads).
+template <int N, int R, int NTHR>
+__device__ inline void pingpong_stage(const double2* src, double2* dst, int Ns, const double2* __restrict__ tw,
+                                      int twstride)
+{
+    for (int j = threadIdx.x; j < N / R; j += NTHR)
+    {
+        double2 v[R];
+#pragma unroll
+        for (int m = 0; m < R; m++) v[m] = src[j + m * (N / R)];
+        const int k = j % Ns;
+        const int step = k * twstride * (N / (R * Ns));
+        if (Ns > 1)
+#pragma unroll
+            for (int m = 1; m < R; m++) v[m] = cmul(v[m], tw[m * step]);
+        double2 y[R];
+        radix_butterfly<R>(v, y);
+        const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+        for (int m = 0; m < R; m++) dst[base + m * Ns] = y[m];
+    }
+    __syncthreads();
+}
+
+// Forward FFT of one length-N series between buffers a and b (the stage
+// schedule of lds_fft); returns the buffer that holds the result.
+template <int N, int NTHR>
+__device__ inline double2* pingpong_fft(double2* a, double2* b, const double2* __restrict__ tw, int twstride)
+{
+    constexpr bool R3 = N % 3 == 0;
+    constexpr int P2 = R3 ? N / 3 : N;
+    static_assert((P2 & (P2 - 1)) == 0 && N >= 4, "N = 2^a or 3*2^a");
+    constexpr int LOG2 = __builtin_ctz(P2);
+    int Ns = 1;
+    double2 *src = a, *dst = b;
+    if (R3)
+    {
+        pingpong_stage<N, 3, NTHR>(src, dst, Ns, tw, twstride);
+        Ns *= 3;
+        double2* t = src; src = dst; dst = t;
+    }
+    if (LOG2 & 1)
+    {
+        pingpong_stage<N, 2, NTHR>(src, dst, Ns, tw, twstride);
+        Ns *= 2;
+        double2* t = src; src = dst; dst = t;
+    }
+#pragma unroll 1
+    for (int s = 0; s < LOG2 / 2; s++)
+    {
+        pingpong_stage<N, 4, NTHR>(src, dst, Ns, tw, twstride);
+        Ns *= 4;
+        double2* t = src; src = dst; dst = t;
+    }
+    return src;
 }
 
 }  // namespace glass

@@ -52,6 +52,7 @@ struct ThreadCtx
     DevBuf dbuf;
     HostBuf hbuf;
     DevBuf dpart;  // reduction partials
+    DevBuf dscr;   // per-workgroup scratch of the BW 2048/4096 derivative kernels
     ~ThreadCtx();
 };
 
@@ -63,6 +64,9 @@ int twiddles(const double2** out);
 // Same for the sizes N = 3*2^a: tw3[k] = exp(-2 pi i k / kTw3N).
 constexpr int kTw3N = 12288;
 int twiddles3(const double2** out);
+// And for the long WDM template series, N = 2^a up to 16384.
+constexpr int kTw16N = 16384;
+int twiddles16(const double2** out);
 
 // Spacecraft positions of the analytic orbit at the npos sample times
 // t0 + T*n/npos (npos a power of two), 9 doubles per sample

@@ -25,6 +25,7 @@
 #include "../../include/glass_hip_wdm.h"
 #include "fastmath.h"
 #include "fft_lds.h"
+#include "reduce.h"
 #include "runtime.h"
 
 namespace glass {
@@ -369,11 +370,123 @@ static int forward_dev(const Plan& p, double A, const double* d_time, double* d_
     return layers_fwd(p, Z, true, win, d_pix, s);
 }
 
+// ------------------------------------- stationary PSD -> WDM pixel PSD --
+//
+// stationary_dft_psd_to_wdm_layer_var (utils/src/glass_wavelet.c:865-899):
+// the variance of a WDM pixel of layer m for stationary noise whose DFT
+// variance is dft_psd[k] (k = 0..ND/2), as the window-weighted sum over the
+// NT - 1 DFT bins of the layer's band, k = m NT/2 + l folded into [0, ND/2];
+// the edge layers 0 and NF see one side of the band and their l = 0 weight
+// is quartered.  One workgroup per layer (a fixed-order tree sum of
+// non-negative terms: within rounding of the reference's running sum), then
+// stationary_dft_psd_to_wdm_psd (:901-911) spreads the layer variances over
+// the NF x NT pixels (row 0: even n layer 0, odd n layer NF).
+constexpr int kPsdThreads = 256;
+
+__global__ void __launch_bounds__(kPsdThreads) layer_var_kernel(int NF, int NT, const double* __restrict__ win,
+                                                                const double* __restrict__ psd,
+                                                                double* __restrict__ lv)
+{
+    const int m = blockIdx.x;
+    const int half = NT / 2;
+    const long long nmax = (long long)NF * NT / 2;
+    double acc = 0.0;
+    for (int l = -(half - 1) + (int)threadIdx.x; l < half; l += kPsdThreads)
+    {
+        if ((m == 0 && l < 0) || (m == NF && l > 0)) continue;
+        const int al = l < 0 ? -l : l;
+        double wt = win[al] * win[al];
+        if (l == 0 && (m == 0 || m == NF)) wt *= 0.25;
+        long long k = (long long)m * half + l;
+        k = (k < 0 ? -k : k) % (2 * nmax);
+        if (k > nmax) k = 2 * nmax - k;
+        acc += wt * psd[k];
+    }
+    acc = block_sum<kPsdThreads>(acc);
+    if (threadIdx.x == 0) lv[m] = (m > 0 && m < NF) ? acc / (2.0 * NT * NT) : acc / ((double)NT * NT);
+}
+
+__global__ void wdm_psd_fill_kernel(int NF, int NT, const double* __restrict__ lv, double* __restrict__ out)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)NF * NT) return;
+    const int m = (int)(i / NT), n = (int)(i % NT);
+    out[i] = m > 0 ? lv[m] : ((n & 1) ? lv[NF] : lv[0]);
+}
+
+static int psd_check(const glass_hip_wdm* w)
+{
+    if (!w) return set_error("NULL wdm"), GLASS_HIP_EINVAL;
+    if (w->NF < 1 || w->NT < 2 || (w->NT & 1))
+        return set_error("wdm psd: NF = %d, NT = %d (NT even >= 2)", w->NF, w->NT), GLASS_HIP_EINVAL;
+    return GLASS_HIP_OK;
+}
+
+static int wdm_psd_dev(const glass_hip_wdm* w, const double* d_psd, double* d_lv, double* d_pix, hipStream_t s)
+{
+    Plan p{};
+    p.NF = w->NF;
+    p.NT = w->NT;
+    const double* win;
+    if (int rc = window_dev(p, w->A, &win)) return rc;
+    layer_var_kernel<<<w->NF + 1, kPsdThreads, 0, s>>>(w->NF, w->NT, win, d_psd, d_lv);
+    if (d_pix)
+    {
+        const long long n = (long long)w->NF * w->NT;
+        wdm_psd_fill_kernel<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(w->NF, w->NT, d_lv, d_pix);
+    }
+    GH_CHECK(hipGetLastError());
+    return GLASS_HIP_OK;
+}
+
 }  // namespace wdm
 }  // namespace glass
 
 using namespace glass;
 using namespace glass::wdm;
+
+extern "C" int glass_hip_wdm_psd(const glass_hip_wdm* w, const double* d_dft_psd, double* d_layer_var,
+                                 double* d_wdm_psd, void* stream)
+{
+    if (int rc = psd_check(w)) return rc;
+    if (!d_dft_psd || !d_layer_var) return set_error("wdm_psd: NULL pointer"), GLASS_HIP_EINVAL;
+    return wdm_psd_dev(w, d_dft_psd, d_layer_var, d_wdm_psd, (hipStream_t)stream);
+}
+
+// host arrays: dft_psd ND/2 + 1 in; layer_var NF + 1 and/or wdm_psd ND out
+static int wdm_psd_host(const glass_hip_wdm* w, const double* dft_psd, double* layer_var, double* wdm_psd)
+{
+    if (int rc = psd_check(w)) return rc;
+    if (!dft_psd) return set_error("wdm psd: NULL dft_psd"), GLASS_HIP_EINVAL;
+    ThreadCtx* c;
+    if (int rc = ctx_get(&c)) return rc;
+    const size_t np = (size_t)w->NF * w->NT / 2 + 1, nl = (size_t)w->NF + 1;
+    const size_t npix = wdm_psd ? (size_t)w->NF * w->NT : 0;
+    if (int rc = c->dbuf.ensure(sizeof(double) * (np + nl + npix))) return rc;
+    double* d_psd = (double*)c->dbuf.p;
+    double* d_lv = d_psd + np;
+    double* d_pix = wdm_psd ? d_lv + nl : nullptr;
+    GH_CHECK(hipMemcpyAsync(d_psd, dft_psd, sizeof(double) * np, hipMemcpyHostToDevice, c->stream));
+    if (int rc = wdm_psd_dev(w, d_psd, d_lv, d_pix, c->stream)) return rc;
+    if (layer_var) GH_CHECK(hipMemcpyAsync(layer_var, d_lv, sizeof(double) * nl, hipMemcpyDeviceToHost, c->stream));
+    if (wdm_psd) GH_CHECK(hipMemcpyAsync(wdm_psd, d_pix, sizeof(double) * npix, hipMemcpyDeviceToHost, c->stream));
+    GH_CHECK(hipStreamSynchronize(c->stream));
+    return GLASS_HIP_OK;
+}
+
+extern "C" int glass_hip_stationary_dft_psd_to_wdm_layer_var(const glass_hip_wdm* w, const double* dft_psd,
+                                                             double* layer_var)
+{
+    if (!layer_var) return set_error("wdm psd: NULL layer_var"), GLASS_HIP_EINVAL;
+    return wdm_psd_host(w, dft_psd, layer_var, nullptr);
+}
+
+extern "C" int glass_hip_stationary_dft_psd_to_wdm_psd(const glass_hip_wdm* w, const double* dft_psd,
+                                                       double* wdm_psd)
+{
+    if (!wdm_psd) return set_error("wdm psd: NULL wdm_psd"), GLASS_HIP_EINVAL;
+    return wdm_psd_host(w, dft_psd, nullptr, wdm_psd);
+}
 
 extern "C" int glass_hip_wdm_transform(const glass_hip_wdm* w, const double* d_time, double* d_pixels,
                                        void* stream)

@@ -53,7 +53,7 @@ __constant__ TdiTerm c_tdi[8] = {
 
 // Natural cubic spline coefficients c = y0|y1|y2|y3 (N each), the
 // recurrence of spline_coefficients (utils/src/glass_math.c:98-178).
-static void spline_host(int N, const double* x, const double* y, double* c)
+void spline_host(int N, const double* x, const double* y, double* c)
 {
 #pragma clang fp contract(off)
     std::vector<double> dx(N, 0.0), alpha(N, 0.0), l(N, 0.0), z(N, 0.0), mu(N, 0.0);
@@ -168,6 +168,7 @@ struct Geo
     const double* win[kMaxLayers + 1];  // per L: window of L+1 layers
     const double2* tw;     // 4096-entry twiddles
     const double2* tw3;    // 12288-entry twiddles
+    const double2* tw16;   // 16384-entry twiddles (template series of NT >= 1024)
 };
 
 struct GeoHost
@@ -303,7 +304,7 @@ static int geo_build(GeoHost& G, const glass_hip_wdm& w, int Norb, const double*
     g.ds[0] = nullptr;
     g.dsi[0] = nullptr;
     g.win[0] = nullptr;
-    if ((rc = twiddles(&g.tw)) || (rc = twiddles3(&g.tw3))) return rc;
+    if ((rc = twiddles(&g.tw)) || (rc = twiddles3(&g.tw3)) || (rc = twiddles16(&g.tw16))) return rc;
     return GLASS_HIP_OK;
 }
 
@@ -311,10 +312,10 @@ static int geo_build(GeoHost& G, const glass_hip_wdm& w, int Norb, const double*
 static int check_grid(const glass_hip_wdm* w, int Norb)
 {
     if (!w) return set_error("NULL wdm"), GLASS_HIP_EINVAL;
-    if (w->NT != 256 && w->NT != 512)
+    if (w->NT != 256 && w->NT != 512 && w->NT != 1024 && w->NT != 2048 && w->NT != 4096)
     {
-        set_error("wavelet templates: NT = %d unsupported (this build computes templates for NT 256 or "
-                  "512; larger NT needs a template FFT beyond one workgroup's LDS)",
+        set_error("wavelet templates: NT = %d unsupported (this build computes templates for NT = 256, 512, "
+                  "1024, 2048 or 4096)",
                   w->NT);
         return GLASS_HIP_EINVAL;
     }
@@ -626,6 +627,8 @@ struct LayerArgs
     double* values;       // [n x 3 x 3 NT]
     double* dlogL;        // [n] (with data)
     double* snr;          // [n] (with data)
+    double2* big;         // NT >= 1024: nbig global scratch slices of big_layer_slice(NT)
+    int nbig;
 };
 
 // X, Y (packed in U) or Z (half-length V) spectrum at bin k, 0 < k < Nds/2
@@ -850,6 +853,215 @@ constexpr size_t layer_lds_bytes()
     return sizeof(double2) * 9 * NT;
 }
 
+// ---------------------------------------------- K2 for NT of 1024..4096 --
+//
+// The heterodyned series of NT (L+1) <= 16384 samples and its spectrum do
+// not fit one workgroup's LDS: they live in a global scratch slice per
+// workgroup (U = X + iY and V = Z packed as before, each with a partner
+// buffer) and are transformed by ping-pong Stockham stages through L2, with
+// the same butterflies, twiddle values and stage order as the LDS path.
+// The windowed band of each layer (NT points, 64 KB at NT = 4096) is
+// inverse-transformed in LDS, one layer at a time.  Workgroups stride over
+// the templates.
+
+constexpr int kBigLayerThreads = 512;
+
+// double2 per scratch slice: U and partner (NDS each) and V and partner
+// (NDS / 2 each) at the largest NDS = 4 NT
+__host__ __device__ inline size_t big_layer_slice(int NT) { return 12 * (size_t)NT; }
+
+template <int NT, int L>
+__device__ void layer_body_big(const Geo& g, const Hdr& h, int e, const LayerArgs& a, double2* G0, double2* B,
+                               int* cnt, const double* __restrict__ scr)
+{
+    constexpr int NDS = NT * (L + 1);
+    constexpr int M = NDS / 2;
+    constexpr int NTHR = kBigLayerThreads;
+    const int tid = threadIdx.x;
+    const int N = g.Norb;
+    const double* cph = scr + 64;
+    const double* ctdi = scr + 64 + 4 * N;
+    const double* ds = g.ds[L];
+    const int* dsi = g.dsi[L];
+    double2* Ua = G0;
+    double2* Ub = G0 + NDS;
+    double2* Va = G0 + 2 * NDS;
+    double2* Vb = Va + M;
+    {
+#pragma clang fp contract(off)
+        const double w0 = PI2 * h.fhet;
+        for (int i = tid; i < NDS; i += NTHR)
+        {
+            const double* e5 = ds + 5 * (size_t)i;
+            const double t = e5[0];
+            double kdotr = 0.0;
+            for (int d = 0; d < 3; d++) kdotr += h.k[d] * e5[1 + d];
+            const double tsc = t - kdotr;
+            const int j = even_index(tsc, g.ot0, g.ods, N);
+            const double phase_ds = spline_at(cph, N, j, tsc - g.ot[j]);
+            const double phase_het = w0 * t;
+            const int si = dsi[i];
+            const double sdx = e5[4];
+            double x[3];
+            for (int ch = 0; ch < 3; ch++)
+            {
+                const double* ca = ctdi + 4 * (size_t)N * (2 * ch);
+                const double* cp = ctdi + 4 * (size_t)N * (2 * ch + 1);
+                const double Amp = spline_at(ca, N, si, sdx);
+                double Phase = spline_at(cp, N, si, sdx);
+                Phase += phase_ds;
+                Phase -= phase_het;
+                double sn, c;
+                sincos_red(Phase, &sn, &c);
+                x[ch] = Amp * c;
+            }
+            Ua[i] = make_double2(x[0], x[1]);
+            if (i & 1) Va[i >> 1].y = x[2];
+            else Va[i >> 1].x = x[2];
+        }
+    }
+    __threadfence_block();
+    __syncthreads();
+    const bool three = (NDS % 3) == 0;
+    const double2* twU = three ? g.tw3 : g.tw16;
+    const int tsU = (three ? kTw3N : kTw16N) / NDS;
+    const double2* U = pingpong_fft<NDS, NTHR>(Ua, Ub, twU, tsU);
+    const double2* V = pingpong_fft<M, NTHR>(Va, Vb, twU, tsU * 2);
+    int* lst = a.list + (size_t)e * 3 * NT;
+    {
+        constexpr int NC = NT * L;
+        constexpr int PER = (NC + NTHR - 1) / NTHR;
+        const int c0 = tid * PER;
+        int mine = 0;
+        for (int c = c0; c < c0 + PER && c < NC; c++)
+        {
+            const int k = c / L + (h.jmin + c % L) * NT;
+            mine += (k >= g.kmin && k < g.kmax);
+        }
+        cnt[tid] = mine;
+        __syncthreads();
+        if (tid == 0)
+        {
+            int sum = 0;
+            for (int t = 0; t < NTHR; t++)
+            {
+                const int v = cnt[t];
+                cnt[t] = sum;
+                sum += v;
+            }
+            a.nwave[e] = sum;
+            cnt[NTHR] = sum;
+        }
+        __syncthreads();
+        int pos = cnt[tid];
+        for (int c = c0; c < c0 + PER && c < NC; c++)
+        {
+            const int k = c / L + (h.jmin + c % L) * NT;
+            if (k >= g.kmin && k < g.kmax) lst[pos++] = k - g.kmin;
+        }
+    }
+    __syncthreads();
+    const int Nw = cnt[NTHR];
+    const double* win = g.win[L];
+    const double norm = sqrt((double)g.NF) / (L + 1.0);
+    double* vals = a.values + (size_t)e * 9 * NT;
+    for (int ch = 0; ch < 3; ch++)
+        for (int off = 0; off < L; off++)
+        {
+            const int m = h.jmin + off;
+            const int center = (off + 1) * NT / 2;
+            for (int jj = tid; jj < NT; jj += NTHR)
+            {
+                const int j = jj - NT / 2;
+                const int idx = center + j;
+                double2 v = make_double2(0.0, 0.0);
+                if (jj > 0 && idx >= 0 && idx <= NDS / 2)
+                {
+                    double wt = win[j < 0 ? -j : j];
+                    if (j == 0 && (m == 0 || m == g.NF)) wt /= 2.0;
+                    const double2 S = spectrum<NDS>(ch, idx, U, V, twU, tsU);
+                    v = make_double2(wt * S.x, -(wt * S.y));
+                }
+                B[jj] = v;
+            }
+            __syncthreads();
+            lds_fft<NT, 1, NTHR>(B, g.tw, kTwN / NT);
+            for (int nn = tid; nn < NT; nn += NTHR)
+            {
+                const double2 r = B[nn];
+                const double re = r.x / NT, im = -r.y / NT;
+                double val;
+                if (m == 0 || m == g.NF) val = (nn % 2 == 0) ? re * M_SQRT2 * norm : 0.0;
+                else
+                {
+                    const int sgn = (m % 2 == 0) ? 1 : -1;
+                    val = ((nn + m) % 2 == 0) ? re * norm : sgn * im * norm;
+                }
+                const int pos = nn * L + off;
+                if (pos < Nw) vals[ch * 3 * NT + pos] = val;
+            }
+            __syncthreads();
+        }
+    if (a.data)
+    {
+        __threadfence_block();
+        __syncthreads();
+        double hh = 0.0, dl = 0.0;
+        for (int t = tid; t < Nw; t += NTHR)
+        {
+            const int k = lst[t];
+            const double hv[3] = {vals[t], vals[3 * NT + t], vals[6 * NT + t]};
+            double C[3][3];
+            C[0][0] = a.invC[k];
+            C[1][1] = a.invC[(size_t)a.npix + k];
+            C[2][2] = a.invC[2 * (size_t)a.npix + k];
+            C[0][1] = C[1][0] = a.invC[3 * (size_t)a.npix + k];
+            C[0][2] = C[2][0] = a.invC[4 * (size_t)a.npix + k];
+            C[1][2] = C[2][1] = a.invC[5 * (size_t)a.npix + k];
+            const double d[3] = {a.data[k], a.data[(size_t)a.npix + k], a.data[2 * (size_t)a.npix + k]};
+            double th = 0.0, tdh = 0.0, tdd = 0.0;
+            for (int p = 0; p < 3; p++)
+                for (int q = 0; q < 3; q++)
+                {
+                    th += hv[p] * hv[q] * C[p][q];
+                    tdh += d[p] * hv[q] * C[p][q];
+                    tdd += d[p] * d[q] * C[p][q];
+                }
+            hh += th;
+            const int i = (k + g.kmin) % g.NT;
+            if (k >= 0 && k < a.npix) dl += (i < g.imin || i > g.imax) ? 0.5 * tdd : tdh - 0.5 * th;
+        }
+        hh = block_sum<NTHR>(hh);
+        dl = block_sum<NTHR>(dl);
+        if (tid == 0)
+        {
+            a.snr[e] = sqrt(hh);
+            a.dlogL[e] = dl;
+        }
+    }
+}
+
+template <int NT>
+__global__ void __launch_bounds__(kBigLayerThreads) layer_kernel_big(Geo g, int n, LayerArgs a,
+                                                                     const double* __restrict__ scratch)
+{
+    extern __shared__ double2 ldsb[];  // B: one layer's band, NT complex
+    __shared__ Hdr H;
+    __shared__ int cnt[kBigLayerThreads + 1];
+    double2* G0 = a.big + big_layer_slice(NT) * (size_t)blockIdx.x;
+    for (int e = blockIdx.x; e < n; e += gridDim.x)
+    {
+        const double* scr = scratch + scratch_doubles(g.Norb) * (size_t)e;
+        __syncthreads();  // the previous template is done with H, cnt and the slice
+        if (threadIdx.x == 0) H = *(const Hdr*)scr;
+        __syncthreads();
+        const Hdr h = H;
+        if (h.L == 1) layer_body_big<NT, 1>(g, h, e, a, G0, ldsb, cnt, scr);
+        else if (h.L == 2) layer_body_big<NT, 2>(g, h, e, a, G0, ldsb, cnt, scr);
+        else layer_body_big<NT, 3>(g, h, e, a, G0, ldsb, cnt, scr);
+    }
+}
+
 // ----------------------------------------------------------- launching --
 
 static int launch_templates(const GeoHost& G, int n, const double* d_params, double* d_scratch,
@@ -861,13 +1073,43 @@ static int launch_templates(const GeoHost& G, int n, const double* d_params, dou
     hipLaunchKernelGGL(resp_kernel, dim3(n), dim3(kRespThreads), resp_lds_bytes(g.Norb), s, g, n, d_params,
                        d_scratch);
     GH_CHECK(hipGetLastError());
-    if (g.NT == 512)
+    if (g.NT >= 1024)
+    {
+        if (!a.big || a.nbig <= 0)
+        {
+            set_error("wavelet templates: NT = %d needs the long-series scratch", g.NT);
+            return GLASS_HIP_EINVAL;
+        }
+        const int grid = n < a.nbig ? n : a.nbig;
+        const size_t lds = sizeof(double2) * (size_t)g.NT;
+        if (g.NT == 1024)
+            hipLaunchKernelGGL(layer_kernel_big<1024>, dim3(grid), dim3(kBigLayerThreads), lds, s, g, n, a,
+                               d_scratch);
+        else if (g.NT == 2048)
+            hipLaunchKernelGGL(layer_kernel_big<2048>, dim3(grid), dim3(kBigLayerThreads), lds, s, g, n, a,
+                               d_scratch);
+        else
+            hipLaunchKernelGGL(layer_kernel_big<4096>, dim3(grid), dim3(kBigLayerThreads), lds, s, g, n, a,
+                               d_scratch);
+    }
+    else if (g.NT == 512)
         hipLaunchKernelGGL(layer_kernel<512>, dim3(n), dim3(kLayerThreads), layer_lds_bytes<512>(), s, g, n, a,
                            d_scratch);
     else
         hipLaunchKernelGGL(layer_kernel<256>, dim3(n), dim3(kLayerThreads), layer_lds_bytes<256>(), s, g, n, a,
                            d_scratch);
     GH_CHECK(hipGetLastError());
+    return GLASS_HIP_OK;
+}
+
+// The calling thread's long-series scratch for n templates of grid NT
+// (nothing below NT = 1024).
+static int big_scratch(ThreadCtx* c, int NT, int n, LayerArgs& a)
+{
+    if (NT < 1024) return GLASS_HIP_OK;
+    if (int rc = c->dscr.ensure(sizeof(double2) * big_layer_slice(NT) * (size_t)n)) return rc;
+    a.big = (double2*)c->dscr.p;
+    a.nbig = n;
     return GLASS_HIP_OK;
 }
 
@@ -879,7 +1121,9 @@ static int set_lds_limits()
         if (hipFuncSetAttribute((const void*)resp_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 (int)resp_lds_bytes(kMaxKnots)) != hipSuccess ||
             hipFuncSetAttribute((const void*)layer_kernel<512>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)layer_lds_bytes<512>()) != hipSuccess)
+                                (int)layer_lds_bytes<512>()) != hipSuccess ||
+            hipFuncSetAttribute((const void*)layer_kernel_big<4096>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)(sizeof(double2) * 4096)) != hipSuccess)
         {
             set_error("hipFuncSetAttribute (dynamic LDS) failed");
             rc = GLASS_HIP_ERUNTIME;
@@ -1214,6 +1458,7 @@ extern "C" int glass_hip_ucb_waveform_wavelet(const glass_hip_wdm* w, int Norb, 
     a.nwave = dnw;
     a.list = dlist;
     a.values = dval;
+    if ((rc = big_scratch(c, NT, 1, a))) return rc;
     if ((rc = launch_templates(*G, 1, dp, dscr, a, c->stream))) return rc;
     GH_CHECK(hipMemcpyAsync(hp, dval, sizeof(double) * 9 * NT + sizeof(int) * (3 * NT + 1), hipMemcpyDeviceToHost,
                             c->stream));
@@ -1253,6 +1498,9 @@ struct glass_hip_wdm_batch
     double* fparams = nullptr;
     double* fD = nullptr;
     double* fM = nullptr;
+    // long-series scratch of the NT >= 1024 layer kernel: nbig slices
+    double2* big = nullptr;
+    int nbig = 0;
 };
 
 extern "C" int glass_hip_wdm_batch_create(glass_hip_wdm_batch** plan, const glass_hip_wdm* w, int Norb,
@@ -1292,6 +1540,19 @@ extern "C" int glass_hip_wdm_batch_create(glass_hip_wdm_batch** plan, const glas
         set_error("wdm_batch_create: device allocation failed");
         return fail(GLASS_HIP_ERUNTIME);
     }
+    if (NT >= 1024)
+    {
+        // the layer kernel strides over the templates with two workgroups
+        // per CU, each with its own long-series slice
+        int ncu = 0;
+        GH_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, P->device));
+        P->nbig = std::min(max_templates, 2 * std::max(ncu, 1));
+        if (hipMalloc(&P->big, sizeof(double2) * big_layer_slice(NT) * (size_t)P->nbig) != hipSuccess)
+        {
+            set_error("wdm_batch_create: device allocation failed");
+            return fail(GLASS_HIP_ERUNTIME);
+        }
+    }
     if (hipMemcpy(P->data, data, sizeof(double) * 3 * (size_t)npix, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(P->invC, invC, sizeof(double) * 6 * (size_t)npix, hipMemcpyHostToDevice) != hipSuccess)
     {
@@ -1307,7 +1568,7 @@ extern "C" int glass_hip_wdm_batch_destroy(glass_hip_wdm_batch* P)
     if (!P) return GLASS_HIP_OK;
     (void)hipSetDevice(P->device);
     for (void* p : {(void*)P->data, (void*)P->invC, (void*)P->scratch, (void*)P->list, (void*)P->values,
-                    (void*)P->nwave, (void*)P->fparams, (void*)P->fD, (void*)P->fM})
+                    (void*)P->nwave, (void*)P->fparams, (void*)P->fD, (void*)P->fM, (void*)P->big})
         if (p) (void)hipFree(p);
     delete P;
     return GLASS_HIP_OK;
@@ -1330,7 +1591,7 @@ extern "C" int glass_hip_wdm_batch_eval(glass_hip_wdm_batch* P, int n, const dou
     if (n < 0 || n > P->max_templates)
         return set_error("wdm_batch_eval: n = %d outside 0..%d", n, P->max_templates), GLASS_HIP_EINVAL;
     GH_CHECK(hipSetDevice(P->device));
-    LayerArgs a;
+    LayerArgs a{};
     a.data = P->data;
     a.invC = P->invC;
     a.npix = P->npix;
@@ -1339,6 +1600,8 @@ extern "C" int glass_hip_wdm_batch_eval(glass_hip_wdm_batch* P, int n, const dou
     a.values = P->values;
     a.dlogL = d_dlogL;
     a.snr = d_snr;
+    a.big = P->big;
+    a.nbig = P->nbig;
     return launch_templates(*P->G, n, d_params, P->scratch, a, (hipStream_t)stream);
 }
 
@@ -1444,6 +1707,7 @@ extern "C" int glass_hip_ucb_fisher_wavelet(const glass_hip_wdm* w, int Norb, co
     a.nwave = dnw;
     a.list = dlist;
     a.values = dval;
+    if ((rc = big_scratch(c, NT, 16, a))) return rc;
     if ((rc = launch_templates(*G, 16, dp, dscr, a, c->stream))) return rc;
     FishArgs fa{};
     fa.nwave = dnw;
@@ -1495,6 +1759,8 @@ extern "C" int glass_hip_wdm_batch_fisher(glass_hip_wdm_batch* P, int n, const d
     a.nwave = P->nwave;
     a.list = P->list;
     a.values = P->values;
+    a.big = P->big;
+    a.nbig = P->nbig;
     int rc;
     if ((rc = launch_templates(*P->G, 16 * n, P->fparams, P->scratch, a, s))) return rc;
     FishArgs fa{};

@@ -6,8 +6,10 @@
   returned by numpy.
 * source_derivs: exact L, dL/dtheta and Fisher for many sources on device
   tensors (include/glass_hip_batch.h).
-* UcbSourceLogL: torch.autograd.Function whose forward is L(theta) and whose
-  backward uses the exact gradient kernel.
+* source_grad: L and dL/dtheta only, from the reverse-mode (adjoint) kernel.
+* UcbSourceLogL: torch.autograd.Function whose forward is L(theta) from the
+  adjoint kernel and whose backward returns the adjoint gradient;
+  ucb_source_logl runs the likelihood-only kernel when no gradient can flow.
 """
 import ctypes
 
@@ -23,6 +25,8 @@ _EXTRA += [
     ("glass_hip_ucb_fisher", [_vp, ctypes.c_char_p, _d, _d, _i, _i, _dp, _dp, _dp], _i),
     ("glass_hip_fd_source_derivs", [_vp, ctypes.c_char_p, _d, _d, _i, _vp, _i, _vp, _i, _vp, _ll, _vp, _ll, _i,
                                     _vp, _vp, _vp, _vp], _i),
+    ("glass_hip_fd_source_grad", [_vp, ctypes.c_char_p, _d, _d, _i, _vp, _i, _vp, _i, _vp, _ll, _vp, _ll, _i,
+                                  _vp, _vp, _vp], _i),
 ]
 
 
@@ -82,6 +86,35 @@ def source_derivs(orbit, T, params, BW, imin, invC, resid=None, t0=0.0, fmt="san
     return out_L, grad, fisher
 
 
+def source_grad(orbit, T, params, BW, imin, invC, resid=None, t0=0.0, fmt="sangria", mode=0, stream=None,
+                want_grad=True):
+    """L and dL/dtheta for n sources from the adjoint kernel (torch tensors
+    on the GPU; arguments as source_derivs, BW 32..1024).  Returns (L [n],
+    grad [n,8]) or (L, None) with want_grad=False."""
+    import torch
+    n = params.shape[0]
+    W = invC.shape[-1]
+    dev = params.device
+    params = params.contiguous().to(torch.float64)
+    imin = imin.contiguous().to(torch.int32)
+    invC = invC.contiguous().to(torch.float64)
+    istride = 0 if invC.dim() == 2 else 6 * W
+    if resid is not None:
+        resid = torch.view_as_real(resid.contiguous().to(torch.complex128)).contiguous()
+        rstride = 0 if resid.dim() == 3 else 6 * W
+        rptr = _vp(resid.data_ptr())
+    else:
+        rstride, rptr = 0, None
+    out_L = torch.empty(n, dtype=torch.float64, device=dev)
+    grad = torch.empty((n, 8), dtype=torch.float64, device=dev) if want_grad else None
+    s = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+    check(_L().glass_hip_fd_source_grad(ctypes.cast(ctypes.byref(orbit), _vp), fmt.encode(), float(T), float(t0),
+                                        int(n), _vp(params.data_ptr()), int(BW), _vp(imin.data_ptr()), int(W), rptr,
+                                        rstride, _vp(invC.data_ptr()), istride, int(mode), _vp(out_L.data_ptr()),
+                                        None if grad is None else _vp(grad.data_ptr()), _vp(s)), "fd_source_grad")
+    return out_L, grad
+
+
 def _autograd_class():
     import torch
 
@@ -90,7 +123,9 @@ def _autograd_class():
 
         @staticmethod
         def forward(ctx, params, orbit, T, BW, imin, invC, resid):
-            L, grad, _ = source_derivs(orbit, T, params.detach(), BW, imin, invC, resid)
+            # one adjoint launch: L and the gradient the backward returns
+            # (never the Fisher kernel)
+            L, grad = source_grad(orbit, T, params.detach(), BW, imin, invC, resid)
             ctx.save_for_backward(grad)
             return L
 
@@ -111,3 +146,13 @@ def UcbSourceLogL():
     if _cls is None:
         _cls = _autograd_class()
     return _cls
+
+
+def ucb_source_logl(params, orbit, T, BW, imin, invC, resid=None):
+    """L(theta) per source as a differentiable torch function: through
+    UcbSourceLogL (the adjoint kernel's L and gradient) when a gradient can
+    flow to params, else the likelihood-only kernel (no gradient work)."""
+    import torch
+    if torch.is_grad_enabled() and params.requires_grad:
+        return UcbSourceLogL().apply(params, orbit, T, BW, imin, invC, resid)
+    return source_grad(orbit, T, params.detach(), BW, imin, invC, resid, want_grad=False)[0]

@@ -62,6 +62,9 @@ _SIGS = [
     ("glass_hip_wavelet_transform_timefreq_by_layers", [_W, _i, _i, _dp, _dp], _i),
     ("glass_hip_ucb_fisher_wavelet", [_W, _i, _dp, _dp, _dp, _dp, _d, _d, _dp, _dp, _i, _dp], _i),
     ("glass_hip_wdm_batch_fisher", [_vp, _i, _vp, _vp, _vp], _i),
+    ("glass_hip_stationary_dft_psd_to_wdm_psd", [_W, _dp, _dp], _i),
+    ("glass_hip_stationary_dft_psd_to_wdm_layer_var", [_W, _dp, _dp], _i),
+    ("glass_hip_wdm_psd", [_W, _vp, _vp, _vp, _vp], _i),
 ]
 _declared = False
 
@@ -200,6 +203,31 @@ def gaussian_log_likelihood_wavelet(w, data, model, mlist, invC9):
         ctypes.byref(w), d[0].ctypes.data_as(_dp), d[1].ctypes.data_as(_dp), d[2].ctypes.data_as(_dp),
         m[0].ctypes.data_as(_dp), m[1].ctypes.data_as(_dp), m[2].ctypes.data_as(_dp), ml.ctypes.data_as(_ip),
         ml.size, ic.ctypes.data_as(_dp), N), "gaussian_log_likelihood_wavelet")
+
+
+def stationary_dft_psd_to_wdm_psd(w, dft_psd):
+    """stationary_dft_psd_to_wdm_psd (utils/src/glass_wavelet.c:901-911):
+    DFT variances at k = 0..ND/2 -> the ND pixel variances."""
+    psd = _f64(dft_psd)
+    if psd.size != w.NF * w.NT // 2 + 1:
+        raise ValueError(f"dft_psd must hold ND/2 + 1 = {w.NF * w.NT // 2 + 1} values")
+    out = np.zeros(w.NF * w.NT)
+    check(_L().glass_hip_stationary_dft_psd_to_wdm_psd(ctypes.byref(w), psd.ctypes.data_as(_dp),
+                                                       out.ctypes.data_as(_dp)), "stationary_dft_psd_to_wdm_psd")
+    return out
+
+
+def stationary_dft_psd_to_wdm_layer_var(w, dft_psd):
+    """stationary_dft_psd_to_wdm_layer_var (utils/src/glass_wavelet.c:865-899):
+    the NF + 1 layer variances."""
+    psd = _f64(dft_psd)
+    if psd.size != w.NF * w.NT // 2 + 1:
+        raise ValueError(f"dft_psd must hold ND/2 + 1 = {w.NF * w.NT // 2 + 1} values")
+    out = np.zeros(w.NF + 1)
+    check(_L().glass_hip_stationary_dft_psd_to_wdm_layer_var(ctypes.byref(w), psd.ctypes.data_as(_dp),
+                                                             out.ctypes.data_as(_dp)),
+          "stationary_dft_psd_to_wdm_layer_var")
+    return out
 
 
 def wdm_transform(w, time_tensor, out=None, stream=None):

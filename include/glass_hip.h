@@ -46,6 +46,26 @@ typedef struct glass_hip_orbit
  * (utils/src/glass_lisa.c:125-136). */
 void glass_hip_analytic_orbit(glass_hip_orbit *orbit);
 
+/* An orbit read from a file: struct Orbit after initialize_numeric_orbit
+ * (utils/src/glass_lisa.c:138-269), whose orbit_function interpolate_orbits
+ * (:62-72) evaluates a natural cubic spline of every coordinate over the
+ * file's Norb knots t.  x, y, z: 3 x Norb each (row i = spacecraft i, as
+ * orbit->x[i]); L, fstar: the average arm length and c / (2 pi L) the
+ * reference stores (glass_hip_numeric_orbit_armlength computes them). */
+typedef struct glass_hip_numeric_orbit
+{
+    int Norb;
+    const double *t;
+    const double *x, *y, *z;
+    double L;
+    double fstar;
+} glass_hip_numeric_orbit;
+
+/* initialize_numeric_orbit's average arm length (glass_lisa.c:207-259) and
+ * transfer frequency fstar = CLIGHT / (2 pi L) of an orbit file's samples. */
+int glass_hip_numeric_orbit_armlength(int Norb, const double *x, const double *y, const double *z,
+                                      double *L, double *fstar);
+
 /* Selects the HIP device for the calling thread (default 0). */
 int glass_hip_set_device(int device);
 
@@ -59,10 +79,20 @@ const char *glass_hip_last_error(void);
  * (ucb/src/glass_ucb_waveform.h:210, ucb/src/glass_ucb_waveform.c:738-1001).
  * format "sangria" | "phase" | "frequency"; writes 2*BW doubles to each of
  * X,Y,Z,A,E ("phase"/"frequency" with NI<=1 write X only, as the reference).
- * BW: a power of two, 16 <= BW <= 2048. */
+ * BW: a power of two, 16 <= BW <= 4096. */
 int glass_hip_ucb_waveform(const glass_hip_orbit *orbit, const char *format, double T, double t0,
                            const double *params, int NParams, double *X, double *Y, double *Z,
                            double *A, double *E, int BW, int NI);
+
+/* ucb_waveform with a numeric orbit (orbit->orbit_function ==
+ * interpolate_orbits): the same template with the spacecraft positions at
+ * the BW slow-sample times t0 + T n / BW taken from the orbit's splines
+ * (coefficients and positions computed once per orbit, T, t0 and BW on the
+ * host in the reference's operation order, then cached on the device).
+ * The sample times must lie within the knots [t[0], t[Norb-1]]. */
+int glass_hip_ucb_waveform_numeric(const glass_hip_numeric_orbit *orbit, const char *format, double T,
+                                   double t0, const double *params, int NParams, double *X, double *Y,
+                                   double *Z, double *A, double *E, int BW, int NI);
 
 /* Replaces int ucb_bandwidth(double L, double fstar, double f, double fdot,
  *   double costheta, double A, double T, int N)

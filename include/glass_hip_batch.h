@@ -188,13 +188,30 @@ int glass_hip_fd_batch_state(glass_hip_fd_batch *plan, double *logL, double *src
  * in the parameter-array units of map_ucb_params_to_array
  * (ucb/src/glass_ucb_model.c:40-53).  mode 0 weights bin m of the
  * template with invC at window bin imin+m; mode 1 reproduces the
- * reference's Fisher alignment (invC at window bin m).  BW: 32..1024. */
+ * reference's Fisher alignment (invC at window bin m).  BW: a power of two,
+ * 32..4096.  BW 2048 and 4096 run decimated kernels over at most one
+ * workgroup per CU with a scratch slice each (0.6 / 1.6 MB) that the
+ * calling host thread keeps (grown with hipMalloc on first use; calls of
+ * one host thread on different streams must not overlap). */
 int glass_hip_fd_source_derivs(const glass_hip_orbit *orbit, const char *format, double T,
                                double t0, int n, const double *d_params, int BW,
                                const int *d_imin, int W, const double *d_resid,
                                long long resid_stride, const double *d_invC,
                                long long invC_stride, int mode, double *d_logL, double *d_grad,
                                double *d_fisher, void *stream);
+
+/* Reverse-mode (adjoint) gradient: the same L and dL/dtheta as
+ * glass_hip_fd_source_derivs (same arguments and conventions; BW 32..1024)
+ * without the Fisher matrix, by transposing the TDI and FFT steps: one
+ * waveform, six more FFTs and one pass of per-sample tangents per source
+ * (no reference counterpart; for gradient-only consumers such as autograd
+ * and gradient-based proposals).  d_grad may be NULL: L only. */
+int glass_hip_fd_source_grad(const glass_hip_orbit *orbit, const char *format, double T,
+                             double t0, int n, const double *d_params, int BW,
+                             const int *d_imin, int W, const double *d_resid,
+                             long long resid_stride, const double *d_invC,
+                             long long invC_stride, int mode, double *d_logL, double *d_grad,
+                             void *stream);
 
 /* Exact derivatives of chain sources at their current parameters, from
  * the plan's state (no reference counterpart; for Fisher-matrix proposals,
@@ -205,7 +222,7 @@ int glass_hip_fd_source_derivs(const glass_hip_orbit *orbit, const char *format,
  * window's invC, as glass_hip_fd_source_derivs mode 0:
  *   d_logL[e] = (R|h) - (h|h)/2,  d_grad[8e..] = (R - h | dh/dtheta),
  *   d_fisher[64e..] = (dh/dtheta_k | dh/dtheta_l).
- * Sources with BW above 2048, or invalid indices, get NaN outputs.  Graph
+ * Sources with a BW outside 32..4096, or invalid indices, get NaN outputs.  Graph
  * safe (no allocation); shares the plan's class lists with evals, so it
  * must not overlap an eval of the same plan on another stream. */
 int glass_hip_fd_batch_derivs(glass_hip_fd_batch *plan, int n, const int *d_chain, const int *d_src,

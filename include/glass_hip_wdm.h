@@ -70,6 +70,21 @@ int glass_hip_wavelet_transform_inverse_freq(const glass_hip_wdm *wdm, const dou
 int glass_hip_wavelet_transform_timefreq_by_layers(const glass_hip_wdm *wdm, int jmin, int Nlayers,
                                                    const double *window, double *timedata);
 
+/* Replaces void stationary_dft_psd_to_wdm_psd(struct Wavelets *wdm,
+ *   double *dft_psd, double *wdm_psd) (utils/src/glass_wavelet.c:901-911,
+ * with stationary_dft_psd_to_wdm_layer_var :865-899): dft_psd the DFT
+ * variance <|FFT[k]|^2> at k = 0..ND/2 (ND/2 + 1 doubles), wdm_psd the
+ * ND = NF*NT pixel variances out (pixel (n, m) at n + NT m; row 0 holds
+ * layer 0 at even n and layer NF at odd n).  NT even; the window is
+ * build_wdm_filter_freq's of (NF, NT, A). */
+int glass_hip_stationary_dft_psd_to_wdm_psd(const glass_hip_wdm *wdm, const double *dft_psd, double *wdm_psd);
+
+/* Replaces void stationary_dft_psd_to_wdm_layer_var(struct Wavelets *wdm,
+ *   double *dft_psd, double *layer_var) (utils/src/glass_wavelet.c:865-899):
+ * the NF + 1 layer variances. */
+int glass_hip_stationary_dft_psd_to_wdm_layer_var(const glass_hip_wdm *wdm, const double *dft_psd,
+                                                  double *layer_var);
+
 /* Replaces void ucb_waveform_wavelet(struct Orbit *orbit,
  *   struct Wavelets *wdm, double Tobs, double t0, double *params,
  *   int *wavelet_list, int *Nwavelet, double *X, double *Y, double *Z)
@@ -124,6 +139,13 @@ int glass_hip_ucb_fisher_wavelet(const glass_hip_wdm *wdm, int Norb, const doubl
                                  const double *params, const double *invC, int N, double *fisher);
 
 /* ------------------------------------------------ device-pointer API --- */
+
+/* stationary_dft_psd_to_wdm_psd on device pointers (no reference
+ * counterpart as a device call): d_dft_psd ND/2 + 1 in, d_layer_var NF + 1
+ * out and, if not NULL, d_wdm_psd ND out; enqueues on stream, no
+ * allocation after the window's first use on a grid. */
+int glass_hip_wdm_psd(const glass_hip_wdm *wdm, const double *d_dft_psd, double *d_layer_var,
+                      double *d_wdm_psd, void *stream);
 
 /* Full forward transform on device memory: d_time (ND samples) ->
  * d_pixels (ND).  Same grid limits as glass_hip_wavelet_transform_timefreq.

@@ -223,6 +223,8 @@ void   orc_ucb_wavelet_layers(const orc_wdm *w, double Tobs, const double *param
                               int *jwidth);
 /* The interpolated TDI series A(t) cos(Phi(t) - 2 pi fhet t) of
  * ucb_waveform_wavelet at N samples t0 + i dt_s (out: 3 x N); see orc_wdm.c */
+void   orc_set_numeric_orbit(int Norb, const double *t, const double *coef);
+double orc_numeric_orbit_armlength(int N, const double *pos);
 void   orc_ucb_barycenter_waveform(const double *params, int N, const double *times, double *phase,
                                    double *amp, double T);
 void   orc_ucb_wavelet_series(int Norb, const double *ot, const double *oc, double Tobs, double t0,

@@ -12,6 +12,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
+LIB_REL_PATH = os.path.join(HERE, "liboracle_rel.so")
 REF_KISSFFT = os.path.join(HERE, "_ref", "libkissfft_ref.so")
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -49,12 +50,8 @@ def build():
 _lib = None
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            build()
-        L = ctypes.CDLL(LIB_PATH)
+def _declare(L):
+    if True:
         L.orc_fft.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_int]
         L.orc_fftr.argtypes = [_dp, _dp, ctypes.c_int]
         L.orc_fftri.argtypes = [_dp, _dp, ctypes.c_int]
@@ -96,8 +93,33 @@ def lib():
         _noise_signatures(L)
         _sampler_signatures(L)
         _fstat_signatures(L)
-        _lib = L
+    return L
+
+
+def lib():
+    """The oracle at the parity flags (-O2 -ffp-contract=off): the checker."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = _declare(ctypes.CDLL(LIB_PATH))
     return _lib
+
+
+_lib_rel = None
+
+
+def lib_release():
+    """The same restatement built with the reference's release flags
+    (-O3 with FP contraction, CMakeLists.txt:37-39; -march=x86-64-v3 stands
+    in for -march=native since the library is built on one host and timed
+    on another): timing only, never the checker."""
+    global _lib_rel
+    if _lib_rel is None:
+        if not os.path.exists(LIB_REL_PATH):
+            build()
+        _lib_rel = _declare(ctypes.CDLL(LIB_REL_PATH))
+    return _lib_rel
 
 
 def analytic_orbit():
@@ -244,13 +266,16 @@ class ChainSet:
             s.src_imin = _i(bufs["src_imin"])
 
 
-def eval_batch(chainset, T, chain_of, sid_of, params, t0=0.0, nthreads=0, orbit=None):
+def eval_batch(chainset, T, chain_of, sid_of, params, t0=0.0, nthreads=0, orbit=None, release=False):
+    """logL of proposals (chain, source, params) and the wall seconds
+    (omp_get_wtime) of the OpenMP loop; release=True runs the release-flags
+    build (timing only)."""
     orbit = orbit or analytic_orbit()
     chain_of = np.ascontiguousarray(chain_of, dtype=np.int32)
     sid_of = np.ascontiguousarray(sid_of, dtype=np.int32)
     params = np.ascontiguousarray(params, dtype=np.float64)
     out = np.zeros(len(chain_of))
-    secs = lib().orc_eval_batch(ctypes.byref(orbit), T, t0, chainset.arr, len(chain_of),
+    secs = (lib_release() if release else lib()).orc_eval_batch(ctypes.byref(orbit), T, t0, chainset.arr, len(chain_of),
                                 _i(chain_of), _i(sid_of), _d(params), _d(out), nthreads)
     return out, secs
 
@@ -313,6 +338,9 @@ def _wdm_signatures(L):
     L.orc_ucb_waveform_wavelet_compact.restype = c_int
     L.orc_ucb_wavelet_series.argtypes = [c_int, _dp, _dp, c_d, c_d, _dp, c_int, c_d, c_d, _dp]
     L.orc_ucb_barycenter_waveform.argtypes = [_dp, c_int, _dp, _dp, _dp, c_d]
+    L.orc_set_numeric_orbit.argtypes = [c_int, _dp, _dp]
+    L.orc_numeric_orbit_armlength.argtypes = [c_int, _dp]
+    L.orc_numeric_orbit_armlength.restype = c_d
     L.orc_set_cr_trig.argtypes = [c_int]
     L.orc_ucb_wavelet_response.argtypes = [c_int, _dp, _dp, c_d, c_d, _dp, _dp, _dp, _dp, _dp]
     L.orc_snr_wavelet.argtypes = [_dp, _dp, _dp, _dp, c_int, _ip, c_int]
@@ -479,6 +507,32 @@ def ucb_wavelet_series(orb, Tobs, params, N, dt_s, fhet, t0=0.0):
     lib().orc_ucb_wavelet_series(len(t), _d(t), _d(coef), float(Tobs), float(t0), _d(p), int(N),
                                  float(dt_s), float(fhet), _d(out))
     return out
+
+
+class numeric_orbit:
+    """Context manager: inside the block the oracle's ucb_waveform takes its
+    spacecraft positions from splines over the knots t [N] and positions
+    pos [9, N] (rows x1 y1 z1 x2 ... as an orbit file's columns), as
+    interpolate_orbits does for a numeric orbit.  .orbit is the Orbit with
+    that file's average arm length and transfer frequency."""
+
+    def __init__(self, t, pos):
+        self.t = np.ascontiguousarray(t, dtype=np.float64)
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        N = len(self.t)
+        self.coef = np.zeros((9, 4 * N))
+        for s in range(9):
+            self.coef[s] = spline_coefficients(self.t, pos[s]).reshape(-1)
+        L = lib().orc_numeric_orbit_armlength(N, _d(pos))
+        self.orbit = Orbit(L, 299792458.0 / (2.0 * np.pi * L), L / (2.0 * 1.73205080757 * 1.49597870700e11)
+                           * 1.49597870700e11)
+
+    def __enter__(self):
+        lib().orc_set_numeric_orbit(len(self.t), _d(self.t), _d(self.coef))
+        return self
+
+    def __exit__(self, *a):
+        lib().orc_set_numeric_orbit(0, None, None)
 
 
 def ucb_barycenter_waveform(params, times, T):

@@ -4,6 +4,8 @@
  *
  * Restates, in FP64 with the reference's operation order:
  *   analytic_orbits               utils/src/glass_lisa.c:77-124
+ *   interpolate_orbits, initialize_numeric_orbit's arm length
+ *                                 utils/src/glass_lisa.c:62-72,207-259
  *   initialize_analytic_orbit     utils/src/glass_lisa.c:125-136
  *   LISA_polarization_tensor      utils/src/glass_lisa.c:1566-1596
  *   LISA_detector_tensor          utils/src/glass_lisa.c:1521-1564
@@ -113,6 +115,58 @@ static void detector_tensor(double L, double ep[4][4], double ec[4][4], const do
     kr[2][3] = k23;  kr[3][2] = -k23;
 }
 
+/* interpolate_orbits (utils/src/glass_lisa.c:62-72): the spacecraft
+ * positions of an orbit read from a file (initialize_numeric_orbit,
+ * :138-269), a natural cubic spline per coordinate over the file's knots,
+ * evaluated after a binary search.  While a numeric orbit is set (knots t,
+ * coefficients [9][4 Norb], row 3 sc + coordinate, as orc_spline_coefficients
+ * writes them) the slow series reads it in place of analytic_orbits, as
+ * ucb_waveform does through orbit->orbit_function. */
+static int g_norb = 0;
+static const double *g_ot = NULL, *g_oc = NULL;
+
+void orc_set_numeric_orbit(int Norb, const double *t, const double *coef)
+{
+    g_norb = (t && coef) ? Norb : 0;
+    g_ot = t;
+    g_oc = coef;
+}
+
+static void orbit_positions(const orc_orbit *o, double t, double *x, double *y, double *z)
+{
+    if (!g_norb)
+    {
+        orc_analytic_orbits(o, t, x, y, z);
+        return;
+    }
+    for (int i = 0; i < 3; i++)
+    {
+        x[i + 1] = orc_spline_eval(g_norb, g_ot, g_oc + 4 * (3 * i + 0) * g_norb, 0, t);
+        y[i + 1] = orc_spline_eval(g_norb, g_ot, g_oc + 4 * (3 * i + 1) * g_norb, 0, t);
+        z[i + 1] = orc_spline_eval(g_norb, g_ot, g_oc + 4 * (3 * i + 2) * g_norb, 0, t);
+    }
+}
+
+/* initialize_numeric_orbit's average arm length (glass_lisa.c:207-259):
+ * pos [9][N] rows x1 y1 z1 x2 y2 z2 x3 y3 z3 */
+double orc_numeric_orbit_armlength(int N, const double *pos)
+{
+    double L12 = 0.0, L23 = 0.0, L31 = 0.0;
+    for (int n = 0; n < N; n++)
+    {
+        double x1 = pos[0 * N + n], y1 = pos[1 * N + n], z1 = pos[2 * N + n];
+        double x2 = pos[3 * N + n], y2 = pos[4 * N + n], z2 = pos[5 * N + n];
+        double x3 = pos[6 * N + n], y3 = pos[7 * N + n], z3 = pos[8 * N + n];
+        L12 += sqrt((x1 - x2) * (x1 - x2) + (y1 - y2) * (y1 - y2) + (z1 - z2) * (z1 - z2));
+        L23 += sqrt((x3 - x2) * (x3 - x2) + (y3 - y2) * (y3 - y2) + (z3 - z2) * (z3 - z2));
+        L31 += sqrt((x1 - x3) * (x1 - x3) + (y1 - y3) * (y1 - y3) + (z1 - z3) * (z1 - z3));
+    }
+    L12 /= (double)N;
+    L31 /= (double)N;
+    L23 /= (double)N;
+    return (L12 + L31 + L23) / 3.;
+}
+
 /* Slow series of the six links, FFT'd, shifted and normalised.
  * glass_ucb_waveform.c:804-962.  d: [6][2*BW], link order 12,21,31,13,23,32 */
 void orc_ucb_slow_series(const orc_orbit *orbit, double T, double t0, const double *params,
@@ -142,7 +196,7 @@ void orc_ucb_slow_series(const orc_orbit *orbit, double T, double t0, const doub
     for (int n = 1; n <= BW; n++)
     {
         double t = t0 + T * (double)(n - 1) / (double)BW;
-        orc_analytic_orbits(orbit, t, x, y, z);
+        orbit_positions(orbit, t, x, y, z);
         detector_tensor(orbit->L, ep, ec, x, y, z, k, dp, dc, kr);
         for (int i = 1; i <= 3; i++)
         {

@@ -174,5 +174,75 @@ def test_autograd_binding(gl, orc):
     L = UcbSourceLogL().apply(params, gl.analytic_orbit(), synth.YEAR_T, BW, imin, invC, R)
     wts = torch.arange(1, 6, dtype=torch.float64, device=dev)
     (L * wts).sum().backward()
-    _, g, _ = source_derivs(gl.analytic_orbit(), synth.YEAR_T, params.detach(), BW, imin, invC, R)
-    assert torch.allclose(params.grad, wts[:, None] * g, rtol=0, atol=0)
+    L2, g, F = source_derivs(gl.analytic_orbit(), synth.YEAR_T, params.detach(), BW, imin, invC, R)
+    # the backward is the adjoint kernel's gradient: the forward-mode one to
+    # rounding, in the Fisher metric
+    sig = torch.sqrt(torch.diagonal(F, dim1=1, dim2=2))
+    assert torch.max(torch.abs(params.grad - wts[:, None] * g) / (wts[:, None] * sig)).item() < 1e-10
+    assert torch.max(torch.abs(L.detach() - L2) / torch.abs(L2)).item() < 1e-12
+    # without a gradient to compute, ucb_source_logl runs the likelihood only
+    from glass_amd.derivs import ucb_source_logl
+    with torch.no_grad():
+        L3 = ucb_source_logl(params, gl.analytic_orbit(), synth.YEAR_T, BW, imin, invC, R)
+    assert torch.equal(L3, L.detach())
+    L4 = ucb_source_logl(params, gl.analytic_orbit(), synth.YEAR_T, BW, imin, invC, R)
+    assert L4.requires_grad and torch.equal(L4.detach(), L.detach())
+
+
+@pytest.mark.parametrize("BW,W,fmt,mode", [(32, 64, "sangria", 0), (64, 128, "phase", 0),
+                                            (128, 256, "frequency", 1), (256, 512, "sangria", 0),
+                                            (512, 1024, "sangria", 1), (1024, 2048, "phase", 0)])
+def test_adjoint_gradient_vs_forward_mode(gl, orc, BW, W, fmt, mode):
+    """The reverse-mode gradient (glass_hip_fd_source_grad) against the
+    forward-mode tangents (glass_hip_fd_source_derivs) on the same sources:
+    the same derivative by another route, so rounding only: 1e-10 in the
+    Fisher metric; L to 1e-12; every TDI format, both weight modes, with
+    and without a residual, sources with |cos theta| = 1 included."""
+    import torch
+    from glass_amd.batch import pack_invC
+    from glass_amd.derivs import source_derivs, source_grad
+    srcs = make_sources(orc, 6, BW + W, BW, W)
+    srcs[0]["p"][1] = 1.0
+    srcs[1]["p"][1] = -1.0
+    dev = torch.device("cuda")
+    params = torch.tensor(np.stack([s["p"] for s in srcs]), device=dev)
+    imin = torch.tensor([s["imin"] for s in srcs], dtype=torch.int32, device=dev)
+    invC = torch.tensor(np.stack([pack_invC(s["invC"]) for s in srcs]), device=dev)
+    for R in (torch.tensor(np.stack([s["R"] for s in srcs]), device=dev), None):
+        o = gl.analytic_orbit()
+        Lf, gf, F = source_derivs(o, T4, params, BW, imin, invC, R, fmt=fmt, mode=mode)
+        La, ga = source_grad(o, T4, params, BW, imin, invC, R, fmt=fmt, mode=mode)
+        L0, none = source_grad(o, T4, params, BW, imin, invC, R, fmt=fmt, mode=mode, want_grad=False)
+        assert none is None and torch.equal(L0, La)
+        Lf, gf, F, La, ga = (x.cpu().numpy() for x in (Lf, gf, F, La, ga))
+        sig = np.sqrt(np.abs(np.einsum("nii->ni", F)))
+        nr = np.sqrt(6.0 * BW + np.abs(F[:, 3, 3]))
+        err = np.max(np.abs(ga - gf) / (sig * nr[:, None]))
+        eL = np.max(np.abs(La - Lf) / (np.abs(F[:, 3, 3]) + np.sqrt(6.0 * BW * np.abs(F[:, 3, 3]))))
+        print(BW, fmt, mode, "resid" if R is not None else "no resid", "adjoint vs forward-mode:", err, eL)
+        assert np.all(np.isfinite(ga))
+        assert err < 1e-10 and eL < 1e-12
+
+
+def test_adjoint_gradient_vs_central_fd(gl, orc):
+    """The adjoint gradient against five-point central differences of the
+    oracle's L(theta) (the forward-mode test's bound and steps)."""
+    import torch
+    from glass_amd.batch import pack_invC
+    from glass_amd.derivs import source_grad
+    BW, W = 256, 512
+    srcs = make_sources(orc, 4, 19, BW, W)
+    dev = torch.device("cuda")
+    L, g = source_grad(gl.analytic_orbit(), T4, torch.tensor(np.stack([s["p"] for s in srcs]), device=dev), BW,
+                       torch.tensor([s["imin"] for s in srcs], dtype=torch.int32, device=dev),
+                       torch.tensor(np.stack([pack_invC(s["invC"]) for s in srcs]), device=dev),
+                       torch.tensor(np.stack([s["R"] for s in srcs]), device=dev))
+    _, _, F = run_derivs(gl, srcs, T4, BW, W)
+    L, g = L.cpu().numpy(), g.cpu().numpy()
+    for e, s in enumerate(srcs):
+        Lref = oracle_L(orc, s["p"], s, T4, BW, W)
+        assert abs(L[e] - Lref) <= 1e-10 * max(abs(Lref), 1.0)
+        gfd = np.array([stencil(lambda x: oracle_L(orc, x, s, T4, BW, W), s["p"], k) for k in range(8)])
+        err = np.abs(g[e] - gfd) / np.sqrt(np.diag(F[e]))
+        print(e, "adjoint grad err / sigma^-1:", err)
+        assert err.max() < 2e-5

@@ -3,6 +3,8 @@
 Gates (the reference's FP64 CPU output as restated by the oracle):
 waveform bins normwise max|dh|/max|h| <= 1e-12; logL, snr, nwip relative
 error <= 1e-10."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -139,3 +141,48 @@ def test_invert_noise_covariance(gl, orc):
     invC, ld = gl.invert_noise_covariance_matrix(C)
     assert np.max(np.abs(invC - invC_ref) / np.abs(invC_ref)) < 1e-13
     assert np.max(np.abs(ld - ld_ref)) < 1e-12
+
+
+def _orbit_file(orc, T, t0, N=400, seed=3):
+    """An orbit 'file': knots over [t0 - T/(N-1), t0 + T + T/(N-1)] with the
+    analytic positions plus a smooth few-thousand-km wobble per spacecraft
+    (so the constellation is not the rigid analytic one)."""
+    o = orc.analytic_orbit()
+    buf = T / (N - 1)
+    t = t0 - buf + np.arange(N) * (T + 2 * buf) / (N - 1)
+    rng = np.random.default_rng(seed)
+    pos = np.zeros((9, N))
+    for n in range(N):
+        x, y, z = np.zeros(4), np.zeros(4), np.zeros(4)
+        orc.lib().orc_analytic_orbits(ctypes.byref(o), t[n], orc._d(x), orc._d(y), orc._d(z))
+        for i in range(3):
+            pos[3 * i:3 * i + 3, n] = x[i + 1], y[i + 1], z[i + 1]
+    ph = rng.uniform(0, 2 * np.pi, 9)
+    pos += 3e6 * np.sin(2 * np.pi * t[None, :] / (0.37 * 3.15581497632e7) + ph[:, None])
+    return t, pos
+
+
+@pytest.mark.parametrize("fmt", ["sangria", "phase", "frequency"])
+def test_numeric_orbit_waveform_vs_oracle(gl, orc, fmt):
+    """ucb_waveform with a numeric orbit (interpolate_orbits' splines over an
+    orbit file's knots) against the oracle on the same knots: the average
+    arm length bitwise, templates 1e-12 normwise for BW 32..2048; the orbit
+    differs from the analytic one, so the analytic template differs."""
+    T = synth.YEAR_T
+    t, pos = _orbit_file(orc, T, 0.0)
+    no = gl.numeric_orbit(t, pos[0::3], pos[1::3], pos[2::3])
+    with orc.numeric_orbit(t, pos) as onum:
+        assert no.L == onum.orbit.L and no.fstar == onum.orbit.fstar
+        P = synth.draw_params(np.random.default_rng(9), 6, T)
+        worst = 0.0
+        for BW, p in zip((32, 64, 256, 512, 1024, 2048), P):
+            got = gl.ucb_waveform_numeric(no, fmt, T, 0.0, p, 8, BW)
+            ref = orc.ucb_waveform(p, T, BW, fmt=fmt, orbit=onum.orbit)
+            n = 5 if fmt == "sangria" else 1
+            worst = max(worst, normwise(got[:n], ref[:n]))
+    ana = orc.ucb_waveform(P[-1], T, 2048, fmt=fmt)
+    print(fmt, "numeric-orbit waveform worst normwise", worst)
+    assert worst < 1e-12
+    assert normwise(got[:1], ana[:1]) > 1e-6
+    with pytest.raises(gl.GlassHipError):
+        gl.ucb_waveform_numeric(no, fmt, 2 * T, 0.0, P[0], 8, 64)   # samples beyond the knots

@@ -269,3 +269,54 @@ def test_steps_with_fstat_proposals_match_oracle(gl, orc):
     assert rel(logl, lref) < 1e-10
     smp.close()
     plan.close()
+
+
+def test_steps_with_bw4096_sources_match_oracle(gl, orc):
+    """Loud high-frequency sources in 4096-bin windows (BW 4096 templates):
+    their exact Fisher matrices are finite and positive definite, so the
+    Fisher proposal draws along real eigen-directions (no 0.01 x fallback
+    of a failed decomposition, ucb_sampler.hip's guard), and the device
+    steps match the oracle step for step."""
+    from glass_amd.batch import ucb_alignment
+    from glass_amd.sampler import UcbSampler
+    cfg = make_cfg(orc, seed=71, ngroup=2, ntemp=3, nprop=1, windows=(4096,), f_lo=15e-3, f_hi=18e-3)
+    T = cfg["T"]
+    o = gl.analytic_orbit()
+    loud = np.log(6e-21)
+    cfg["src_params"][:, 0, 3] = loud
+    C = len(cfg["chain_group"])
+    bws = []
+    for c in range(C):
+        g = int(cfg["chain_group"][c])
+        bws.append(ucb_alignment(o, T, int(cfg["group_W"][g]), int(cfg["group_qmin"][g]),
+                                 cfg["src_params"][c, 0])[0])
+    print("BW of the loud sources", bws)
+    assert sum(b == 4096 for b in bws) >= 2
+    plan = new_plan(gl, cfg, C)
+    pr = priors(cfg)
+    smp = UcbSampler(plan, 2, 3, pr, weights=W_DEFAULT, snr_min=1.0)
+    rng = np.random.default_rng(72)
+    seeds = rng.integers(1, 2 ** 32, C, dtype=np.uint64).astype(np.uint32)
+    temps = np.tile(1.25 ** np.arange(3), 2)
+    smp.set_state(seeds, temps)
+    smp.fisher()
+    ev, la = smp.eigen()
+    for c in range(C):
+        n = int(cfg["chain_nsrc"][c])
+        assert np.all(np.isfinite(la[c, :n])) and np.all(la[c, :n] > 0), (c, la[c, :n])
+        assert np.all(np.isfinite(ev[c, :n]))
+    logl0, _ = plan.state()
+    nsteps = 12
+    smp.steps(nsteps)
+    st = smp.state()
+    logl, sp = plan.state()
+    params, lref, sref, trials, accepts = oracle_steps(orc, cfg, pr, seeds, temps, ev, la, nsteps, 1.0, logl0)
+    print("accepts", st["accepts"].sum(axis=0), "trials", st["trials"].sum(axis=0))
+    assert st["trials"][:, 4].sum() > 0
+    assert np.array_equal(st["trials"], trials)
+    assert np.array_equal(st["accepts"], accepts)
+    assert np.array_equal(st["seeds"], sref.astype(np.uint32))
+    assert rel(sp, params) < 1e-12
+    assert rel(logl, lref) < 1e-10
+    smp.close()
+    plan.close()

@@ -214,12 +214,100 @@ def test_logl_wavelet_well_conditioned(orc, W, g2):
     assert abs(L - L_ref) < 1e-10 * abs(L_ref)
 
 
+@pytest.fixture(scope="module")
+def gref(orc, gl, W):
+    """The reference's default wavelet grid at Tobs = 1 yr (initialize_wavelet,
+    utils/src/glass_wavelet.c:210-216): NT = ceil(T / 7680 s) = 4096 pixels
+    of 7680 s, NF = 7680 s / 5 s = 1536 layers of 1/15360 Hz."""
+    NF, NT = 1536, 4096
+    ow, gw = grids(orc, W, NF, NT, 7680.0)
+    assert abs(gw.df - 6.51041666666667e-5) < 1e-15 and NT * 7680.0 == T1
+    return ow, gw, orc.iorbit(T1), W.interpolated_analytic_orbit(gl.analytic_orbit(), T1)
+
+
+def test_ucb_waveform_wavelet_reference_grid(orc, W, gref):
+    """Templates on the reference's own grid (NT 4096: the long-series layer
+    kernel): identical pixel lists, values against the correctly rounded
+    oracle to 1e-12 normwise and against glibc within its envelope."""
+    ow, gw, oorb, gorb = gref
+    P = synth.draw_params(np.random.default_rng(23), 24, T1)
+    # a layer boundary inside [f0 (1 - v/c), f0 (1 + v/c)]: three layers
+    # (an NT x 4 = 16384-sample series); the others take two (12288)
+    for i in range(4):
+        P[i, 0] = int(P[i, 0] / T1 / gw.df) * gw.df * T1
+        P[i, 7] = 0.0
+    X = np.zeros(gw.npix)
+    Y, Z = np.zeros_like(X), np.zeros_like(X)
+    worst, layers = 0.0, []
+    for e, p in enumerate(P):
+        lst, v, vcr = oracle_templates(orc, ow, oorb, p)
+        layers.append(orc.ucb_wavelet_layers(ow, T1, p)[1])
+        glst, X, Y, Z = W.ucb_waveform_wavelet(gw, gorb, T1, p, X=X, Y=Y, Z=Z)
+        assert np.array_equal(glst, lst), e
+        got = np.stack([X[glst], Y[glst], Z[glst]])
+        assert normwise(got, v) <= 1e-12 + normwise(v, vcr), e
+        worst = max(worst, normwise(got, vcr))
+        X[glst] = Y[glst] = Z[glst] = 0.0
+    print("reference grid: layers per template", sorted(set(layers)), "worst normwise vs CR oracle", worst)
+    assert set(layers) == {2, 3}
+    assert worst < 1e-12
+
+
+@pytest.mark.parametrize("NF,NT", [(2048, 1024), (1024, 2048)])
+def test_ucb_waveform_wavelet_long_series_grids(orc, gl, W, NF, NT):
+    """The long-series layer kernel at NT = 1024 and 2048 (two and three
+    layers): lists identical, values 1e-12 normwise against the CR oracle."""
+    ow, gw = grids(orc, W, NF, NT, T1 / NT)
+    oorb, gorb = orc.iorbit(T1), W.interpolated_analytic_orbit(gl.analytic_orbit(), T1)
+    P = synth.draw_params(np.random.default_rng(NT + 1), 8, T1)
+    for i in range(3):
+        P[i, 0] = int(P[i, 0] / T1 / gw.df) * gw.df * T1
+        P[i, 7] = 0.0
+    worst, layers = 0.0, set()
+    for e, p in enumerate(P):
+        lst, v, vcr = oracle_templates(orc, ow, oorb, p)
+        layers.add(orc.ucb_wavelet_layers(ow, T1, p)[1])
+        glst, X, Y, Z = W.ucb_waveform_wavelet(gw, gorb, T1, p)
+        assert np.array_equal(glst, lst), e
+        worst = max(worst, normwise(np.stack([X[glst], Y[glst], Z[glst]]), vcr))
+    print(NF, NT, "layers", sorted(layers), "worst normwise vs CR oracle", worst)
+    assert worst < 1e-12
+
+
+def test_wavelet_batch_reference_grid(orc, W, gref):
+    """The batched templates with fused wavelet_nwip on the reference's grid
+    (6.3 M pixels, well-conditioned random pixel covariance): lists, values,
+    dlogL and snr against the oracle."""
+    import torch
+    ow, gw, oorb, gorb = gref
+    d, ic9 = synth.wdm_random_noise(ow.NF, ow.NT, ow.kmin, ow.kmax, 29)
+    ic6 = ic9[[0, 4, 8, 1, 2, 5]]
+    P = synth.draw_params(np.random.default_rng(31), 40, T1)
+    B = W.WdmBatch(gw, gorb, T1, d, ic6, max_templates=40)
+    dl, sn = B.eval(torch.tensor(P, device="cuda"))
+    dl, sn = dl.cpu().numpy(), sn.cpu().numpy()
+    with orc.cr_trig():
+        ref_dl, ref_sn, ref_nw, _ = orc.wdm_eval_batch(ow, oorb, T1, P, d, ic9)
+    nw, lst, val = B.templates(40)
+    assert np.array_equal(nw, ref_nw)
+    for e in range(0, 40, 8):
+        olst, ov, ovcr = oracle_templates(orc, ow, oorb, P[e])
+        assert np.array_equal(lst[e, :nw[e]], olst)
+        assert normwise(val[e, :, :nw[e]], ovcr) < 1e-12
+    rel_dl = np.max(np.abs(dl - ref_dl) / np.maximum(np.abs(ref_dl), 1.0))
+    rel_sn = np.max(np.abs(sn - ref_sn) / ref_sn)
+    print("reference grid batch: dlogL rel", rel_dl, "snr rel", rel_sn)
+    assert rel_dl < 1e-10 and rel_sn < 1e-10
+    B.close()
+
+
 def test_wavelet_bad_inputs(gl, W, g2):
     from glass_amd import GlassHipError
     ow, gw, oorb, gorb = g2
-    bad = W.Wdm.grid(4096, 1024, T1 / 1024)
-    with pytest.raises(GlassHipError):
-        W.ucb_waveform_wavelet(bad, gorb, T1, synth.draw_params(np.random.default_rng(1), 1, T1)[0])
+    for NT in (8192, 768):
+        bad = W.Wdm.grid(256, NT, T1 / NT)
+        with pytest.raises(GlassHipError):
+            W.ucb_waveform_wavelet(bad, gorb, T1, synth.draw_params(np.random.default_rng(1), 1, T1)[0])
     with pytest.raises(GlassHipError):
         X = np.zeros(8)
         W.snr_wavelet(X, X, X, np.zeros((9, 8)), [0, 9])
@@ -307,3 +395,21 @@ def test_wdm_batch_fisher_matches_dropin(orc, W, g2):
     with pytest.raises(GlassHipError):
         B.fisher(torch.tensor(np.tile(P, (2, 1)), device="cuda"))   # 48 sources > 384 / 16
     B.close()
+
+
+@pytest.mark.parametrize("NF,NT", [(4096, 512), (1536, 4096), (64, 32)])
+def test_stationary_psd_to_wdm_vs_oracle(orc, W, NF, NT):
+    """stationary_dft_psd_to_wdm_psd on the device: layer variances against
+    the oracle's running sum (all terms >= 0: relative 1e-13), every pixel
+    of the NF x NT map from its layer (row 0: even n layer 0, odd n NF)."""
+    ow, gw = grids(orc, W, NF, NT)
+    ND = NF * NT
+    rng = np.random.default_rng(NF + NT)
+    k = np.arange(ND // 2 + 1)
+    psd = (1.0 + (k + 1.0) ** -1.5 * 1e3) * rng.uniform(0.5, 1.5, ND // 2 + 1)
+    ref = orc.wdm_layer_var(ow, psd)
+    lv = W.stationary_dft_psd_to_wdm_layer_var(gw, psd)
+    assert np.max(np.abs(lv - ref) / ref) < 1e-13
+    pix = W.stationary_dft_psd_to_wdm_psd(gw, psd).reshape(NF, NT)
+    assert np.array_equal(pix[1:], np.repeat(lv[1:NF, None], NT, axis=1))
+    assert np.all(pix[0, 0::2] == lv[0]) and np.all(pix[0, 1::2] == lv[NF])

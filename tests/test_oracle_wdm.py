@@ -359,3 +359,22 @@ def test_fisher_wavelet_vs_five_point(orc, grid2):
         print("five-point vs oracle FD Fisher, normalised", err)
         assert err < 1e-3, err
         del pos
+
+
+def test_wdm_layer_var_white_closed_form(orc):
+    """The Meyer window's normalisation makes the WDM basis orthonormal:
+    a white DFT variance of ND (unit-variance samples) gives interior layer
+    variance 1 exactly, and each edge layer the sum of its one-sided
+    window weights (l = 0 quartered) over NT^2; linear in the PSD."""
+    for NF, NT in ((64, 32), (256, 128), (1536, 64)):
+        w = orc.Wdm(NF, NT, 100.0)
+        ND = NF * NT
+        lv = orc.wdm_layer_var(w, np.full(ND // 2 + 1, float(ND)))
+        assert np.max(np.abs(lv[1:NF] - 1.0)) < 1e-12
+        phif = orc.wdm_filter(NF, NT)
+        edge = (0.25 * phif[0] ** 2 + np.sum(phif[1:NT // 2] ** 2)) * ND / NT ** 2
+        assert abs(lv[0] - edge) < 1e-12 * edge and abs(lv[NF] - edge) < 1e-12 * edge
+        rng = np.random.default_rng(NF)
+        a, b = rng.uniform(1, 2, ND // 2 + 1), rng.uniform(1, 2, ND // 2 + 1)
+        lab = orc.wdm_layer_var(w, 2 * a + 3 * b)
+        assert np.max(np.abs(lab - (2 * orc.wdm_layer_var(w, a) + 3 * orc.wdm_layer_var(w, b))) / lab) < 1e-13
