@@ -14,7 +14,9 @@
 // partials land in plan memory and a second kernel sums each item's chunks
 // in chunk order.  Results are deterministic run to run.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "../../include/glass_hip_batch.h"
@@ -86,6 +88,67 @@ __global__ void logl_finish_kernel(int n, const LoglItem* __restrict__ items, co
     double s = 0.0;
     for (int c = it.c0; c < it.c1; c++) s += part[c];
     logl[e] = -s;  // -chi^2/2 with chi^2 = 2 * sum
+}
+
+// ---- allocation-free form: a fixed grid of cpi chunks per item --------
+//
+// Item e owns chunks e*cpi .. e*cpi + cpi - 1 of kChunk bins each (cpi =
+// ceil(max_W / kChunk)); chunk k covers bins [k kChunk, (k+1) kChunk) of the
+// item's window, as the plan's chunk table does, so the partial sums and
+// their order are the plan's.  The item table is read from the caller's
+// device arrays; the partials live in the file-scope array below.
+
+constexpr long long kNoAllocParts = 1 << 20;  // partials: 8 MiB per device
+__device__ __align__(256) double scratch[kNoAllocParts];
+
+__global__ __launch_bounds__(kLThr) void logl_chunk_grid_kernel(int cpi, int max_W, const int* __restrict__ Wv,
+                                                                const long long* __restrict__ doff,
+                                                                const long long* __restrict__ moff,
+                                                                const long long* __restrict__ ioff,
+                                                                const double2* __restrict__ data,
+                                                                const double2* __restrict__ model,
+                                                                const double* __restrict__ invC,
+                                                                double* __restrict__ part)
+{
+    const int e = blockIdx.x / cpi, k = blockIdx.x - e * cpi;
+    const int Wi = Wv[e];
+    const int b0 = k * kChunk;
+    const int nb = (Wi > 0 && Wi <= max_W) ? min(kChunk, Wi - b0) : 0;
+    if (nb <= 0)
+    {
+        if (threadIdx.x == 0) part[blockIdx.x] = 0.0;
+        return;
+    }
+    const size_t W = (size_t)Wi;
+    const double2* D = data + 3 * doff[e] + b0;
+    const double2* M = model + 3 * moff[e] + b0;
+    const double* I = invC + 6 * ioff[e] + b0;
+    double acc = 0.0;
+#pragma unroll
+    for (int j = 0; j < kPerThr; j++)
+    {
+        const int n = threadIdx.x + j * kLThr;
+        if (n < nb) acc += bin_chi2(D, M, I, W, n);
+    }
+    acc = block_sum<kLThr>(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+__global__ void logl_finish_grid_kernel(int n, int cpi, int max_W, const int* __restrict__ Wv,
+                                        const double* __restrict__ part, double* __restrict__ logl)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int Wi = Wv[e];
+    if (Wi <= 0 || Wi > max_W)
+    {
+        logl[e] = NAN;  // a window the call's max_W does not cover
+        return;
+    }
+    const int nck = (Wi + kChunk - 1) / kChunk;
+    double s = 0.0;
+    for (int c = 0; c < nck; c++) s += part[(size_t)e * cpi + c];
+    logl[e] = -s;
 }
 
 }  // namespace glass
@@ -176,6 +239,65 @@ extern "C" int glass_hip_logl_batch_eval(glass_hip_logl_batch* P, const double* 
     logl_chunk_kernel<<<P->nchunk, kLThr, 0, s>>>(P->chunks, P->items, (const double2*)d_data,
                                                   (const double2*)d_model, d_invC, P->part);
     logl_finish_kernel<<<(P->n + 255) / 256, 256, 0, s>>>(P->n, P->items, P->part, d_logL);
+    GH_CHECK(hipGetLastError());
+    return GLASS_HIP_OK;
+}
+
+// Device address of `scratch` on each device, looked up once per device
+// with that device current (host code never passes the symbol itself).
+static std::atomic<double*> g_scratch_ptr[64];
+static std::mutex g_scratch_mu;
+
+static int scratch_ptr(double** out)
+{
+    int dev = 0;
+    GH_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64)
+    {
+        set_error("logl_batch_eval_noalloc: device %d beyond the 64 this library tracks", dev);
+        return GLASS_HIP_ENODEV;
+    }
+    double* p = g_scratch_ptr[dev].load(std::memory_order_acquire);
+    if (!p)
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        p = g_scratch_ptr[dev].load(std::memory_order_relaxed);
+        if (!p)
+        {
+            void* q = nullptr;
+            GH_CHECK(hipGetSymbolAddress(&q, HIP_SYMBOL(glass::scratch)));
+            p = (double*)q;
+            g_scratch_ptr[dev].store(p, std::memory_order_release);
+        }
+    }
+    *out = p;
+    return GLASS_HIP_OK;
+}
+
+extern "C" int glass_hip_logl_batch_eval_noalloc(int n, const int* d_W, const long long* d_data_off,
+                                                 const long long* d_model_off, const long long* d_invC_off,
+                                                 int max_W, const double* d_data, const double* d_model,
+                                                 const double* d_invC, double* d_logL, void* stream)
+{
+    if (n <= 0 || max_W <= 0 || !d_W || !d_data_off || !d_model_off || !d_invC_off || !d_data || !d_model ||
+        !d_invC || !d_logL)
+    {
+        set_error("logl_batch_eval_noalloc: bad arguments (n %d, max_W %d)", n, max_W);
+        return GLASS_HIP_EINVAL;
+    }
+    const int cpi = (max_W + kChunk - 1) / kChunk;
+    if ((long long)n * cpi > kNoAllocParts)
+    {
+        set_error("logl_batch_eval_noalloc: %d items x %d chunks of %d bins exceed the %lld partials of the "
+                  "static scratch", n, cpi, kChunk, kNoAllocParts);
+        return GLASS_HIP_EINVAL;
+    }
+    double* part = nullptr;
+    if (int rc = scratch_ptr(&part)) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    logl_chunk_grid_kernel<<<n * cpi, kLThr, 0, s>>>(cpi, max_W, d_W, d_data_off, d_model_off, d_invC_off,
+                                                     (const double2*)d_data, (const double2*)d_model, d_invC, part);
+    logl_finish_grid_kernel<<<(n + 255) / 256, 256, 0, s>>>(n, cpi, max_W, d_W, part, d_logL);
     GH_CHECK(hipGetLastError());
     return GLASS_HIP_OK;
 }

@@ -260,6 +260,24 @@ int glass_hip_logl_batch_eval(glass_hip_logl_batch *plan, const double *d_data,
                               const double *d_model, const double *d_invC, double *d_logL,
                               void *stream);
 
+/* glass_hip_logl_batch_eval without a plan and without allocating: no
+ * counterpart in the reference; for callers that cannot call hipMalloc
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * allocation).  The item table is passed as device arrays (d_W[n] and the
+ * three offset arrays, in bins, as glass_hip_logl_batch_create takes them on
+ * the host) with max_W >= every W[e]; the chunk partial sums go to a static
+ * device array of 2^20 doubles (8 MiB per GPU, held until the process ends):
+ * n * ceil(max_W / 2048) <= 2^20, else GLASS_HIP_EINVAL.  Results equal
+ * glass_hip_logl_batch_eval's bit for bit (same chunks, same order); an item
+ * with W[e] <= 0 or W[e] > max_W gets NaN.  Enqueues two kernels on stream,
+ * no allocation, copy or synchronisation.  Every call on one GPU shares its
+ * scratch array: calls that overlap on one GPU (other streams, other host
+ * threads) race and must be ordered by the caller. */
+int glass_hip_logl_batch_eval_noalloc(int n, const int *d_W, const long long *d_data_off,
+                                      const long long *d_model_off, const long long *d_invC_off,
+                                      int max_W, const double *d_data, const double *d_model,
+                                      const double *d_invC, double *d_logL, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
