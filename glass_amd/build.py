@@ -27,6 +27,27 @@ FLAGS = [
     "-mllvm", "-disable-machine-licm",
 ]
 EXTRA = os.environ.get("GLASS_HIP_EXTRA_FLAGS", "").split()
+# the LDC HDF5 readers: host code, linked against libhdf5 (HDF5_DIR/include,
+# HDF5_DIR/lib; the image's copy lives under /opt/conda)
+LDC_LIB = os.path.join(HERE, "libglass_ldc.so")
+HDF5_DIR = os.environ.get("HDF5_DIR", "/opt/conda")
+
+
+def build_ldc(verbose=False):
+    """Builds libglass_ldc.so (g++, -lhdf5 with an rpath to HDF5_DIR/lib).
+    Returns its path, or None when hdf5.h is not under HDF5_DIR."""
+    inc, libdir = os.path.join(HDF5_DIR, "include"), os.path.join(HDF5_DIR, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")):
+        print(f"libglass_ldc.so not built: no hdf5.h under {inc}", flush=True)
+        return None
+    tmp = LDC_LIB + ".tmp"
+    cmd = ["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-Wall", "-I" + inc,
+           os.path.join(CSRC, "ldc_hdf5.cpp"), "-o", tmp, "-L" + libdir, "-Wl,-rpath," + libdir, "-lhdf5"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LDC_LIB)
+    return LDC_LIB
 
 
 def build(verbose=False, jobs=None):
@@ -60,4 +81,5 @@ def build(verbose=False, jobs=None):
 
 if __name__ == "__main__":
     build(verbose=True)
+    build_ldc(verbose=True)
     sys.exit(0)
