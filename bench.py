@@ -99,6 +99,79 @@ def plain_rate(gl, synth, seconds=2.0):
     return n / (time.perf_counter() - t0)
 
 
+def plain_leg(gl, cfg, logL_batched, seconds=2.0, nthreads=16, seed=5):
+    """The plain method on the same workload: a sample of this sweep's
+    proposals (all proposals of 32 random chains), each evaluated as the
+    reference's sampler does per proposal (glass_ucb_sampler.c:139-155) but
+    through the per-call drop-ins on host memory: ucb_alignment, the
+    ucb_waveform drop-in, the window model (the chain's other sources'
+    stored templates plus the new one), the gaussian_log_likelihood drop-in
+    and the snr drop-in.  Timed from one host thread and from nthreads host
+    threads (the reference calls these from its OpenMP threads over chains,
+    apps/src/ucb_mcmc.c:230; each thread has its own HIP stream).  Also
+    checks that the drop-in logL equals the batched one."""
+    from concurrent.futures import ThreadPoolExecutor
+    from glass_amd.batch import ucb_alignment
+    o = gl.analytic_orbit()
+    T = cfg["T"]
+    rng = np.random.default_rng(seed)
+    C = len(cfg["chain_group"])
+    chains = rng.choice(C, size=min(32, C), replace=False)
+    base = {}
+    for c in chains:
+        g = int(cfg["chain_group"][c])
+        W, qmin = int(cfg["group_W"][g]), int(cfg["group_qmin"][g])
+        n = int(cfg["chain_nsrc"][c])
+        tm = []
+        for s_ in range(n):
+            b, _, i = ucb_alignment(o, T, W, qmin, cfg["src_params"][c, s_])
+            h = gl.ucb_waveform(o, "sangria", T, 0.0, cfg["src_params"][c, s_], 8, b)
+            m = np.zeros((5, W), complex)
+            lo, hi = max(0, i), min(W, i + b)
+            m[:, lo:hi] = h[:, lo - i:hi - i]
+            tm.append(m)
+        tot = np.sum(tm, axis=0)
+        d = np.zeros((5, W), complex)
+        d[:3] = cfg["data"][g]
+        base[int(c)] = dict(W=W, qmin=qmin, data=d, invC=np.ascontiguousarray(cfg["invC"][g]),
+                            minus=[tot - t for t in tm])
+    sel = np.nonzero(np.isin(cfg["p_chain"], chains))[0]
+    rng.shuffle(sel)
+
+    def one(q):
+        c, s_ = int(cfg["p_chain"][q]), int(cfg["p_src"][q])
+        p = cfg["p_params"][q]
+        B = base[c]
+        W = B["W"]
+        b, _, i = ucb_alignment(o, T, W, B["qmin"], p)
+        h = gl.ucb_waveform(o, "sangria", T, 0.0, p, 8, b)
+        m = B["minus"][s_].copy()
+        lo, hi = max(0, i), min(W, i + b)
+        m[:, lo:hi] += h[:, lo - i:hi - i]
+        ll = gl.gaussian_log_likelihood(B["data"], m, B["invC"])
+        gl.snr(h[0], h[1], h[2], B["invC"], b)
+        return ll
+
+    ll = np.array([one(q) for q in sel[:64]])
+    rel = float(np.max(np.abs(ll - logL_batched[sel[:64]]) / np.abs(logL_batched[sel[:64]])))
+
+    def timed(nt):
+        k, t0 = 0, time.perf_counter()
+        if nt == 1:
+            while time.perf_counter() - t0 < seconds:
+                one(sel[k % len(sel)])
+                k += 1
+        else:
+            with ThreadPoolExecutor(nt) as ex:
+                while time.perf_counter() - t0 < seconds:
+                    list(ex.map(one, sel[:4 * nt] if len(sel) >= 4 * nt else sel))
+                    k += min(len(sel), 4 * nt)
+        return k / (time.perf_counter() - t0)
+
+    return dict(rate1=timed(1), rateN=timed(nthreads), nthreads=nthreads, n=len(sel), nchains=len(chains),
+                max_rel_vs_batched=rel)
+
+
 def fisher_leg(gl, synth, n=8192, BW=256, W=512, steps=5, seed=3):
     """Exact-gradient Fisher evaluations (adjoint gradients case): n sources,
     BW 256, Tobs 4 yr, each with its own 512-bin window (noise residual and
@@ -143,6 +216,27 @@ def fisher_leg(gl, synth, n=8192, BW=256, W=512, steps=5, seed=3):
     e1.record()
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / steps
+    # reverse mode: L + dL/dtheta from the adjoint kernel on the same
+    # sources (one backward pass instead of the forward-mode tangents)
+    from glass_amd.derivs import source_grad
+    Lf, gf, _ = source_derivs(o, T, params, BW, im, invC, R)
+    La, ga = source_grad(o, T, params, BW, im, invC, R)
+    torch.cuda.synchronize()
+    scale = torch.clamp(gf.abs().amax(dim=1, keepdim=True), min=1e-300)
+    grad_err = float(((ga - gf).abs() / scale).max())
+    L_err = float(((La - Lf).abs() / Lf.abs().clamp(min=1e-300)).max())
+    e0.record()
+    for _ in range(steps):
+        source_grad(o, T, params, BW, im, invC, R)
+    e1.record()
+    torch.cuda.synchronize()
+    ms_grad = e0.elapsed_time(e1) / steps
+    e0.record()
+    for _ in range(steps):
+        source_grad(o, T, params, BW, im, invC, R, want_grad=False)
+    e1.record()
+    torch.cuda.synchronize()
+    ms_L = e0.elapsed_time(e1) / steps
     # plain: the per-call forward-difference drop-in on host arrays
     invC33 = np.zeros((3, 3, W))
     nplain, t0 = 0, time.perf_counter()
@@ -154,7 +248,8 @@ def fisher_leg(gl, synth, n=8192, BW=256, W=512, steps=5, seed=3):
         ucb_fisher(o, "sangria", T, 0.0, W, int(qmin[i]), P[i], invC33, eigen=False)
         nplain += 1
     plain = nplain / (time.perf_counter() - t0)
-    return dict(value=n / (ms * 1e-3), ms=ms, plain=plain, n=n)
+    return dict(value=n / (ms * 1e-3), ms=ms, plain=plain, n=n, ms_grad=ms_grad, ms_L=ms_L, grad_err=grad_err,
+                L_err=L_err)
 
 
 def wdm_leg(gl, synth, n=4096, steps=5, seed=7, cpu_seconds=6.0, nthreads=0, parity_n=64, plain=True):
@@ -675,7 +770,7 @@ def globalfit_leg(gl, synth, steps=3, seed=4, cpu_seconds=2.0, nthreads=0, seria
     return res
 
 
-def oracle_leg(cfg, logL_gpu, seconds, nthreads, seed):
+def oracle_leg(cfg, logL_gpu, seconds, nthreads, seed, variants=False):
     """CPU baseline (the oracle restatement of the reference, OpenMP over
     proposals) on a bounded sample of the same workload, and the parity of
     the GPU results for those proposals."""
@@ -729,9 +824,41 @@ def oracle_leg(cfg, logL_gpu, seconds, nthreads, seed):
     for _ in range(reps):
         _, s = orc.eval_batch(cs, T, ch, cfg["p_src"][sel], cfg["p_params"][sel], nthreads=nthreads)
         secs += s
-    return dict(rate=n * (reps + 1) / secs, n=n, reps=reps + 1, secs=secs, nchains=len(chains),
-                max_rel=float(rel.max()), max_scaled=float(scaled.max()),
-                max_rel_above_05mHz=float(rel[hi].max()) if hi.any() else None)
+    res = dict(rate=n * (reps + 1) / secs, n=n, reps=reps + 1, secs=secs, nchains=len(chains),
+               max_rel=float(rel.max()), max_scaled=float(scaled.max()),
+               max_rel_above_05mHz=float(rel[hi].max()) if hi.any() else None)
+    if variants:
+        # the same sample at the reference's release flags, and both builds
+        # on one core, each for about a quarter of `seconds`
+        res["variants"] = {"parity_flags": {str(nthreads): round(res["rate"], 2)}, "release_flags": {}}
+        for flags, release in (("release_flags", True), ("parity_flags", False)):
+            for nt in ((nthreads, 1) if release else (1,)):
+                k, tot = 0, 0.0
+                while tot < seconds / 4 or k == 0:
+                    _, s = orc.eval_batch(cs, T, ch, cfg["p_src"][sel], cfg["p_params"][sel], nthreads=nt,
+                                          release=release)
+                    tot += s
+                    k += 1
+                res["variants"][flags][str(nt)] = round(n * k / tot, 2)
+    return res
+
+
+def cpu_host():
+    """nproc, the cores this process may use and the CPU model name."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count()
+    return {"nproc": os.cpu_count(), "usable_cores": usable, "cpu_model": model}
 
 
 def main():
@@ -757,7 +884,9 @@ def main():
     ap.add_argument("--serial-classes", action="store_true",
                     help="run the BW-class kernels of each sweep in turn on one stream (profiling runs)")
     ap.add_argument("--globalfit", action="store_true",
-                    help="also run the global-fit block at full scale (1,024,000 chains, ~170 GB of HBM)")
+                    help="run the global-fit block (the default; kept for older command lines)")
+    ap.add_argument("--no-globalfit", action="store_true",
+                    help="skip the global-fit block at full scale (1,024,000 chains, ~170 GB of HBM)")
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="also run a 256 MiB device copy, the reference point that scales the "
                          "FETCH_SIZE/WRITE_SIZE counters of a profiled run")
@@ -841,14 +970,15 @@ def main():
     byts = kcount[kd] * eval_bytes(Nd)
     ach_tf = flops / (avg_ms * 1e-3) / 1e12
     ach_gbs = byts / (avg_ms * 1e-3) / 1e9
-    roofline = {"bound": "mfma", "pipe": "fp64 VALU (no MFMA: nothing on the path is a dense contraction)",
+    roofline = {"bound": "fp64-valu", "pipe": "FP64 vector ALU (no MFMA: nothing on the path is a dense contraction)",
                 "kernel": f"eval_kernel<{Nd},{tiles_now[kd][0]},{'true' if tiles_now[kd][1] else 'false'},"
                           f"{tiles_now[kd][2]}>",
                 "achieved": round(ach_tf, 4),
                 "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(ach_tf / FP64_PEAK_TFLOPS, 5),
                 "traffic": MEASURED_TRAFFIC.get(Nd),
-                "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes of this workload, calibrated "
-                                  "on a 256 MiB copy; bytes per launch (L2/MALL reuse keeps it below the "
+                "traffic_source": "constant, not measured in this run: profiles/eval_kernel_batched_fd_sweep.md "
+                                  "(rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes of this workload, calibrated "
+                                  "on a 256 MiB copy); bytes per launch (L2/MALL reuse keeps it below the "
                                   "algorithmic bytes)",
                 "avg_launch_ms": round(avg_ms, 4), "evals_per_launch": int(kcount[kd]),
                 "hbm_algorithmic_GBs": round(ach_gbs, 2), "hbm_frac": round(ach_gbs / HBM_PEAK_GBS, 5)}
@@ -862,10 +992,13 @@ def main():
         log("copy calibration: 256 MiB read + 256 MiB written")
         del a, b
 
-    plain = None
+    plain = plain0 = None
     if not args.no_plain:
-        plain = plain_rate(gl, synth)
-        log(f"plain per-call drop-ins: {plain:.1f} evals/s")
+        nthr_p = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+        plain = plain_leg(gl, cfg, logL, nthreads=nthr_p)
+        plain0 = plain_rate(gl, synth)
+        log(f"plain per-call drop-ins on this sweep's proposals: {plain['rate1']:.1f} evals/s from one host "
+            f"thread, {plain['rateN']:.1f} from {nthr_p}; config-0 case {plain0:.1f} evals/s")
 
     fisher = None
     if not args.no_fisher:
@@ -877,8 +1010,18 @@ def main():
                   "config": "8192 sources, BW 256, Tobs 4 yr, own 512-bin window each; one launch -> logL, "
                             "dlogL/dtheta[8], Fisher[8x8] per source; plain = per-call forward-difference "
                             "ucb_fisher drop-in"}
+        fisher["adjoint_gradient"] = {
+            "value": round(fl["n"] / (fl["ms_grad"] * 1e-3), 1), "unit": "logL + dlogL/dtheta[8] evals/s",
+            "ms_per_launch": round(fl["ms_grad"], 4), "ms_logL_only": round(fl["ms_L"], 4),
+            "vs_forward_mode": round(fl["ms"] / fl["ms_grad"], 2),
+            "parity_vs_forward_mode": {"max_grad_err_over_row_max": fl["grad_err"], "max_logL_rel_err": fl["L_err"],
+                                       "gate": {"grad": 1e-9, "logL": 1e-10},
+                                       "pass": bool(fl["grad_err"] <= 1e-9 and fl["L_err"] <= 1e-10)},
+            "method": "reverse mode (glass_hip_fd_source_grad): template forward, TDI and FFT transposed on the "
+                      "residual-weighted adjoint, one contraction per parameter; same 8192 sources"}
         log(f"exact Fisher: {fl['value']:.4e} evals/s ({fl['ms']:.3f} ms / {fl['n']}), "
-            f"plain FD ucb_fisher {fl['plain']:.1f}/s")
+            f"plain FD ucb_fisher {fl['plain']:.1f}/s; adjoint gradient {fl['ms_grad']:.3f} ms "
+            f"(logL only {fl['ms_L']:.3f} ms), grad err {fl['grad_err']:.2e}")
 
     wdm = None
     if not args.no_wdm:
@@ -962,15 +1105,22 @@ def main():
     parity = None
     if not args.no_cpu_baseline:
         nthr = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
-        leg = oracle_leg(cfg, logL, args.cpu_seconds, nthr, args.seed + 17)
-        cpu = {"value": round(leg["rate"], 2), "unit": "evals/s", "cores": nthr, "kind": "port",
+        leg = oracle_leg(cfg, logL, args.cpu_seconds / 2, nthr, args.seed + 17, variants=True)
+        rel = leg["variants"]["release_flags"][str(nthr)]
+        cpu = {"value": rel, "unit": "evals/s", "cores": nthr, "kind": "port",
+               "flags": "release: -O3 -march=x86-64-v3, FP contraction on (the reference's -O3 -march=native, "
+                        "CMakeLists.txt:37-39)",
                "sample": f"{leg['n']} proposals (all of {leg['nchains']} randomly chosen chains) of the "
-                         f"same sweep, evaluated {leg['reps']}x by the oracle restatement of "
-                         f"generate_ucb_model + gaussian_log_likelihood, OpenMP over proposals, "
-                         f"{leg['secs']:.1f} s"}
+                         f"same sweep, each evaluated by the oracle restatement of generate_ucb_model + "
+                         f"gaussian_log_likelihood, OpenMP over proposals (the reference's parallelism over "
+                         f"chains); about {args.cpu_seconds / 8:.1f} s per variant",
+               "variants_evals_per_s": {"release_flags": leg["variants"]["release_flags"],
+                                        "parity_flags (-O2 -ffp-contract=off)": leg["variants"]["parity_flags"]},
+               "host": cpu_host()}
         parity = {"checked": leg["n"], "max_logL_rel_err": leg["max_rel"], "gate": 1e-10,
                   "pass": bool(leg["max_rel"] <= 1e-10)}
-        log(f"cpu baseline {leg['rate']:.1f} evals/s on {nthr} threads; parity max rel {leg['max_rel']:.3e}")
+        log(f"cpu baseline {rel:.1f} evals/s on {nthr} threads (release flags), variants {leg['variants']}; "
+            f"parity max rel {leg['max_rel']:.3e}")
 
     fst = None
     if not args.no_fstat:
@@ -1014,7 +1164,7 @@ def main():
             f"{fmts['write_s']:.3f} s, read back in {fmts['read_s']:.3f} s ({fmts['sources_per_s']:.3e} sources/s)")
 
     gfit = None
-    if args.globalfit:
+    if not args.no_globalfit:
         plan.close()
         torch.cuda.empty_cache()
         nthr = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
@@ -1052,10 +1202,19 @@ def main():
                    "evals_per_step": E, "chains": len(cfg["chain_group"]), "format": "sangria",
                    "autotune": tune, "concurrent_classes": not args.serial_classes,
                    "graph_replay_ms_per_step": round(graph_ms, 4)},
-        "plain": None if plain is None else {"value": round(plain, 1), "unit": "evals/s",
-                                             "method": "per-call ucb_waveform + gaussian_log_likelihood "
-                                                       "drop-ins, host memory, BW 128, W 256"},
-        "batched_vs_plain": None if plain is None else round(value / plain, 1),
+        "plain": None if plain is None else {
+            "value": round(plain["rate1"], 1), "unit": "evals/s",
+            "threads": {"1": round(plain["rate1"], 1), str(plain["nthreads"]): round(plain["rateN"], 1)},
+            "method": "per-call drop-ins on host memory, per proposal: ucb_alignment, ucb_waveform, window model, "
+                      "gaussian_log_likelihood, snr; the same sweep's proposals (all proposals of %d chains, %d "
+                      "evaluations); from 1 host thread and from %d (the reference's OpenMP callers)"
+                      % (plain["nchains"], plain["n"], plain["nthreads"]),
+            "max_logL_rel_vs_batched": plain["max_rel_vs_batched"],
+            "config0_case": {"value": round(plain0, 1),
+                             "method": "the same drop-ins on the plain method's own case: 64 vectors, BW 128, "
+                                       "W 256, Tobs 1 yr, one host thread"}},
+        "batched_vs_plain": None if plain is None else round(value / plain["rate1"], 1),
+        "batched_vs_plain_threads": None if plain is None else round(value / plain["rateN"], 1),
         "fisher": fisher,
         "wdm": wdm,
         "inner_product": inner,
@@ -1068,7 +1227,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
-    if not args.globalfit:
+    if args.no_globalfit:
         plan.close()
     print(json.dumps(line), flush=True)
 

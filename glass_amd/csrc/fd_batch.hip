@@ -164,19 +164,6 @@ __global__ __launch_bounds__(256) void align_kernel(int n, Items it, BatchDev B,
 // register footprint stays that of one template), or a grid sized to a
 // multiple of the resident capacity striding over the class list.
 
-// The persistent loops rebuild a template per iteration from tables that do
-// not change between iterations (FFT twiddles, spacecraft positions).  Left
-// alone, the compiler hoists those loads out of the loop and keeps every
-// twiddle of every FFT stage live in registers (~100 extra VGPRs, halving
-// occupancy).  Passing the table pointers through an empty asm each
-// iteration makes the loads loop-variant again.
-template <typename T>
-__device__ inline const T* opaque(const T* p)
-{
-    asm volatile("" : "+s"(p));
-    return p;
-}
-
 // symmetric 3x3 weighted bilinear form of complex 3-vectors:
 // 2 * sum_ab Re(u_a conj v_b) w_ab, w planes XX,YY,ZZ,XY,XZ,YZ
 __device__ inline double bform(const double2* u, const double2* v, const double* w)
@@ -307,8 +294,9 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(MINW, 8)))
 // threads.
 #define GH_EVAL_TILES(X)                                                                                      \
     X(0, 32, 64, 1) X(1, 64, 64, 1) X(1, 64, 64, 6) X(2, 128, 64, 1) X(2, 128, 128, 1) X(2, 128, 128, 6)     \
-    X(3, 256, 128, 1) X(3, 256, 256, 1) X(3, 256, 256, 5) X(3, 256, 256, 6) X(4, 512, 256, 1)                \
-    X(4, 512, 512, 1) X(4, 512, 512, 6) X(5, 1024, 512, 1) X(5, 1024, 1024, 1) X(5, 1024, 256, 1)            \
+    X(3, 256, 128, 1) X(3, 256, 128, 4) X(3, 256, 256, 1) X(3, 256, 256, 5) X(3, 256, 256, 6)                \
+    X(4, 512, 256, 1) X(4, 512, 512, 1) X(4, 512, 512, 6) X(5, 1024, 512, 4) X(5, 1024, 512, 1)              \
+    X(5, 1024, 1024, 1) X(5, 1024, 256, 1)                                                                   \
     X(6, 2048, 512, 1) X(6, 2048, 1024, 1) X(7, 4096, 512, 1)
 
 // Template of each listed item into its source's slot.  LOOP: the grid
@@ -585,14 +573,19 @@ static int resident_per_cu(int N, int nthr, int waves)
     return per < 1 ? 1 : per;
 }
 
+// Waves per SIMD of the default tiles (the first candidate of each class):
+// 4 at 1024 bins, where the compiler's choice lands a few VGPRs above the
+// 128 that four waves allow.
+constexpr int batch_default_waves(int N) { return N == 1024 ? 4 : 1; }
+
 static int setup_grids(glass_hip_fd_batch* P)
 {
     GH_CHECK(hipDeviceGetAttribute(&P->ncu, hipDeviceAttributeMultiprocessorCount, P->device));
     for (int k = 0; k < kNClass; k++)
     {
         const int N = 32 << k;
-        P->tgrid[k] = P->ncu * resident_per_cu(N, fd_batch_threads(N), 1);
-        P->tile[k] = glass_hip_tile{fd_batch_threads(N), 0, 1};
+        P->tgrid[k] = P->ncu * resident_per_cu(N, fd_batch_threads(N), batch_default_waves(N));
+        P->tile[k] = glass_hip_tile{fd_batch_threads(N), 0, batch_default_waves(N)};
     }
     return GLASS_HIP_OK;
 }

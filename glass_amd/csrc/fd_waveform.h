@@ -171,7 +171,8 @@ __device__ inline void fd_link_spectra(const UcbConst& c, double2* lds, const do
 #pragma unroll 1
         for (int p = 0; p < P; p++)
         {
-            fd_samples_to_lds<N, NTHR, P>(c, lds, pos, posstride, p);
+            fd_samples_to_lds<N, NTHR, P>(c, lds, opaque(pos), posstride, p);
+            tw = opaque(tw);
             lds_fft<NS, 6, NTHR>(lds, tw, twstride * P);
 #pragma unroll
             for (int u = 0; u < U; u++)
@@ -261,7 +262,7 @@ __device__ inline void fd_link_bins(const UcbConst& c, double2* lds, const doubl
 #pragma unroll
         for (int pass = 0; pass < 2; pass++)
         {
-            lds_fft<H, 6, NTHR>(lds, tw, 2 * twstride);
+            lds_fft<H, 6, NTHR>(lds, opaque(tw), 2 * twstride);
             // bin m = 2t + pass is FFT index (m + H) mod N, of parity pass
             const int m = 2 * t + pass;
             const int s = ((m + H) & (N - 1)) >> 1;

@@ -24,6 +24,24 @@ __device__ inline double2 cmul(double2 a, double2 b)
     return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
+// Tables that do not change between templates (FFT twiddles, spacecraft
+// positions) are read through opaque(p) wherever a loop or a second pass
+// would read them again.  Left alone, the compiler keeps the values of the
+// first read live in registers until the second (every twiddle of every
+// FFT stage: ~30-100 VGPRs, halving occupancy).  Passing the pointer
+// through an empty asm makes each read a fresh load.  The pointer goes
+// through the asm as a global-memory pointer: a generic pointer of unknown
+// origin would turn every table read into a flat load, which the hardware
+// counts against the LDS wait counter as well, so each FFT stage's LDS
+// wait would also wait for its twiddle loads.
+template <typename T>
+__device__ inline const T* opaque(const T* p)
+{
+    auto g = (const __attribute__((address_space(1))) T*)p;
+    asm volatile("" : "+s"(g));
+    return (const T*)g;
+}
+
 // Radix-R butterfly (R = 2, 3, 4) of twiddled inputs v into y:
 // y[m] = sum_q v[q] exp(-2 pi i q m / R).
 template <int R>
@@ -63,13 +81,33 @@ __device__ inline void radix_butterfly(const double2 (&v)[R], double2 (&y)[R])
 
 // One Stockham stage of radix R over NL transforms; butterflies are spread
 // over the workgroup's NTHR threads.  Ns is the span of the previous stage.
+// The twiddles are loaded first, so their memory latency overlaps the LDS
+// reads and the barrier; when the workgroup spans whole series (NTHR a
+// multiple of N/R) every butterfly of a thread has the same index j and
+// shares one set.  Twiddle indices are in range for any b (j < N/R), so
+// the loads need no guard.
 template <int N, int NL, int R, int NTHR>
 __device__ inline void stockham_stage(double2* buf, int Ns, const double2* __restrict__ tw, int twstride)
 {
     constexpr int NB = NL * (N / R);                 // butterflies in the stage
     constexpr int PER = (NB + NTHR - 1) / NTHR;      // per thread
-    double2 v[PER][R];
+    constexpr bool SAMEJ = NTHR % (N / R) == 0;
+    constexpr int NW = SAMEJ ? 1 : PER;
     const int tid = threadIdx.x;
+    // twiddle exp(-2 pi i m k /(R Ns)) = tw[m*k*(NTW/(R*Ns))]; the first
+    // stage (Ns = 1, a constant once inlined) has k = 0, all twiddles 1,
+    // and skips the loads and products
+    double2 w[NW][R - 1];
+    if (Ns > 1)
+#pragma unroll
+        for (int q = 0; q < NW; q++)
+        {
+            const int k = ((tid + q * NTHR) % (N / R)) % Ns;
+            const int step = k * twstride * (N / (R * Ns));
+#pragma unroll
+            for (int m = 1; m < R; m++) w[q][m - 1] = tw[m * step];
+        }
+    double2 v[PER][R];
 #pragma unroll
     for (int p = 0; p < PER; p++)
     {
@@ -91,13 +129,9 @@ __device__ inline void stockham_stage(double2* buf, int Ns, const double2* __res
         {
             int l = b / (N / R), j = b % (N / R);
             int k = j % Ns;
-            // twiddle exp(-2 pi i m k /(R Ns)) = tw[m*k*(NTW/(R*Ns))]; the
-            // first stage (Ns = 1, a constant once inlined) has k = 0, all
-            // twiddles 1, and skips the loads and products
-            int step = k * twstride * (N / (R * Ns));
             if (Ns > 1)
 #pragma unroll
-                for (int m = 1; m < R; m++) v[p][m] = cmul(v[p][m], tw[m * step]);
+                for (int m = 1; m < R; m++) v[p][m] = cmul(v[p][m], w[SAMEJ ? 0 : p][m - 1]);
             double2 y[R];
             radix_butterfly<R>(v[p], y);
             double2* dst = buf + l * N;
