@@ -25,12 +25,22 @@ def make_cfg(orc, seed=3, ngroup=6, ntemp=3, nprop=6, **kw):
     return cfg
 
 
-def oracle_chains(orc, cfg, src_params):
-    """orc_chain structs for the chains' current state."""
+class _ChainView:
+    """A ChainSet of some chains, its structs indexed by chain number."""
+
+    def __init__(self, cs, sel):
+        self.cs = cs
+        self.arr = {c: cs.arr[i] for i, c in enumerate(sel)}
+
+
+def oracle_chains(orc, cfg, src_params, chains=None):
+    """orc_chain structs for the chains' current state (all chains, or only
+    the chain numbers in `chains`, then indexed by chain number)."""
     T = cfg["T"]
-    chains = []
+    out = []
     bwmax = 4096
-    for c in range(len(cfg["chain_group"])):
+    sel = range(len(cfg["chain_group"])) if chains is None else [int(c) for c in chains]
+    for c in sel:
         g = cfg["chain_group"][c]
         W = int(cfg["group_W"][g])
         qmin = int(cfg["group_qmin"][g])
@@ -43,12 +53,13 @@ def oracle_chains(orc, cfg, src_params):
             h = orc.ucb_waveform(src_params[c, s], T, b)
             tm[s, :, :2 * b] = h[:3].view(np.float64).reshape(3, -1)
             bw[s], im[s] = b, i
-        chains.append(dict(nsrc=n, W=W, qmin=qmin, bwmax=bwmax,
-                           data=np.ascontiguousarray(cfg["data"][g]).view(np.float64).reshape(-1).copy(),
-                           invC=np.ascontiguousarray(cfg["invC"][g]).reshape(-1).copy(),
-                           src_params=np.ascontiguousarray(src_params[c, :max(n, 1)]).copy(),
-                           tmpl=tm.reshape(-1).copy(), src_bw=bw, src_imin=im))
-    return orc.ChainSet(chains), chains
+        out.append(dict(nsrc=n, W=W, qmin=qmin, bwmax=bwmax,
+                        data=np.ascontiguousarray(cfg["data"][g]).view(np.float64).reshape(-1).copy(),
+                        invC=np.ascontiguousarray(cfg["invC"][g]).reshape(-1).copy(),
+                        src_params=np.ascontiguousarray(src_params[c, :max(n, 1)]).copy(),
+                        tmpl=tm.reshape(-1).copy(), src_bw=bw, src_imin=im))
+    cs = orc.ChainSet(out)
+    return (cs if chains is None else _ChainView(cs, sel)), out
 
 
 def oracle_snr(orc, cfg, c, params):
