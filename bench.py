@@ -325,6 +325,27 @@ def wdm_leg(gl, synth, n=4096, steps=5, seed=7, cpu_seconds=6.0, nthreads=0, par
     res = dict(value=n / (ms * 1e-3), ms=ms, n=n, layers=layers, transform_ms=tms, ND=ND, plain=rate,
                fisher=dict(n=nf, ms=fms, value=nf / (fms * 1e-3), plain=frate))
     B.close()
+    # the same templates with a forced layer count (templates touching 8-64
+    # frequency layers: NT (L + 1) heterodyned samples, L inverse transforms
+    # and NT L pixels each), first n_forced of them
+    n_forced = min(n, 1024)
+    forced = {}
+    for L in (7, 15, 31, 63):
+        BF = W.WdmBatch(w, orb, T, cfg["data"], cfg["invC6"], max_templates=n_forced, layers=L)
+        PF = P[:n_forced].contiguous()
+        BF.eval(PF)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(2):
+            BF.eval(PF)
+        e1.record()
+        torch.cuda.synchronize()
+        fl_ms = e0.elapsed_time(e1) / 2
+        forced[str(L)] = dict(ms_per_launch=round(fl_ms, 3), templates=n_forced,
+                              value=round(n_forced / (fl_ms * 1e-3), 1),
+                              pixels_per_template=L * cfg["NT"])
+        BF.close()
+    res["forced_layers"] = forced
     # parity and CPU baseline with the oracle
     try:
         from oracle import orc
@@ -1043,6 +1064,11 @@ def main():
                           "method": "ucb_fisher_wavelet central differences: 16 templates + 36-element "
                                     "nine-term Gram per source"},
                "parity": wl.get("parity"),
+               "forced_layers": {"unit": "wavelet templates + wavelet_nwip evals/s", "by_layers": wl.get("forced_layers"),
+                                 "method": "the first 1024 templates with every template forced to L frequency "
+                                           "layers (glass_hip_wdm_batch_create_layers; the reference's "
+                                           "ucb_wavelet_layers gives 1-3): NT (L + 1) heterodyned samples, "
+                                           "L inverse transforms, NT L pixels per template"},
                "config": "Tobs 1 yr, NT 512 x NF 4096 (15 s cadence), 4096 templates (config-0 parameter "
                          "distributions), stationary XYZ pixel noise; plain = per-call ucb_waveform_wavelet "
                          "drop-in on host arrays"}
@@ -1052,7 +1078,8 @@ def main():
                                    "sample": f"{wl['cpu']['n']} templates x {wl['cpu']['reps']} by the oracle "
                                              f"restatement, OpenMP over templates, {wl['cpu']['secs']:.1f} s"}
         log(f"wdm: {wl['value']:.4e} templates/s ({wl['ms']:.3f} ms / {wl['n']}), transform "
-            f"{wl['transform_ms']:.3f} ms, plain {wl['plain']}/s, parity {wl.get('parity')}")
+            f"{wl['transform_ms']:.3f} ms, plain {wl['plain']}/s, parity {wl.get('parity')}; forced layers "
+            + ", ".join(f"L={k}: {v['value']:.3e}/s" for k, v in (wl.get('forced_layers') or {}).items()))
 
     inner = None
     if not args.no_inner:

@@ -67,6 +67,9 @@ int twiddles3(const double2** out);
 // And for the long WDM template series, N = 2^a up to 16384.
 constexpr int kTw16N = 16384;
 int twiddles16(const double2** out);
+// And up to 32768 (wavelet templates with a forced layer count).
+constexpr int kTw32N = 32768;
+int twiddles32(const double2** out);
 
 // Spacecraft positions of the analytic orbit at the npos sample times
 // t0 + T*n/npos (npos a power of two), 9 doubles per sample

@@ -92,7 +92,7 @@ int ctx_get(ThreadCtx** out)
 }
 
 static std::mutex g_tab_mu;
-static std::map<int, double2*> g_tw, g_tw3, g_tw16;
+static std::map<int, double2*> g_tw, g_tw3, g_tw16, g_tw32;
 typedef std::tuple<int, double, double, double, double, double, int> PosKey;
 static std::map<PosKey, double*> g_pos;
 
@@ -126,6 +126,7 @@ static int twiddle_table(std::map<int, double2*>& cache, int ntw, const double2*
 int twiddles(const double2** out) { return twiddle_table(g_tw, kTwN, out); }
 int twiddles3(const double2** out) { return twiddle_table(g_tw3, kTw3N, out); }
 int twiddles16(const double2** out) { return twiddle_table(g_tw16, kTw16N, out); }
+int twiddles32(const double2** out) { return twiddle_table(g_tw32, kTw32N, out); }
 
 void analytic_positions(const glass_hip_orbit& o, double t, double* p)
 {

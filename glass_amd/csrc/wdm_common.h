@@ -18,6 +18,18 @@ constexpr double PI2 = 6.283185307179586;
 constexpr double CLIGHT = 299792458.;
 constexpr double VEARTH = 0.00010103671;
 constexpr int kMaxLayers = 3;   // ucb_wavelet_layers spans at most j-1..j+1
+// A layer count forced on every template of a batched plan (a workload
+// knob with no reference counterpart: the reference always takes
+// ucb_wavelet_layers' 1-3): L + 1 a power of two from 8 to 64 with
+// NT (L + 1) <= 32768, at NT = 256 or 512.  Its tables take the slot after
+// the reference counts'.
+constexpr int kForcedSlot = kMaxLayers + 1;
+constexpr int kMaxForcedSamples = 32768;
+GH_HD inline bool forced_layers_ok(int NT, int L)
+{
+    const int n = L + 1;
+    return (NT == 256 || NT == 512) && n >= 8 && n <= 64 && (n & (n - 1)) == 0 && NT * n <= kMaxForcedSamples;
+}
 constexpr int kMaxKnots = 512;  // orbit knots per template (200 per year: 2.5 yr)
 
 // Index lookups of spline_interpolation (utils/src/glass_math.c:393-467).

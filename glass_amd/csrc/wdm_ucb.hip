@@ -163,12 +163,15 @@ struct Geo
     const double* tssb;    // Norb response times t0 + i Tobs/(Norb-1)
     const double* prep_s;  // 3 x Norb  on the response times
     const double* resp;    // Norb x 24: x[3][3]/c, n[3][3], L[3] at tssb
-    const double* ds[kMaxLayers + 1];   // per L: N_ds x 5: t, r0x, r0y, r0z, dx
-    const int* dsi[kMaxLayers + 1];     // per L: response-time interval
-    const double* win[kMaxLayers + 1];  // per L: window of L+1 layers
+    // per layer count L (slot L; a forced count Lf in slot kForcedSlot):
+    const double* ds[kForcedSlot + 1];   // N_ds x 5: t, r0x, r0y, r0z, dx
+    const int* dsi[kForcedSlot + 1];     // response-time interval
+    const double* win[kForcedSlot + 1];  // window of L+1 layers
+    int Lf;                // forced layer count of every template (0: ucb_wavelet_layers' own)
     const double2* tw;     // 4096-entry twiddles
     const double2* tw3;    // 12288-entry twiddles
     const double2* tw16;   // 16384-entry twiddles (template series of NT >= 1024)
+    const double2* tw32;   // 32768-entry twiddles (forced layer counts)
 };
 
 struct GeoHost
@@ -200,9 +203,10 @@ static double spline_host_at(const std::vector<double>& x, const double* c, int 
     return c[n] + c[N + n] * dx + c[2 * N + n] * dx * dx + c[3 * N + n] * dx * dx * dx;
 }
 
-// Fills the tables.  pos: 9 x Norb positions (row 3*sc + coord).
+// Fills the tables.  pos: 9 x Norb positions (row 3*sc + coord); Lf > 0
+// adds the tables of that forced layer count.
 static int geo_build(GeoHost& G, const glass_hip_wdm& w, int Norb, const double* ot_in,
-                     const double* pos, double Tobs, double t0)
+                     const double* pos, double Tobs, double t0, int Lf)
 {
 #pragma clang fp contract(off)
     Geo& g = G.g;
@@ -272,8 +276,10 @@ static int geo_build(GeoHost& G, const glass_hip_wdm& w, int Norb, const double*
         return rc;
     // per layer count L: the downsampled times, spacecraft 1 there, the
     // response-time interval and offset of each sample, and the window
-    for (int L = 1; L <= kMaxLayers; L++)
+    g.Lf = Lf;
+    for (int slot = 1; slot <= (Lf > 0 ? kForcedSlot : kMaxLayers); slot++)
     {
+        const int L = slot == kForcedSlot ? Lf : slot;
         const int Nds = w.NT * (L + 1);
         const double dt_ds = w.dt / (double)(L + 1);
         std::vector<double> ds(5 * (size_t)Nds);
@@ -297,14 +303,23 @@ static int geo_build(GeoHost& G, const glass_hip_wdm& w, int Norb, const double*
         }
         std::vector<double> win(w.NT / 2 + 1);
         wdm_window(L + 1, w.NT, w.A, win.data());
-        if ((rc = upload(G, ds, &g.ds[L])) || (rc = upload(G, dsi, &g.dsi[L])) ||
-            (rc = upload(G, win, &g.win[L])))
+        if ((rc = upload(G, ds, &g.ds[slot])) || (rc = upload(G, dsi, &g.dsi[slot])) ||
+            (rc = upload(G, win, &g.win[slot])))
             return rc;
     }
     g.ds[0] = nullptr;
     g.dsi[0] = nullptr;
     g.win[0] = nullptr;
-    if ((rc = twiddles(&g.tw)) || (rc = twiddles3(&g.tw3)) || (rc = twiddles16(&g.tw16))) return rc;
+    if (Lf <= 0)
+    {
+        g.ds[kForcedSlot] = nullptr;
+        g.dsi[kForcedSlot] = nullptr;
+        g.win[kForcedSlot] = nullptr;
+    }
+    g.tw32 = nullptr;
+    if ((rc = twiddles(&g.tw)) || (rc = twiddles3(&g.tw3)) || (rc = twiddles16(&g.tw16)) ||
+        (Lf > 0 && (rc = twiddles32(&g.tw32))))
+        return rc;
     return GLASS_HIP_OK;
 }
 
@@ -350,6 +365,7 @@ __global__ void __launch_bounds__(256) hdr_kernel(Geo g, int n, const double* __
     if (e >= n) return;
     Hdr h;
     make_hdr(params + 8 * (size_t)e, g.Tobs, g.df, g.A, &h);
+    if (g.Lf > 0) h.L = g.Lf;  // forced layer count: same first layer and heterodyne
     *(Hdr*)(scratch + scratch_doubles(g.Norb) * (size_t)e) = h;
 }
 
@@ -623,12 +639,14 @@ struct LayerArgs
     const double* invC;   // 6 planes XX,YY,ZZ,XY,XZ,YZ of npix
     int npix;
     int* nwave;           // [n]
-    int* list;            // [n x 3 NT] (compact pixel list)
-    double* values;       // [n x 3 x 3 NT]
+    int* list;            // [n x vcap] (compact pixel list)
+    double* values;       // [n x 3 x vcap]
     double* dlogL;        // [n] (with data)
     double* snr;          // [n] (with data)
-    double2* big;         // NT >= 1024: nbig global scratch slices of big_layer_slice(NT)
+    double2* big;         // long-series kernel: nbig global scratch slices of bigslice double2
     int nbig;
+    int vcap;             // pixels per template: 3 NT (NT max(3, Lf) with a forced layer count)
+    size_t bigslice;
 };
 
 // X, Y (packed in U) or Z (half-length V) spectrum at bin k, 0 < k < Nds/2
@@ -705,7 +723,7 @@ __device__ void layer_body(const Geo& g, const Hdr& h, int e, const LayerArgs& a
     lds_fft<M, 1, kLayerThreads>(V, twU, tsU * 2);
     // pixel list (ucb/src/glass_ucb_waveform.c:1189-1212): candidates in
     // (time i, layer j) order, kept when kmin <= k < kmax
-    int* lst = a.list + (size_t)e * 3 * NT;
+    int* lst = a.list + (size_t)e * a.vcap;
     {
         constexpr int NC = NT * L;
         constexpr int PER = (NC + kLayerThreads - 1) / kLayerThreads;
@@ -742,7 +760,7 @@ __device__ void layer_body(const Geo& g, const Hdr& h, int e, const LayerArgs& a
     const int Nw = cnt[kLayerThreads];
     const double* win = g.win[L];
     const double norm = sqrt((double)g.NF) / (L + 1.0);
-    double* vals = a.values + (size_t)e * 9 * NT;
+    double* vals = a.values + (size_t)e * 3 * a.vcap;
     for (int ch = 0; ch < 3; ch++)
     {
         // windowed bands of the L layers, conjugated so the forward FFT
@@ -780,7 +798,7 @@ __device__ void layer_body(const Geo& g, const Hdr& h, int e, const LayerArgs& a
                 val = ((n + m) % 2 == 0) ? re * norm : sgn * im * norm;
             }
             const int pos = n * L + off;
-            if (pos < Nw) vals[ch * 3 * NT + pos] = val;
+            if (pos < Nw) vals[ch * a.vcap + pos] = val;
         }
         __syncthreads();
     }
@@ -795,7 +813,7 @@ __device__ void layer_body(const Geo& g, const Hdr& h, int e, const LayerArgs& a
     for (int t = tid; t < Nw; t += kLayerThreads)
     {
         const int k = lst[t];
-        const double hv[3] = {vals[t], vals[3 * NT + t], vals[6 * NT + t]};
+        const double hv[3] = {vals[t], vals[a.vcap + t], vals[2 * a.vcap + t]};
         double C[3][3];
         C[0][0] = a.invC[k];
         C[1][1] = a.invC[(size_t)a.npix + k];
@@ -867,8 +885,11 @@ constexpr size_t layer_lds_bytes()
 constexpr int kBigLayerThreads = 512;
 
 // double2 per scratch slice: U and partner (NDS each) and V and partner
-// (NDS / 2 each) at the largest NDS = 4 NT
-__host__ __device__ inline size_t big_layer_slice(int NT) { return 12 * (size_t)NT; }
+// (NDS / 2 each), NDS = NT (L + 1) at the largest L (3, or a forced count)
+__host__ __device__ inline size_t big_layer_slice(int NT, int Lmax = kMaxLayers)
+{
+    return 3 * (size_t)NT * (size_t)((Lmax > kMaxLayers ? Lmax : kMaxLayers) + 1);
+}
 
 template <int NT, int L>
 __device__ void layer_body_big(const Geo& g, const Hdr& h, int e, const LayerArgs& a, double2* G0, double2* B,
@@ -877,12 +898,13 @@ __device__ void layer_body_big(const Geo& g, const Hdr& h, int e, const LayerArg
     constexpr int NDS = NT * (L + 1);
     constexpr int M = NDS / 2;
     constexpr int NTHR = kBigLayerThreads;
+    constexpr int SLOT = L > kMaxLayers ? kForcedSlot : L;
     const int tid = threadIdx.x;
     const int N = g.Norb;
     const double* cph = scr + 64;
     const double* ctdi = scr + 64 + 4 * N;
-    const double* ds = g.ds[L];
-    const int* dsi = g.dsi[L];
+    const double* ds = g.ds[SLOT];
+    const int* dsi = g.dsi[SLOT];
     double2* Ua = G0;
     double2* Ub = G0 + NDS;
     double2* Va = G0 + 2 * NDS;
@@ -923,11 +945,11 @@ __device__ void layer_body_big(const Geo& g, const Hdr& h, int e, const LayerArg
     __threadfence_block();
     __syncthreads();
     const bool three = (NDS % 3) == 0;
-    const double2* twU = three ? g.tw3 : g.tw16;
-    const int tsU = (three ? kTw3N : kTw16N) / NDS;
+    const double2* twU = three ? g.tw3 : (NDS > kTw16N ? g.tw32 : g.tw16);
+    const int tsU = (three ? kTw3N : (NDS > kTw16N ? kTw32N : kTw16N)) / NDS;
     const double2* U = pingpong_fft<NDS, NTHR>(Ua, Ub, twU, tsU);
     const double2* V = pingpong_fft<M, NTHR>(Va, Vb, twU, tsU * 2);
-    int* lst = a.list + (size_t)e * 3 * NT;
+    int* lst = a.list + (size_t)e * a.vcap;
     {
         constexpr int NC = NT * L;
         constexpr int PER = (NC + NTHR - 1) / NTHR;
@@ -962,9 +984,9 @@ __device__ void layer_body_big(const Geo& g, const Hdr& h, int e, const LayerArg
     }
     __syncthreads();
     const int Nw = cnt[NTHR];
-    const double* win = g.win[L];
+    const double* win = g.win[SLOT];
     const double norm = sqrt((double)g.NF) / (L + 1.0);
-    double* vals = a.values + (size_t)e * 9 * NT;
+    double* vals = a.values + (size_t)e * 3 * a.vcap;
     for (int ch = 0; ch < 3; ch++)
         for (int off = 0; off < L; off++)
         {
@@ -998,7 +1020,7 @@ __device__ void layer_body_big(const Geo& g, const Hdr& h, int e, const LayerArg
                     val = ((nn + m) % 2 == 0) ? re * norm : sgn * im * norm;
                 }
                 const int pos = nn * L + off;
-                if (pos < Nw) vals[ch * 3 * NT + pos] = val;
+                if (pos < Nw) vals[ch * a.vcap + pos] = val;
             }
             __syncthreads();
         }
@@ -1010,7 +1032,7 @@ __device__ void layer_body_big(const Geo& g, const Hdr& h, int e, const LayerArg
         for (int t = tid; t < Nw; t += NTHR)
         {
             const int k = lst[t];
-            const double hv[3] = {vals[t], vals[3 * NT + t], vals[6 * NT + t]};
+            const double hv[3] = {vals[t], vals[a.vcap + t], vals[2 * a.vcap + t]};
             double C[3][3];
             C[0][0] = a.invC[k];
             C[1][1] = a.invC[(size_t)a.npix + k];
@@ -1048,7 +1070,7 @@ __global__ void __launch_bounds__(kBigLayerThreads) layer_kernel_big(Geo g, int 
     extern __shared__ double2 ldsb[];  // B: one layer's band, NT complex
     __shared__ Hdr H;
     __shared__ int cnt[kBigLayerThreads + 1];
-    double2* G0 = a.big + big_layer_slice(NT) * (size_t)blockIdx.x;
+    double2* G0 = a.big + a.bigslice * (size_t)blockIdx.x;
     for (int e = blockIdx.x; e < n; e += gridDim.x)
     {
         const double* scr = scratch + scratch_doubles(g.Norb) * (size_t)e;
@@ -1058,7 +1080,15 @@ __global__ void __launch_bounds__(kBigLayerThreads) layer_kernel_big(Geo g, int 
         const Hdr h = H;
         if (h.L == 1) layer_body_big<NT, 1>(g, h, e, a, G0, ldsb, cnt, scr);
         else if (h.L == 2) layer_body_big<NT, 2>(g, h, e, a, G0, ldsb, cnt, scr);
-        else layer_body_big<NT, 3>(g, h, e, a, G0, ldsb, cnt, scr);
+        else if (h.L == 3) layer_body_big<NT, 3>(g, h, e, a, G0, ldsb, cnt, scr);
+        else if constexpr (NT <= 512)
+        {
+            // forced layer counts (forced_layers_ok)
+            if (h.L == 7) layer_body_big<NT, 7>(g, h, e, a, G0, ldsb, cnt, scr);
+            else if (h.L == 15) layer_body_big<NT, 15>(g, h, e, a, G0, ldsb, cnt, scr);
+            else if (h.L == 31) layer_body_big<NT, 31>(g, h, e, a, G0, ldsb, cnt, scr);
+            else if (h.L == 63) layer_body_big<NT, 63>(g, h, e, a, G0, ldsb, cnt, scr);
+        }
     }
 }
 
@@ -1073,16 +1103,22 @@ static int launch_templates(const GeoHost& G, int n, const double* d_params, dou
     hipLaunchKernelGGL(resp_kernel, dim3(n), dim3(kRespThreads), resp_lds_bytes(g.Norb), s, g, n, d_params,
                        d_scratch);
     GH_CHECK(hipGetLastError());
-    if (g.NT >= 1024)
+    if (g.NT >= 1024 || g.Lf > 0)
     {
-        if (!a.big || a.nbig <= 0)
+        if (!a.big || a.nbig <= 0 || a.bigslice < big_layer_slice(g.NT, g.Lf) || a.vcap < g.NT * (g.Lf > 3 ? g.Lf : 3))
         {
-            set_error("wavelet templates: NT = %d needs the long-series scratch", g.NT);
+            set_error("wavelet templates: NT = %d (forced layers %d) needs the long-series scratch", g.NT, g.Lf);
             return GLASS_HIP_EINVAL;
         }
         const int grid = n < a.nbig ? n : a.nbig;
         const size_t lds = sizeof(double2) * (size_t)g.NT;
-        if (g.NT == 1024)
+        if (g.NT == 256)
+            hipLaunchKernelGGL(layer_kernel_big<256>, dim3(grid), dim3(kBigLayerThreads), lds, s, g, n, a,
+                               d_scratch);
+        else if (g.NT == 512)
+            hipLaunchKernelGGL(layer_kernel_big<512>, dim3(grid), dim3(kBigLayerThreads), lds, s, g, n, a,
+                               d_scratch);
+        else if (g.NT == 1024)
             hipLaunchKernelGGL(layer_kernel_big<1024>, dim3(grid), dim3(kBigLayerThreads), lds, s, g, n, a,
                                d_scratch);
         else if (g.NT == 2048)
@@ -1103,11 +1139,13 @@ static int launch_templates(const GeoHost& G, int n, const double* d_params, dou
 }
 
 // The calling thread's long-series scratch for n templates of grid NT
-// (nothing below NT = 1024).
+// (nothing below NT = 1024).  Sets the default pixel capacity 3 NT.
 static int big_scratch(ThreadCtx* c, int NT, int n, LayerArgs& a)
 {
+    a.vcap = 3 * NT;
     if (NT < 1024) return GLASS_HIP_OK;
-    if (int rc = c->dscr.ensure(sizeof(double2) * big_layer_slice(NT) * (size_t)n)) return rc;
+    a.bigslice = big_layer_slice(NT);
+    if (int rc = c->dscr.ensure(sizeof(double2) * a.bigslice * (size_t)n)) return rc;
     a.big = (double2*)c->dscr.p;
     a.nbig = n;
     return GLASS_HIP_OK;
@@ -1352,15 +1390,20 @@ static std::mutex g_geo_mu;
 static std::map<std::pair<int, GeoKey>, GeoHost*> g_geo;
 
 static int geo_get(const glass_hip_wdm* w, int Norb, const double* ot, const double* ox, const double* oy,
-                   const double* oz, double Tobs, double t0, const GeoHost** out)
+                   const double* oz, double Tobs, double t0, const GeoHost** out, int Lf = 0)
 {
     int rc = check_grid(w, Norb);
     if (rc) return rc;
+    if (Lf != 0 && !forced_layers_ok(w->NT, Lf))
+        return set_error("wavelet templates: forced layer count %d unsupported at NT = %d (L + 1 a power of two "
+                         "from 8 to 64 with NT (L + 1) <= %d, NT = 256 or 512)",
+                         Lf, w->NT, kMaxForcedSamples),
+               GLASS_HIP_EINVAL;
     int dev = 0;
     GH_CHECK(hipGetDevice(&dev));
     GeoKey key;
     key.blob = {(double)w->NF, (double)w->NT, w->dt, w->df, w->A, (double)w->kmin, (double)w->kmax,
-                (double)w->imin, (double)w->imax, Tobs, t0, (double)Norb};
+                (double)w->imin, (double)w->imax, Tobs, t0, (double)Norb, (double)Lf};
     key.blob.insert(key.blob.end(), ot, ot + Norb);
     // pos rows: spacecraft i coordinate d at 3i+d
     std::vector<double> pos(9 * (size_t)Norb);
@@ -1380,7 +1423,7 @@ static int geo_get(const glass_hip_wdm* w, int Norb, const double* ot, const dou
     }
     if ((rc = set_lds_limits())) return rc;
     GeoHost* G = new GeoHost();
-    if ((rc = geo_build(*G, *w, Norb, ot, pos.data(), Tobs, t0)))
+    if ((rc = geo_build(*G, *w, Norb, ot, pos.data(), Tobs, t0, Lf)))
     {
         delete G;
         return rc;
@@ -1498,15 +1541,19 @@ struct glass_hip_wdm_batch
     double* fparams = nullptr;
     double* fD = nullptr;
     double* fM = nullptr;
-    // long-series scratch of the NT >= 1024 layer kernel: nbig slices
+    // long-series scratch of the NT >= 1024 (or forced layer count) layer
+    // kernel: nbig slices of bigslice double2
     double2* big = nullptr;
     int nbig = 0;
+    size_t bigslice = 0;
+    int vcap = 0;  // pixels per template and channel
+    int layers = 0;
 };
 
-extern "C" int glass_hip_wdm_batch_create(glass_hip_wdm_batch** plan, const glass_hip_wdm* w, int Norb,
-                                          const double* orbit_t, const double* orbit_x, const double* orbit_y,
-                                          const double* orbit_z, double Tobs, double t0, const double* data,
-                                          const double* invC, int npix, int max_templates)
+static int wdm_batch_create(glass_hip_wdm_batch** plan, const glass_hip_wdm* w, int Norb, const double* orbit_t,
+                            const double* orbit_x, const double* orbit_y, const double* orbit_z, double Tobs,
+                            double t0, const double* data, const double* invC, int npix, int max_templates,
+                            int layers)
 {
     if (!plan || !orbit_t || !orbit_x || !orbit_y || !orbit_z || !data || !invC)
         return set_error("wdm_batch_create: NULL argument"), GLASS_HIP_EINVAL;
@@ -1521,7 +1568,7 @@ extern "C" int glass_hip_wdm_batch_create(glass_hip_wdm_batch** plan, const glas
         glass_hip_wdm_batch_destroy(P);
         return r;
     };
-    if ((rc = geo_get(w, Norb, orbit_t, orbit_x, orbit_y, orbit_z, Tobs, t0, &P->G))) return fail(rc);
+    if ((rc = geo_get(w, Norb, orbit_t, orbit_x, orbit_y, orbit_z, Tobs, t0, &P->G, layers))) return fail(rc);
     if (w->kmax - w->kmin != npix)
     {
         set_error("wdm_batch_create: npix = %d but kmax - kmin = %d", npix, w->kmax - w->kmin);
@@ -1529,25 +1576,28 @@ extern "C" int glass_hip_wdm_batch_create(glass_hip_wdm_batch** plan, const glas
     }
     P->npix = npix;
     P->max_templates = max_templates;
+    P->layers = layers;
     const int NT = w->NT;
+    P->vcap = NT * (layers > kMaxLayers ? layers : kMaxLayers);
     if (hipMalloc(&P->data, sizeof(double) * 3 * (size_t)npix) != hipSuccess ||
         hipMalloc(&P->invC, sizeof(double) * 6 * (size_t)npix) != hipSuccess ||
         hipMalloc(&P->scratch, sizeof(double) * scratch_doubles(Norb) * (size_t)max_templates) != hipSuccess ||
-        hipMalloc(&P->list, sizeof(int) * 3 * NT * (size_t)max_templates) != hipSuccess ||
-        hipMalloc(&P->values, sizeof(double) * 9 * NT * (size_t)max_templates) != hipSuccess ||
+        hipMalloc(&P->list, sizeof(int) * (size_t)P->vcap * max_templates) != hipSuccess ||
+        hipMalloc(&P->values, sizeof(double) * 3 * (size_t)P->vcap * max_templates) != hipSuccess ||
         hipMalloc(&P->nwave, sizeof(int) * (size_t)max_templates) != hipSuccess)
     {
         set_error("wdm_batch_create: device allocation failed");
         return fail(GLASS_HIP_ERUNTIME);
     }
-    if (NT >= 1024)
+    if (NT >= 1024 || layers > 0)
     {
         // the layer kernel strides over the templates with two workgroups
         // per CU, each with its own long-series slice
         int ncu = 0;
         GH_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, P->device));
         P->nbig = std::min(max_templates, 2 * std::max(ncu, 1));
-        if (hipMalloc(&P->big, sizeof(double2) * big_layer_slice(NT) * (size_t)P->nbig) != hipSuccess)
+        P->bigslice = big_layer_slice(NT, layers);
+        if (hipMalloc(&P->big, sizeof(double2) * P->bigslice * (size_t)P->nbig) != hipSuccess)
         {
             set_error("wdm_batch_create: device allocation failed");
             return fail(GLASS_HIP_ERUNTIME);
@@ -1560,6 +1610,37 @@ extern "C" int glass_hip_wdm_batch_create(glass_hip_wdm_batch** plan, const glas
         return fail(GLASS_HIP_ERUNTIME);
     }
     *plan = P;
+    return GLASS_HIP_OK;
+}
+
+extern "C" int glass_hip_wdm_batch_create(glass_hip_wdm_batch** plan, const glass_hip_wdm* w, int Norb,
+                                          const double* orbit_t, const double* orbit_x, const double* orbit_y,
+                                          const double* orbit_z, double Tobs, double t0, const double* data,
+                                          const double* invC, int npix, int max_templates)
+{
+    return wdm_batch_create(plan, w, Norb, orbit_t, orbit_x, orbit_y, orbit_z, Tobs, t0, data, invC, npix,
+                            max_templates, 0);
+}
+
+extern "C" int glass_hip_wdm_batch_create_layers(glass_hip_wdm_batch** plan, const glass_hip_wdm* w, int Norb,
+                                                 const double* orbit_t, const double* orbit_x,
+                                                 const double* orbit_y, const double* orbit_z, double Tobs,
+                                                 double t0, const double* data, const double* invC, int npix,
+                                                 int max_templates, int layers)
+{
+    if (layers <= 0)
+        return set_error("wdm_batch_create_layers: layer count %d (use glass_hip_wdm_batch_create for the "
+                         "reference's own)",
+                         layers),
+               GLASS_HIP_EINVAL;
+    return wdm_batch_create(plan, w, Norb, orbit_t, orbit_x, orbit_y, orbit_z, Tobs, t0, data, invC, npix,
+                            max_templates, layers);
+}
+
+extern "C" int glass_hip_wdm_batch_capacity(const glass_hip_wdm_batch* P, int* vcap)
+{
+    if (!P || !vcap) return set_error("wdm_batch_capacity: NULL argument"), GLASS_HIP_EINVAL;
+    *vcap = P->vcap;
     return GLASS_HIP_OK;
 }
 
@@ -1602,6 +1683,8 @@ extern "C" int glass_hip_wdm_batch_eval(glass_hip_wdm_batch* P, int n, const dou
     a.snr = d_snr;
     a.big = P->big;
     a.nbig = P->nbig;
+    a.bigslice = P->bigslice;
+    a.vcap = P->vcap;
     return launch_templates(*P->G, n, d_params, P->scratch, a, (hipStream_t)stream);
 }
 
@@ -1612,10 +1695,10 @@ extern "C" int glass_hip_wdm_batch_templates(glass_hip_wdm_batch* P, int n, int*
         return set_error("wdm_batch_templates: n = %d outside 0..%d", n, P->max_templates), GLASS_HIP_EINVAL;
     GH_CHECK(hipSetDevice(P->device));
     GH_CHECK(hipDeviceSynchronize());
-    const size_t NT = P->G->g.NT;
+    const size_t vcap = P->vcap;
     GH_CHECK(hipMemcpy(nwave, P->nwave, sizeof(int) * n, hipMemcpyDeviceToHost));
-    GH_CHECK(hipMemcpy(list, P->list, sizeof(int) * 3 * NT * n, hipMemcpyDeviceToHost));
-    GH_CHECK(hipMemcpy(values, P->values, sizeof(double) * 9 * NT * n, hipMemcpyDeviceToHost));
+    GH_CHECK(hipMemcpy(list, P->list, sizeof(int) * vcap * n, hipMemcpyDeviceToHost));
+    GH_CHECK(hipMemcpy(values, P->values, sizeof(double) * 3 * vcap * n, hipMemcpyDeviceToHost));
     return GLASS_HIP_OK;
 }
 
@@ -1742,6 +1825,11 @@ extern "C" int glass_hip_wdm_batch_fisher(glass_hip_wdm_batch* P, int n, const d
         return set_error("wdm_batch_fisher: n = %d sources need %lld template slots of %d", n, 16LL * n,
                          P->max_templates),
                GLASS_HIP_EINVAL;
+    if (P->layers > 0)
+        return set_error("wdm_batch_fisher: the plan forces %d layers per template; ucb_fisher_wavelet takes the "
+                         "reference's own layer count",
+                         P->layers),
+               GLASS_HIP_EINVAL;
     if (n == 0) return GLASS_HIP_OK;
     GH_CHECK(hipSetDevice(P->device));
     const int NT = P->G->g.NT;
@@ -1761,6 +1849,8 @@ extern "C" int glass_hip_wdm_batch_fisher(glass_hip_wdm_batch* P, int n, const d
     a.values = P->values;
     a.big = P->big;
     a.nbig = P->nbig;
+    a.bigslice = P->bigslice;
+    a.vcap = P->vcap;
     int rc;
     if ((rc = launch_templates(*P->G, 16 * n, P->fparams, P->scratch, a, s))) return rc;
     FishArgs fa{};

@@ -54,6 +54,8 @@ _SIGS = [
     ("glass_hip_gaussian_log_likelihood_wavelet", [_W, _dp, _dp, _dp, _dp, _dp, _dp, _ip, _i, _dp, _i], _d),
     ("glass_hip_wdm_transform", [_W, _vp, _vp, _vp], _i),
     ("glass_hip_wdm_batch_create", [_vp, _W, _i, _dp, _dp, _dp, _dp, _d, _d, _dp, _dp, _i, _i], _i),
+    ("glass_hip_wdm_batch_create_layers", [_vp, _W, _i, _dp, _dp, _dp, _dp, _d, _d, _dp, _dp, _i, _i, _i], _i),
+    ("glass_hip_wdm_batch_capacity", [_vp, _vp], _i),
     ("glass_hip_wdm_batch_destroy", [_vp], _i),
     ("glass_hip_wdm_batch_buffers", [_vp, _vp, _vp, _vp], _i),
     ("glass_hip_wdm_batch_eval", [_vp, _i, _vp, _vp, _vp, _vp], _i),
@@ -249,19 +251,29 @@ class WdmBatch:
     where dlogL is the change of the reference's wavelet logL when the
     template is added to an empty model."""
 
-    def __init__(self, w, orb, Tobs, data, invC6, max_templates, t0=0.0):
+    def __init__(self, w, orb, Tobs, data, invC6, max_templates, t0=0.0, layers=0):
+        """layers > 0: every template takes that many frequency layers
+        (glass_hip_wdm_batch_create_layers; a timing knob, no reference
+        counterpart) instead of ucb_wavelet_layers' 1-3."""
         import torch
         self.w = w
         self.max = int(max_templates)
+        self.layers = int(layers)
         t, x, y, z = (_f64(a) for a in orb)
         d, ic = _f64(data), _f64(invC6)
         if d.shape != (3, w.npix) or ic.shape != (6, w.npix):
             raise ValueError("data must be [3, npix] and invC6 [6, npix]")
         self._p = _vp()
-        check(_L().glass_hip_wdm_batch_create(ctypes.byref(self._p), ctypes.byref(w), len(t), t.ctypes.data_as(_dp),
-                                              x.ctypes.data_as(_dp), y.ctypes.data_as(_dp), z.ctypes.data_as(_dp),
-                                              float(Tobs), float(t0), d.ctypes.data_as(_dp),
-                                              ic.ctypes.data_as(_dp), w.npix, self.max), "wdm_batch_create")
+        args = (ctypes.byref(self._p), ctypes.byref(w), len(t), t.ctypes.data_as(_dp), x.ctypes.data_as(_dp),
+                y.ctypes.data_as(_dp), z.ctypes.data_as(_dp), float(Tobs), float(t0), d.ctypes.data_as(_dp),
+                ic.ctypes.data_as(_dp), w.npix, self.max)
+        if self.layers:
+            check(_L().glass_hip_wdm_batch_create_layers(*args, self.layers), "wdm_batch_create_layers")
+        else:
+            check(_L().glass_hip_wdm_batch_create(*args), "wdm_batch_create")
+        vc = ctypes.c_int()
+        check(_L().glass_hip_wdm_batch_capacity(self._p, ctypes.byref(vc)), "wdm_batch_capacity")
+        self.vcap = vc.value
         dev = torch.device("cuda")
         self.dlogL = torch.empty(self.max, dtype=torch.float64, device=dev)
         self.snr = torch.empty(self.max, dtype=torch.float64, device=dev)
@@ -288,12 +300,11 @@ class WdmBatch:
         return out
 
     def templates(self, n):
-        """(nwave [n], list [n, 3NT], values [n, 3, 3NT]) of the last eval,
-        copied to the host (synchronous)."""
-        NT = self.w.NT
+        """(nwave [n], list [n, vcap], values [n, 3, vcap]) of the last eval,
+        copied to the host (synchronous); vcap = 3 NT, or NT x layers."""
         h_nw = np.zeros(n, dtype=np.int32)
-        h_l = np.zeros((n, 3 * NT), dtype=np.int32)
-        h_v = np.zeros((n, 3, 3 * NT))
+        h_l = np.zeros((n, self.vcap), dtype=np.int32)
+        h_v = np.zeros((n, 3, self.vcap))
         check(_L().glass_hip_wdm_batch_templates(self._p, n, h_nw.ctypes.data_as(_ip), h_l.ctypes.data_as(_ip),
                                                  h_v.ctypes.data_as(_dp)), "wdm_batch_templates")
         return h_nw, h_l, h_v

@@ -161,22 +161,38 @@ int glass_hip_wdm_transform(const glass_hip_wdm *wdm, const double *d_time, doub
  *   d_snr[e]   = snr_wavelet of template e over its pixel list;
  *   d_dlogL[e] = gaussian_log_likelhood_wavelet(data, model = template e)
  *                - gaussian_log_likelhood_wavelet(data, empty model),
- * and keeps every template's Nwavelet, pixel list (3*NT ints per template)
- * and values (3 channels x 3*NT doubles per template, in list order) in
- * plan buffers (glass_hip_wdm_batch_buffers). */
+ * and keeps every template's Nwavelet, pixel list (vcap ints per template)
+ * and values (3 channels x vcap doubles per template, in list order) in
+ * plan buffers (glass_hip_wdm_batch_buffers); vcap = 3*NT
+ * (glass_hip_wdm_batch_capacity). */
 typedef struct glass_hip_wdm_batch glass_hip_wdm_batch;
 
 int glass_hip_wdm_batch_create(glass_hip_wdm_batch **plan, const glass_hip_wdm *wdm, int Norb,
                                const double *orbit_t, const double *orbit_x, const double *orbit_y,
                                const double *orbit_z, double Tobs, double t0, const double *data,
                                const double *invC, int npix, int max_templates);
+/* A plan whose templates all take `layers` frequency layers instead of the
+ * reference's ucb_wavelet_layers count (1-3, ucb/src/glass_ucb_waveform.c:
+ * 1003-1042): the same first layer and heterodyne frequency (jmin - 1) df,
+ * then NT (layers + 1) heterodyned samples, the window of layers + 1 layers
+ * and layers inverse transforms (glass_ucb_waveform.c:1110-1180 with
+ * Nlayers = layers).  No reference counterpart: a workload knob for timing
+ * templates that touch 8-64 layers.  layers + 1 a power of two from 8 to
+ * 64, NT = 256 or 512, NT (layers + 1) <= 32768; vcap = NT * layers.
+ * glass_hip_wdm_batch_fisher refuses such a plan. */
+int glass_hip_wdm_batch_create_layers(glass_hip_wdm_batch **plan, const glass_hip_wdm *wdm, int Norb,
+                                      const double *orbit_t, const double *orbit_x, const double *orbit_y,
+                                      const double *orbit_z, double Tobs, double t0, const double *data,
+                                      const double *invC, int npix, int max_templates, int layers);
+/* Pixels per template and channel in the plan's list and value buffers. */
+int glass_hip_wdm_batch_capacity(const glass_hip_wdm_batch *plan, int *vcap);
 int glass_hip_wdm_batch_destroy(glass_hip_wdm_batch *plan);
 int glass_hip_wdm_batch_buffers(glass_hip_wdm_batch *plan, int **d_nwave, int **d_list,
                                 double **d_values);
 int glass_hip_wdm_batch_eval(glass_hip_wdm_batch *plan, int n, const double *d_params,
                              double *d_dlogL, double *d_snr, void *stream);
 /* Host copies of the last eval's first n templates (synchronous): nwave[n],
- * list[n*3*NT], values[n*3*3*NT] (channel-major per template). */
+ * list[n*vcap], values[n*3*vcap] (channel-major per template). */
 int glass_hip_wdm_batch_templates(glass_hip_wdm_batch *plan, int n, int *nwave, int *list,
                                   double *values);
 /* ucb_fisher_wavelet (as glass_hip_ucb_fisher_wavelet) for n sources in

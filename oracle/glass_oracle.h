@@ -217,6 +217,7 @@ void   orc_unwrap_phase(int N, double *phase);
 /* 0 (default): the template path calls the platform libm's sin/cos, as the
  * reference does; 1: correctly rounded sin/cos (see orc_wdm.c). */
 void   orc_set_cr_trig(int on);
+void   orc_set_wavelet_layers(int L);
 
 /* ucb/src/glass_ucb_waveform.c:1003-1037 */
 void   orc_ucb_wavelet_layers(const orc_wdm *w, double Tobs, const double *params, int *jstart,

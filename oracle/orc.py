@@ -342,6 +342,7 @@ def _wdm_signatures(L):
     L.orc_numeric_orbit_armlength.argtypes = [c_int, _dp]
     L.orc_numeric_orbit_armlength.restype = c_d
     L.orc_set_cr_trig.argtypes = [c_int]
+    L.orc_set_wavelet_layers.argtypes = [c_int]
     L.orc_ucb_wavelet_response.argtypes = [c_int, _dp, _dp, c_d, c_d, _dp, _dp, _dp, _dp, _dp]
     L.orc_snr_wavelet.argtypes = [_dp, _dp, _dp, _dp, c_int, _ip, c_int]
     L.orc_snr_wavelet.restype = c_d
@@ -484,17 +485,22 @@ class cr_trig:
         lib().orc_set_cr_trig(0)
 
 
-def ucb_waveform_wavelet(w, orb, Tobs, params, t0=0.0):
-    """Compact ucb_waveform_wavelet: (list [Nw] int32, values [3, Nw])."""
+def ucb_waveform_wavelet(w, orb, Tobs, params, t0=0.0, layers=0):
+    """Compact ucb_waveform_wavelet: (list [Nw] int32, values [3, Nw]).
+    layers > 0 forces that layer count (orc_set_wavelet_layers)."""
     t, _, coef = orb
     p = np.ascontiguousarray(params, dtype=np.float64)
-    cap = 4 * w.NT
+    cap = max(4, layers + 1) * w.NT
     lst = np.zeros(cap, dtype=np.int32)
     vals = np.zeros((3, cap))
     nw = ctypes.c_int()
-    lib().orc_ucb_waveform_wavelet_compact(w.ref, len(t), _d(t), _d(coef), float(Tobs), float(t0),
-                                           _d(p), _i(lst), ctypes.byref(nw), _d(vals[0]),
-                                           _d(vals[1]), _d(vals[2]))
+    lib().orc_set_wavelet_layers(int(layers))
+    try:
+        lib().orc_ucb_waveform_wavelet_compact(w.ref, len(t), _d(t), _d(coef), float(Tobs), float(t0),
+                                               _d(p), _i(lst), ctypes.byref(nw), _d(vals[0]),
+                                               _d(vals[1]), _d(vals[2]))
+    finally:
+        lib().orc_set_wavelet_layers(0)
     n = nw.value
     return lst[:n].copy(), vals[:, :n].copy()
 
@@ -587,7 +593,8 @@ def gaussian_log_likelihood_wavelet(w, data, model, mlist, invC9):
                                                      npix)
 
 
-def wdm_eval_batch(w, orb, Tobs, params, data, invC9, t0=0.0, nthreads=0):
+def wdm_eval_batch(w, orb, Tobs, params, data, invC9, t0=0.0, nthreads=0, layers=0):
+    """(dlogL, snr, nwave, seconds); layers > 0 forces that layer count."""
     t, _, coef = orb
     p = np.ascontiguousarray(params, dtype=np.float64)
     d = np.ascontiguousarray(data, dtype=np.float64)
@@ -596,8 +603,12 @@ def wdm_eval_batch(w, orb, Tobs, params, data, invC9, t0=0.0, nthreads=0):
     dl = np.zeros(n)
     sn = np.zeros(n)
     nw = np.zeros(n, dtype=np.int32)
-    secs = lib().orc_wdm_eval_batch(w.ref, len(t), _d(t), _d(coef), float(Tobs), float(t0), n, _d(p),
-                                    _d(d), _d(ic), d.shape[1], _d(dl), _d(sn), _i(nw), nthreads)
+    lib().orc_set_wavelet_layers(int(layers))
+    try:
+        secs = lib().orc_wdm_eval_batch(w.ref, len(t), _d(t), _d(coef), float(Tobs), float(t0), n, _d(p),
+                                        _d(d), _d(ic), d.shape[1], _d(dl), _d(sn), _i(nw), nthreads)
+    finally:
+        lib().orc_set_wavelet_layers(0)
     return dl, sn, nw, secs
 
 

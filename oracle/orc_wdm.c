@@ -763,12 +763,20 @@ void orc_ucb_wavelet_response(int Norb, const double *ot, const double *oc, doub
     free(time_ssb);
 }
 
+/* A layer count forced on every template (0: ucb_wavelet_layers' own).  The
+ * reference has no such switch; it is the timing harness's override of
+ * Nlayers, applied after ucb_wavelet_layers as if that had returned it. */
+static int g_force_layers = 0;
+
+void orc_set_wavelet_layers(int L) { g_force_layers = L > 0 ? L : 0; }
+
 int orc_ucb_waveform_wavelet_compact(const orc_wdm *w, int Norb, const double *ot,
                                      const double *oc, double Tobs, double t0, const double *params,
                                      int *list, int *Nwavelet, double *wX, double *wY, double *wZ)
 {
     int min_layer, Nlayers;
     orc_ucb_wavelet_layers(w, Tobs, params, &min_layer, &Nlayers);
+    if (g_force_layers > 0) Nlayers = g_force_layers;
     int Nt = w->NT;
     int N_ds = Nt * (Nlayers + 1);
     double dt_ds = w->dt / (double)(Nlayers + 1);
@@ -802,7 +810,7 @@ int orc_ucb_waveform_wavelet(const orc_wdm *w, int Norb, const double *ot, const
                              double Tobs, double t0, const double *params, int *list,
                              int *Nwavelet, double *X, double *Y, double *Z)
 {
-    int cap = 4 * w->NT;
+    int cap = (g_force_layers > 3 ? g_force_layers + 1 : 4) * w->NT;
     double *v = malloc(3 * (size_t)cap * sizeof(double));
     int L = orc_ucb_waveform_wavelet_compact(w, Norb, ot, oc, Tobs, t0, params, list, Nwavelet, v,
                                              v + cap, v + 2 * cap);
@@ -914,7 +922,7 @@ double orc_wdm_eval_batch(const orc_wdm *w, int Norb, const double *ot, const do
     if (nthreads > 0) omp_set_num_threads(nthreads);
     t_start = omp_get_wtime();
 #endif
-    int cap = 4 * w->NT;
+    int cap = (g_force_layers > 3 ? g_force_layers + 1 : 4) * w->NT;
 #pragma omp parallel
     {
         int *list = malloc(cap * sizeof(int));

@@ -184,6 +184,49 @@ def test_wavelet_batch_vs_oracle(orc, W, g2):
     B.close()
 
 
+@pytest.mark.parametrize("layers", [7, 15, 31, 63])
+def test_wavelet_batch_forced_layers_vs_oracle(orc, W, g2, layers):
+    """A plan that forces every template to `layers` frequency layers (the
+    timing knob for templates touching 8-64 layers; long-series layer kernel
+    at NT 512): pixel lists, values, dlogL and snr against the oracle with
+    the same override; the wavelet Fisher refuses such a plan."""
+    import torch
+    ow, gw, oorb, gorb = g2
+    n = 16
+    cfg = synth.config2(n=n, seed=8)
+    P = cfg["params"]
+    B = W.WdmBatch(gw, gorb, T1, cfg["data"], cfg["invC6"], max_templates=n, layers=layers)
+    assert B.vcap == gw.NT * layers
+    dl, sn = B.eval(torch.tensor(P, device="cuda"))
+    dl, sn = dl.cpu().numpy(), sn.cpu().numpy()
+    nw, lst, val = B.templates(n)
+    with orc.cr_trig():
+        ref_dl, ref_sn, ref_nw, _ = orc.wdm_eval_batch(ow, oorb, T1, P, cfg["data"], cfg["invC9"], layers=layers)
+    assert np.array_equal(nw, ref_nw)
+    worst = 0.0
+    for e in range(0, n, 5):
+        with orc.cr_trig():
+            olst, ov = orc.ucb_waveform_wavelet(ow, oorb, T1, P[e], layers=layers)
+        assert np.array_equal(lst[e, :nw[e]], olst)
+        worst = max(worst, normwise(val[e, :, :nw[e]], ov))
+    rel_dl = np.max(np.abs(dl - ref_dl) / np.maximum(np.abs(ref_dl), 1.0))
+    rel_sn = np.max(np.abs(sn - ref_sn) / ref_sn)
+    print("forced", layers, "layers: template worst normwise (CR oracle)", worst, "dlogL rel", rel_dl, "snr rel", rel_sn)
+    assert worst < 1e-12
+    assert rel_dl < 1e-10 and rel_sn < 1e-10
+    with pytest.raises(Exception, match="forces"):
+        B.fisher(torch.tensor(P[:1], device="cuda"))
+    B.close()
+
+
+def test_wavelet_batch_forced_layers_refused(W, g2):
+    _, gw, _, gorb = g2
+    cfg = synth.config2(n=2, seed=9)
+    for bad in (5, 8, 127):
+        with pytest.raises(Exception, match="forced layer count"):
+            W.WdmBatch(gw, gorb, T1, cfg["data"], cfg["invC6"], max_templates=2, layers=bad)
+
+
 def test_snr_and_logl_wavelet_dropins(orc, W, g2):
     ow, gw, oorb, gorb = g2
     cfg = synth.config2(n=4, seed=6)

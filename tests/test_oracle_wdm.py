@@ -67,7 +67,8 @@ def test_wdm_round_trip_random(orc, NF, NT):
     assert np.array_equal(pix, pix2)
 
 
-@pytest.mark.parametrize("jmin,L,kb", [(11, 2, 3), (12, 2, 20), (20, 3, 5), (21, 1, 9), (30, 3, 30), (31, 3, 41)])
+@pytest.mark.parametrize("jmin,L,kb", [(11, 2, 3), (12, 2, 20), (20, 3, 5), (21, 1, 9), (30, 3, 30), (31, 3, 41),
+                                       (2, 7, 5), (3, 15, 11), (2, 31, 40)])
 def test_by_layers_heterodyne_identity(orc, jmin, L, kb):
     NF, NT, dt = 64, 32, 100.0
     w = orc.Wdm(NF, NT, dt)
@@ -182,6 +183,36 @@ def test_wavelet_waveform_vs_full_cadence(orc, grid2):
         inner = (i >= w.imin) & (i <= w.imax)
         err = np.abs(a - np.sqrt(L + 1) * v)[:, inner].max() / np.abs(a).max()
         assert err < 1e-5, err
+
+
+@pytest.mark.parametrize("layers", [7, 15])
+def test_wavelet_waveform_forced_layers_vs_full_cadence(orc, grid2, layers):
+    """A forced layer count (the timing harness's override of Nlayers) keeps
+    the template an exact by-layers transform: sqrt(layers + 1) times its
+    pixels equal the full transform of the full-cadence series on every
+    listed pixel, also in the layers the reference rule would not touch
+    (where both are small); the list covers layers jmin .. jmin + layers - 1."""
+    w, orb = grid2
+    NF, NT = w.NF, w.NT
+    ND = NF * NT
+    for p in _params(23, 2):
+        lst, v = orc.ucb_waveform_wavelet(w, orb, T1, p, layers=layers)
+        j0, L = orc.ucb_wavelet_layers(w, T1, p)
+        k = lst + w.kmin
+        assert set(np.unique(k // NT)) == set(range(j0, j0 + layers))
+        ser = orc.ucb_wavelet_series(orb, T1, p, ND, T1 / ND, 0.0)
+        full = np.stack([orc.wavelet_transform_timefreq(w, ser[c]) for c in range(3)])
+        a = full[:, k]
+        i = k % NT
+        inner = (i >= w.imin) & (i <= w.imax)
+        err = np.abs(a - np.sqrt(layers + 1) * v)[:, inner].max() / np.abs(a).max()
+        assert err < 1e-5, err
+        # the rule's own template, rescaled, is the same signal on its pixels
+        lr, vr = orc.ucb_waveform_wavelet(w, orb, T1, p)
+        pos = {int(x): n for n, x in enumerate(lst)}
+        sel = np.array([pos[int(x)] for x in lr])
+        d = np.abs(np.sqrt(layers + 1) * v[:, sel] - np.sqrt(L + 1) * vr)[:, inner[sel]]
+        assert d.max() < 2e-5 * np.abs(a).max()
 
 
 def test_wavelet_series_vs_fd_waveform(orc, grid2):
