@@ -78,31 +78,41 @@ GH_HD inline double poly_cos(double r)
     return w + (((1.0 - w) - h) + z * z * p);
 }
 
+// Flips the sign of x when bit 31 of m is set (an exact negation).
+GH_HD inline double flip_sign(double x, unsigned m)
+{
+    unsigned long long b;
+    __builtin_memcpy(&b, &x, sizeof b);
+    b ^= (unsigned long long)(m & 0x80000000u) << 32;
+    __builtin_memcpy(&x, &b, sizeof b);
+    return x;
+}
+
+// Without branches: arguments outside the domain are reduced as 0 and the
+// result replaced by NaN at the end; the quadrant rotation is a swap by
+// selects plus sign flips of the sign bits (bit-identical to negation).
 GH_HD inline void sincos_red(double x, double* s, double* c)
 {
-    if (!(fabs(x) < 536870912.0))  // outside |x| < 2^29 (or NaN): no result
-    {
-        *s = *c = __builtin_nan("");
-        return;
-    }
+    const bool bad = !(fabs(x) < 536870912.0);  // outside |x| < 2^29 (or NaN): no result
+    const double xs = bad ? 0.0 : x;
     const double TWO_OVER_PI = 0.63661977236758134308;
     const double C1 = 1.5707963267948966;          // pi/2 rounded
     const double C2 = 6.123233995736766e-17;       // pi/2 - C1
     const double C3 = -1.4973849048591698e-33;     // pi/2 - C1 - C2
-    const double k = rint(x * TWO_OVER_PI);
-    double r = fma(-k, C1, x);
+    const double k = rint(xs * TWO_OVER_PI);
+    double r = fma(-k, C1, xs);
     r = fma(-k, C2, r);
     r = fma(-k, C3, r);
     const double ps = poly_sin(r), pc = poly_cos(r);
-    const long q = (long)k & 3;
-    // x = k pi/2 + r: quadrant rotation of (sin r, cos r)
-    double ss, cc;
-    if (q == 0) { ss = ps; cc = pc; }
-    else if (q == 1) { ss = pc; cc = -ps; }
-    else if (q == 2) { ss = -ps; cc = -pc; }
-    else { ss = -pc; cc = ps; }
-    *s = ss;
-    *c = cc;
+    // x = k pi/2 + r, q = k mod 4: (sin, cos) = (ps, pc), (pc, -ps),
+    // (-ps, -pc), (-pc, ps)
+    const int q = (int)k;
+    const bool odd = (q & 1) != 0;
+    const double ss = odd ? pc : ps, cc = odd ? ps : pc;
+    const double so = flip_sign(ss, (unsigned)(q & 2) << 30);
+    const double co = flip_sign(cc, (unsigned)((q ^ (q >> 1)) & 1) << 31);
+    *s = bad ? __builtin_nan("") : so;
+    *c = bad ? __builtin_nan("") : co;
 }
 
 // Double-double values hi + lo (|lo| <= ulp(hi)/2), passed by value.  The

@@ -369,9 +369,20 @@ __global__ void __launch_bounds__(256) hdr_kernel(Geo g, int n, const double* __
     *(Hdr*)(scratch + scratch_doubles(g.Norb) * (size_t)e) = h;
 }
 
-// One lane's tridiagonal sweep of the natural spline through y (values at
-// the knots described by prep): writes z and y2 (N each).
-__device__ inline void spline_sweep(int N, const double* __restrict__ prep, const double* y, double* z, double* y2)
+// The right-hand side alpha_i (1 <= i <= N-2) of the natural spline
+// through y (knots described by prep), into z[i]: independent per knot, so
+// the whole workgroup computes them before one lane's sweep.
+__device__ inline void spline_alpha(int N, const double* __restrict__ prep, const double* y, double* z, int i)
+{
+#pragma clang fp contract(off)
+    const double* dx = prep;
+    z[i] = 3.0 * ((y[i + 1] - y[i]) / dx[i] - (y[i] - y[i - 1]) / dx[i - 1]);
+}
+
+// One lane's tridiagonal sweep of that spline, z holding the alpha_i of
+// spline_alpha: writes z and y2 (N each).  Only the recurrence itself is
+// serial (one division per knot each way it needs).
+__device__ inline void spline_sweep(int N, const double* __restrict__ prep, double* z, double* y2)
 {
 #pragma clang fp contract(off)
     const double *dx = prep, *l = prep + N, *mu = prep + 2 * N;
@@ -379,7 +390,7 @@ __device__ inline void spline_sweep(int N, const double* __restrict__ prep, cons
     double zp = 0.0;
     for (int i = 1; i < N - 1; i++)
     {
-        const double alpha = 3.0 * ((y[i + 1] - y[i]) / dx[i] - (y[i] - y[i - 1]) / dx[i - 1]);
+        const double alpha = z[i];
         zp = (alpha - dx[i - 1] * zp) / l[i];
         z[i] = zp;
     }
@@ -445,7 +456,13 @@ __global__ void __launch_bounds__(kRespThreads) __attribute__((amdgpu_waves_per_
         ssb[4 * N + i] = ucb_phase(t, h.f0, h.phi0, h.fdot);
     }
     __syncthreads();
-    if (tid < 2) spline_sweep(N, g.prep_o, ssb + 4 * N * tid, Mx + N * tid, ssb + 4 * N * tid + 2 * N);
+    for (int it = tid; it < 2 * (N - 2); it += kRespThreads)
+    {
+        const int sp = it / (N - 2), i = 1 + it % (N - 2);
+        spline_alpha(N, g.prep_o, ssb + 4 * N * sp, Mx + N * sp, i);
+    }
+    __syncthreads();
+    if (tid < 2) spline_sweep(N, g.prep_o, Mx + N * tid, ssb + 4 * N * tid + 2 * N);
     __syncthreads();
     for (int i = tid; i < 2 * N; i += kRespThreads)
     {
@@ -607,11 +624,14 @@ __global__ void __launch_bounds__(kRespThreads) __attribute__((amdgpu_waves_per_
     __syncthreads();
     // six TDI splines on the response times: spline s = 2*ch + (0 amp, 1 phase)
     double* y2s = As;  // 6N over As and the flip half of fl
-    if (tid < 6)
+    for (int it = tid; it < 6 * (N - 2); it += kRespThreads)
     {
-        const int ch = tid >> 1, ph = tid & 1;
-        spline_sweep(N, g.prep_s, Mx + (3 * ph + ch) * N, ssb + tid * N, y2s + tid * N);
+        const int sp = it / (N - 2), i = 1 + it % (N - 2);
+        const int ch = sp >> 1, ph = sp & 1;
+        spline_alpha(N, g.prep_s, Mx + (3 * ph + ch) * N, ssb + sp * N, i);
     }
+    __syncthreads();
+    if (tid < 6) spline_sweep(N, g.prep_s, ssb + tid * N, y2s + tid * N);
     __syncthreads();
     for (int it = tid; it < 6 * N; it += kRespThreads)
     {
