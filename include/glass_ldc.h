@@ -6,7 +6,9 @@
  * arrays of that many doubles (no memory crosses the boundary).  Every
  * function returns GLASS_LDC_OK (0) or a negative code, with the message in
  * glass_ldc_last_error() on the calling thread.  HDF5's own error printing
- * is off during each call.
+ * is off during each call.  Callable from several host threads when libhdf5
+ * is a thread-safe build (H5_HAVE_THREADSAFE, as the image's 1.10.6 is),
+ * which serialises the HDF5 calls themselves.
  */
 #ifndef GLASS_LDC_H
 #define GLASS_LDC_H
