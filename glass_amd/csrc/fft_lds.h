@@ -42,6 +42,29 @@ __device__ inline const T* opaque(const T* p)
     return (const T*)g;
 }
 
+// LDS slot of complex element i of a buffer transformed with the swizzled
+// layout (SWZ = true below): bits 0-3 of i XOR-ed with bits 2-5 and bit 0
+// also with bit 3.  The map is linear over GF(2) (slot(a ^ b) = slot(a) ^
+// slot(b)) and a bijection of every aligned block of 64 elements.  With it
+// the reads of consecutive elements stay conflict-free and so do the
+// strided writes of the first Stockham stages (64 bytes apart per lane at
+// Ns = 1, runs of four lanes 256 bytes apart at Ns = 4), which in the plain
+// layout put 2-4 lanes of every 8 or 16 on the same LDS banks.  Linearity
+// lets a butterfly swizzle its base index once: its R elements sit at
+// disjoint-bit offsets m*span, whose slots are compile-time constants.
+__device__ __forceinline__ int lds_swz(int i) { return i ^ ((i >> 2) & 15) ^ ((i >> 3) & 1); }
+template <bool SWZ>
+__device__ __forceinline__ int lds_slot(int i)
+{
+    return SWZ ? lds_swz(i) : i;
+}
+// Slot of a + c for a and c with no set bit in common, from the slot of a.
+template <bool SWZ>
+__device__ __forceinline__ int lds_slot_add(int slot_a, int c)
+{
+    return SWZ ? (slot_a ^ lds_swz(c)) : slot_a + c;
+}
+
 // Radix-R butterfly (R = 2, 3, 4) of twiddled inputs v into y:
 // y[m] = sum_q v[q] exp(-2 pi i q m / R).
 template <int R>
@@ -86,14 +109,15 @@ __device__ inline void radix_butterfly(const double2 (&v)[R], double2 (&y)[R])
 // multiple of N/R) every butterfly of a thread has the same index j and
 // shares one set.  Twiddle indices are in range for any b (j < N/R), so
 // the loads need no guard.
-template <int N, int NL, int R, int NTHR>
-__device__ inline void stockham_stage(double2* buf, int Ns, const double2* __restrict__ tw, int twstride)
+template <int N, int NL, int R, int NTHR, bool SWZ = false>
+__device__ __forceinline__ void stockham_stage(double2* buf, int Ns, const double2* __restrict__ tw, int twstride,
+                                               int tid = threadIdx.x)
 {
+    static_assert(!SWZ || (N % 64 == 0 && R != 3), "swizzled layout: power-of-two N >= 64");
     constexpr int NB = NL * (N / R);                 // butterflies in the stage
     constexpr int PER = (NB + NTHR - 1) / NTHR;      // per thread
     constexpr bool SAMEJ = NTHR % (N / R) == 0;
     constexpr int NW = SAMEJ ? 1 : PER;
-    const int tid = threadIdx.x;
     // twiddle exp(-2 pi i m k /(R Ns)) = tw[m*k*(NTW/(R*Ns))]; the first
     // stage (Ns = 1, a constant once inlined) has k = 0, all twiddles 1,
     // and skips the loads and products
@@ -115,9 +139,9 @@ __device__ inline void stockham_stage(double2* buf, int Ns, const double2* __res
         if (NB % NTHR == 0 || b < NB)
         {
             int l = b / (N / R), j = b % (N / R);
-            const double2* src = buf + l * N;
+            const int sj = lds_slot<SWZ>(j);  // j < N/R: disjoint from m*(N/R)
 #pragma unroll
-            for (int m = 0; m < R; m++) v[p][m] = src[j + m * (N / R)];
+            for (int m = 0; m < R; m++) v[p][m] = buf[l * N + lds_slot_add<SWZ>(sj, m * (N / R))];
         }
     }
     __syncthreads();
@@ -134,23 +158,33 @@ __device__ inline void stockham_stage(double2* buf, int Ns, const double2* __res
                 for (int m = 1; m < R; m++) v[p][m] = cmul(v[p][m], w[SAMEJ ? 0 : p][m - 1]);
             double2 y[R];
             radix_butterfly<R>(v[p], y);
-            double2* dst = buf + l * N;
-            int base = (j / Ns) * Ns * R + k;
+            // base has no bits in [log2 Ns, log2 (R Ns)), where m*Ns lives
+            const int sb = lds_slot<SWZ>((j / Ns) * Ns * R + k);
 #pragma unroll
-            for (int m = 0; m < R; m++) dst[base + m * Ns] = y[m];
+            for (int m = 0; m < R; m++) buf[l * N + lds_slot_add<SWZ>(sb, m * Ns)] = y[m];
         }
     }
     __syncthreads();
 }
 
 // Full forward FFT of NL length-N series in LDS. twstride = NTW / N.
-template <int N, int NL, int NTHR>
-__device__ inline void lds_fft(double2* buf, const double2* __restrict__ tw, int twstride)
+// SWZ: element n of series l lives at buf[l*N + lds_swz(n)] on entry and
+// on return.
+template <int N, int NL, int NTHR, bool SWZ = false>
+__device__ __forceinline__ void lds_fft(double2* buf, const double2* __restrict__ tw, int twstride)
 {
     constexpr bool R3 = N % 3 == 0;
     constexpr int P2 = R3 ? N / 3 : N;
     static_assert((P2 & (P2 - 1)) == 0 && N >= 4, "N = 2^a or 3*2^a");
     constexpr int LOG2 = __builtin_ctz(P2);
+    int tid = threadIdx.x;
+    if constexpr (SWZ)
+    {
+        // a fresh thread index per transform: its swizzled addresses are
+        // not shared with another transform's, so none stays live between
+        asm volatile("" : "+v"(tid));
+        tid &= 1023;  // the range threadIdx.x has
+    }
     int Ns = 1;
     if (R3)
     {
@@ -159,13 +193,13 @@ __device__ inline void lds_fft(double2* buf, const double2* __restrict__ tw, int
     }
     if (LOG2 & 1)
     {
-        stockham_stage<N, NL, 2, NTHR>(buf, Ns, tw, twstride);
+        stockham_stage<N, NL, 2, NTHR, SWZ>(buf, Ns, tw, twstride, tid);
         Ns *= 2;
     }
 #pragma unroll
     for (int s = 0; s < LOG2 / 2; s++)
     {
-        stockham_stage<N, NL, 4, NTHR>(buf, Ns, tw, twstride);
+        stockham_stage<N, NL, 4, NTHR, SWZ>(buf, Ns, tw, twstride, tid);
         Ns *= 4;
     }
 }
