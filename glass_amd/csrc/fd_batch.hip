@@ -69,10 +69,19 @@ struct BatchDev
     UcbConst* pc;             // per-item template constants, written by align_kernel
     // constants
     double T, t0, L, fstar;
-    const double* pos;  // positions table, kPosN samples
+    const double* pos;      // positions table, kPosN samples
+    const double* pos_soa;  // structure-of-arrays position blocks per series length
     const double2* tw;
     int fmt;
 };
+
+// Positions for fd_link_bins at N bins: the structure-of-arrays block of N
+// for the split shape (coalesced coordinate loads), else the record table.
+template <int N, int NTHR>
+__device__ __forceinline__ const double* batch_pos(const BatchDev& B)
+{
+    return fd_split(N, NTHR) ? B.pos_soa + pos_soa_offset(N) : B.pos;
+}
 
 // Work items: item e refers to entry q = sel ? sel[e] : e of the
 // (chain, src, params[8]) arrays.
@@ -203,8 +212,8 @@ __device__ __forceinline__ void eval_item(int w, double2* lds, const int* __rest
         const double* iC = B.invC + (size_t)B.g_off[g] * 6;
         double acc = 0.0, s2 = 0.0;
         // new-template bins (and the old template where they overlap)
-        fd_link_bins<N, NTHR, fd_split(N, NTHR)>(
-            k, lds, opaque(B.pos), kPosN / N, opaque(B.tw), kTwN / N, [&](int m, const double2* dl) {
+        fd_link_bins<N, NTHR, fd_split(N, NTHR), fd_split(N, NTHR)>(
+            k, lds, opaque(batch_pos<N, NTHR>(B)), kPosN / N, opaque(B.tw), kTwN / N, [&](int m, const double2* dl) {
                 double2 v[5];
                 fd_tdi<N>(k, m, B.fmt, dl, v);
                 if (it.stg)
@@ -309,7 +318,8 @@ __device__ __forceinline__ void tmpl_item(int w, double2* lds, const int* __rest
     const UcbConst& k = ucb_load_shared(B.pc + e);
     const int c = item_chain(it, e), cap = chain_cap(B, c);
     double2* o = src_tmpl(B, c, item_src(it, e));
-    fd_link_bins<N, NTHR, fd_split(N, NTHR)>(k, lds, opaque(B.pos), kPosN / N, opaque(B.tw), kTwN / N,
+    fd_link_bins<N, NTHR, fd_split(N, NTHR), fd_split(N, NTHR)>(k, lds, opaque(batch_pos<N, NTHR>(B)), kPosN / N,
+                                                                opaque(B.tw), kTwN / N,
                                              [&](int m, const double2* dl) {
                                                  double2 v[5];
                                                  fd_tdi<N>(k, m, B.fmt, dl, v);
@@ -909,9 +919,12 @@ extern "C" int glass_hip_fd_batch_create_ex(glass_hip_fd_batch** plan, const gla
         set_error("fd_batch_create: upload failed");
         return fail(GLASS_HIP_ERUNTIME);
     }
-    const double* pos;
+    const double *pos, *pos_soa;
     const double2* tw;
-    if ((rc = positions(*orbit, T, t0, kPosN, &pos)) || (rc = twiddles(&tw))) return fail(rc);
+    static_assert(kPosN == kPosSoaMax, "one position grid for both tables");
+    if ((rc = positions(*orbit, T, t0, kPosN, &pos)) || (rc = positions_soa(*orbit, T, t0, &pos_soa)) ||
+        (rc = twiddles(&tw)))
+        return fail(rc);
     BatchDev& B = P->B;
     B.g_qmin = g_qmin;
     B.g_W = g_W;
@@ -934,6 +947,7 @@ extern "C" int glass_hip_fd_batch_create_ex(glass_hip_fd_batch** plan, const gla
     B.L = orbit->L;
     B.fstar = orbit->fstar;
     B.pos = pos;
+    B.pos_soa = pos_soa;
     B.tw = tw;
     B.fmt = fmt;
     if ((rc = setup_grids(P))) return fail(rc);

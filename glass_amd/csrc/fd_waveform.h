@@ -211,11 +211,15 @@ __device__ inline void fd_link_spectra(const UcbConst& c, double2* lds, const do
 }
 
 // f(m, d) for every output bin m of this thread, d[l] = link l at bin m
-// as fd_link_spectra leaves it.  Ends with a barrier.
-template <int N, int NTHR, bool SPLIT, typename F>
+// as fd_link_spectra leaves it.  Ends with a barrier.  SOA (split shape
+// only): pos is the N-sample structure-of-arrays block (9 rows of N
+// samples) and posstride is unused; else pos is the record table read with
+// stride posstride.
+template <int N, int NTHR, bool SPLIT, bool SOA = false, typename F>
 __device__ inline void fd_link_bins(const UcbConst& c, double2* lds, const double* __restrict__ pos, int posstride,
                                     const double2* __restrict__ tw, int twstride, F&& f)
 {
+    static_assert(SPLIT || !SOA, "structure-of-arrays positions: split shape only");
     if constexpr (!SPLIT)
     {
         using S = FdShape<N, NTHR>;
@@ -240,12 +244,15 @@ __device__ inline void fd_link_bins(const UcbConst& c, double2* lds, const doubl
         const double inv2N = 1. / (double)(2 * N);
         const int t = threadIdx.x;
         const int ts = lds_slot<SW>(t);
-        ucb_slow_sample(c, ucb_sample_time(c.T, c.t0, t, N), pos + (size_t)t * posstride * 9, lds, H, ts);
+        constexpr int KS = SOA ? N : 1;
+        const double* p0 = SOA ? pos + t : pos + (size_t)t * posstride * 9;
+        const double* p1 = SOA ? pos + t + H : pos + (size_t)(t + H) * posstride * 9;
+        ucb_slow_sample<KS>(c, ucb_sample_time(c.T, c.t0, t, N), p0, lds, H, ts);
         // links 0-2 and 3-5 of the odd-bin series, shifted in as the
         // spacecraft loop produces them (l < 3 is known per unrolled link)
         const double2 w = tw[t * twstride];
         double2 ya[3], yb[3];
-        ucb_slow_sample_to<false>(c, ucb_sample_time(c.T, c.t0, t + H, N), pos + (size_t)(t + H) * posstride * 9,
+        ucb_slow_sample_to<false, KS>(c, ucb_sample_time(c.T, c.t0, t + H, N), p1,
                                   [&](int l, double2 x1) {
                                       const double2 x0 = lds[l * H + ts];
                                       lds[l * H + ts] = make_double2(x0.x + x1.x, x0.y + x1.y);

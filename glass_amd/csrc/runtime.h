@@ -75,6 +75,13 @@ int twiddles32(const double2** out);
 // t0 + T*n/npos (npos a power of two), 9 doubles per sample
 // (x1,x2,x3,y1,y2,y3,z1,z2,z3), resident on the current device.
 int positions(const glass_hip_orbit& o, double T, double t0, int npos, const double** out);
+// Spacecraft positions per series length N = kPosSoaMin..kPosSoaMax (powers
+// of two) in structure-of-arrays form: the block of N is 9 rows of N
+// samples (x1,x2,x3,y1,...,z3 at t0 + T n / N) at pos_soa_offset(N), so the
+// threads of a wave read consecutive samples of one coordinate.
+constexpr int kPosSoaMin = 32, kPosSoaMax = 4096;
+__host__ __device__ constexpr int pos_soa_offset(int N) { return 9 * (N - kPosSoaMin); }
+int positions_soa(const glass_hip_orbit& o, double T, double t0, const double** out);
 
 // Host restatement of analytic_orbits (utils/src/glass_lisa.c:77-124), used
 // only to fill the positions table.

@@ -95,6 +95,7 @@ static std::mutex g_tab_mu;
 static std::map<int, double2*> g_tw, g_tw3, g_tw16, g_tw32;
 typedef std::tuple<int, double, double, double, double, double, int> PosKey;
 static std::map<PosKey, double*> g_pos;
+static std::map<PosKey, double*> g_pos_soa;
 
 static int twiddle_table(std::map<int, double2*>& cache, int ntw, const double2** out)
 {
@@ -175,6 +176,45 @@ int positions(const glass_hip_orbit& o, double T, double t0, int npos, const dou
     GH_CHECK(hipMalloc(&d, sizeof(double) * h.size()));
     GH_CHECK(hipMemcpy(d, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
     g_pos[key] = d;
+    *out = d;
+    return GLASS_HIP_OK;
+}
+
+int positions_soa(const glass_hip_orbit& o, double T, double t0, const double** out)
+{
+    int dev = 0;
+    GH_CHECK(hipGetDevice(&dev));
+    PosKey key(dev, o.L, o.fstar, o.R, T, t0, kPosSoaMax);
+    std::lock_guard<std::mutex> lk(g_tab_mu);
+    auto it = g_pos_soa.find(key);
+    if (it != g_pos_soa.end())
+    {
+        *out = it->second;
+        return GLASS_HIP_OK;
+    }
+    // sample n of an N-sample series is sample n (kPosSoaMax / N) of the
+    // finest grid: t0 + T n / N, the same double (the ratio is a power of two)
+    std::vector<double> fine((size_t)kPosSoaMax * 9);
+    for (int n = 0; n < kPosSoaMax; n++)
+    {
+        double t;
+        {
+#pragma clang fp contract(off)
+            t = t0 + T * (double)(n) / (double)kPosSoaMax;
+        }
+        analytic_positions(o, t, &fine[(size_t)n * 9]);
+    }
+    std::vector<double> h((size_t)9 * (2 * kPosSoaMax - kPosSoaMin));
+    for (int N = kPosSoaMin; N <= kPosSoaMax; N *= 2)
+    {
+        double* blk = h.data() + pos_soa_offset(N);
+        for (int n = 0; n < N; n++)
+            for (int k = 0; k < 9; k++) blk[(size_t)k * N + n] = fine[(size_t)n * (kPosSoaMax / N) * 9 + k];
+    }
+    double* d = nullptr;
+    GH_CHECK(hipMalloc(&d, sizeof(double) * h.size()));
+    GH_CHECK(hipMemcpy(d, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+    g_pos_soa[key] = d;
     *out = d;
     return GLASS_HIP_OK;
 }

@@ -106,17 +106,18 @@ __device__ inline double quad3(const double* E, double a, double b, double cc)
 
 // One slow-series sample of the six links at time t: out(l, value) receives
 // link l in the order 12, 21, 31, 13, 23, 32.
-// pos: spacecraft positions x1,x2,x3,y1,y2,y3,z1,z2,z3 at t (metres).
+// pos: spacecraft positions x1,x2,x3,y1,y2,y3,z1,z2,z3 at t (metres), KS
+// doubles apart (1 in a sample record, N in a structure-of-arrays table).
 // glass_ucb_waveform.c:843-936, LISA_detector_tensor glass_lisa.c:1521-1564.
 // The loop runs over the sending spacecraft i; its two links (i,j) share
 // xi_i, arg2_i and the frequency at spacecraft i.  UNROLL unrolls it so that
 // l is a compile-time index (values may then stay in registers).
-template <bool UNROLL, typename Out>
+template <bool UNROLL, int KS = 1, typename Out>
 __device__ inline void ucb_slow_sample_to(const UcbConst& c, double t, const double* __restrict__ pos, Out&& out)
 {
-    const double x1 = pos[0], x2 = pos[1], x3 = pos[2];
-    const double y1 = pos[3], y2 = pos[4], y3 = pos[5];
-    const double z1 = pos[6], z2 = pos[7], z3 = pos[8];
+    const double x1 = pos[0], x2 = pos[KS], x3 = pos[2 * KS];
+    const double y1 = pos[3 * KS], y2 = pos[4 * KS], y3 = pos[5 * KS];
+    const double z1 = pos[6 * KS], z2 = pos[7 * KS], z3 = pos[8 * KS];
 
     // unit arm vectors (amplitude path)
     const double r12a = (x2 - x1) * c.invL, r12b = (y2 - y1) * c.invL, r12c = (z2 - z1) * c.invL;
@@ -184,10 +185,11 @@ __device__ inline void ucb_slow_sample_to(const UcbConst& c, double t, const dou
 }
 
 // The sample written to lds[l*stride + j].
+template <int KS = 1>
 __device__ inline void ucb_slow_sample(const UcbConst& c, double t, const double* __restrict__ pos,
                                        double2* lds, int stride, int j)
 {
-    ucb_slow_sample_to<false>(c, t, pos, [&](int l, double2 v) { lds[l * stride + j] = v; });
+    ucb_slow_sample_to<false, KS>(c, t, pos, [&](int l, double2 v) { lds[l * stride + j] = v; });
 }
 
 // Time of slow sample n (0-based) of a BW-sample series:
