@@ -291,7 +291,30 @@ def wdm_leg(gl, synth, n=4096, steps=5, seed=7, cpu_seconds=6.0, nthreads=0, par
         W.wdm_transform(w, x, out)
     e1.record()
     torch.cuda.synchronize()
+    tms_call = e0.elapsed_time(e1) / steps
+    # the device rate: the three launches of one transform replayed from a
+    # graph (per call, the host's issue time exceeds the ~60 us of kernels);
+    # every replay must give the eager pixels bit for bit
+    ref_pix = out.clone()
+    gs = torch.cuda.Stream()
+    gs.wait_stream(torch.cuda.current_stream())
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=gs):
+        W.wdm_transform(w, x, out)
+    torch.cuda.synchronize()
+    graph_equal = True
+    for _ in range(3):
+        out.zero_()
+        graph.replay()
+        torch.cuda.synchronize()
+        graph_equal = graph_equal and bool(torch.equal(out, ref_pix))
+    e0.record()
+    for _ in range(steps):
+        graph.replay()
+    e1.record()
+    torch.cuda.synchronize()
     tms = e0.elapsed_time(e1) / steps
+    del graph
     # plain: the per-call drop-in (host arrays)
     rate = None
     if plain:
@@ -322,7 +345,8 @@ def wdm_leg(gl, synth, n=4096, steps=5, seed=7, cpu_seconds=6.0, nthreads=0, par
             W.ucb_fisher_wavelet(w, orb, T, cfg["params"][nfp % n], cfg["invC9"])
             nfp += 1
         frate = nfp / (time.perf_counter() - t0)
-    res = dict(value=n / (ms * 1e-3), ms=ms, n=n, layers=layers, transform_ms=tms, ND=ND, plain=rate,
+    res = dict(value=n / (ms * 1e-3), ms=ms, n=n, layers=layers, transform_ms=tms, transform_ms_call=tms_call,
+               transform_graph_equal=graph_equal, ND=ND, plain=rate,
                fisher=dict(n=nf, ms=fms, value=nf / (fms * 1e-3), plain=frate))
     B.close()
     # the same templates with a forced layer count (templates touching 8-64
@@ -1055,6 +1079,10 @@ def main():
                "plain_dropin": None if wl["plain"] is None else round(wl["plain"], 1),
                "vs_plain": None if wl["plain"] is None else round(wl["value"] / wl["plain"], 1),
                "transform": {"ND": wl["ND"], "ms": round(wl["transform_ms"], 4),
+                             "timing": "graph replay of the three launches (device rate); ms_per_call: "
+                                       "the same through the per-call API from Python",
+                             "ms_per_call": round(wl["transform_ms_call"], 4),
+                             "graph_replay_bitwise_equal": wl["transform_graph_equal"],
                              "algorithmic_GBs": round(byts / (wl["transform_ms"] * 1e-3) / 1e9, 1),
                              "hbm_frac": round(byts / (wl["transform_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                              "normwise_err_vs_oracle": wl.get("transform_normwise_err")},
@@ -1078,7 +1106,8 @@ def main():
                                    "sample": f"{wl['cpu']['n']} templates x {wl['cpu']['reps']} by the oracle "
                                              f"restatement, OpenMP over templates, {wl['cpu']['secs']:.1f} s"}
         log(f"wdm: {wl['value']:.4e} templates/s ({wl['ms']:.3f} ms / {wl['n']}), transform "
-            f"{wl['transform_ms']:.3f} ms, plain {wl['plain']}/s, parity {wl.get('parity')}; forced layers "
+            f"{wl['transform_ms']:.3f} ms (graph; per call {wl['transform_ms_call']:.3f} ms, replays equal "
+            f"{wl['transform_graph_equal']}), plain {wl['plain']}/s, parity {wl.get('parity')}; forced layers "
             + ", ".join(f"L={k}: {v['value']:.3e}/s" for k, v in (wl.get('forced_layers') or {}).items()))
 
     inner = None

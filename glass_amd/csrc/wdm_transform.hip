@@ -33,7 +33,18 @@ namespace wdm {
 
 void wdm_window(int Nf, int Nt, double A, double* phif);  // wdm_ucb.hip
 
-constexpr int kTfThreads = 256;
+constexpr int kTfThreads = 256;  // the gather kernel and the PSD kernels
+
+// Threads of a workgroup that transforms count series of length n in LDS:
+// one radix-4 butterfly per thread and stage (n count / 4), a power of two
+// from 64 to 1024.  With fewer threads than butterflies the workgroups of
+// the column and row passes held one wave per SIMD.
+__host__ __device__ constexpr int fft_threads(int n, int count)
+{
+    int b = n * count / 4, t = 64;
+    while (t * 2 <= b && t < 1024) t *= 2;
+    return t;
+}
 constexpr double kPI2 = 6.283185307179586;
 
 // twiddle table (and its stride) for an FFT of length n
@@ -48,6 +59,16 @@ __host__ __device__ constexpr int fold_count(int n)  // transforms per workgroup
     return n > 2048 ? 1 : (n > 1024 ? 2 : 4);
 }
 
+// Work index of workgroup b of nb: workgroups are dealt to the 8 XCDs in
+// turn (b mod 8), so consecutive indices go to one XCD's blocks in order
+// and neighbouring columns / rows / layers, which share cache lines, meet
+// in the same L2.  The last nb mod 8 workgroups keep their own index.
+__device__ inline int xcd_order(int b, int nb)
+{
+    const int full = nb & ~7;
+    return b < full ? (b & 7) * (full >> 3) + (b >> 3) : b;
+}
+
 __device__ inline double2 twiddle_of(long long p, long long M)
 {
     // exp(-2 pi i p / M), 0 <= p < M
@@ -56,22 +77,22 @@ __device__ inline double2 twiddle_of(long long p, long long M)
     return make_double2(c, s);
 }
 
-template <int M1>
-__global__ void __launch_bounds__(kTfThreads) col_kernel(const double2* __restrict__ z, double2* __restrict__ y,
-                                                         int M2, TwSel t)
+template <int M1, int NTH = fft_threads(M1, fold_count(M1))>
+__global__ void __launch_bounds__(NTH) col_kernel(const double2* __restrict__ z, double2* __restrict__ y, int M2,
+                                                  TwSel t)
 {
     constexpr int TC = fold_count(M1);
     extern __shared__ double2 buf[];
-    const int col0 = blockIdx.x * TC;
+    const int col0 = xcd_order(blockIdx.x, gridDim.x) * TC;
     const long long M = (long long)M1 * M2;
-    for (int i = threadIdx.x; i < TC * M1; i += kTfThreads)
+    for (int i = threadIdx.x; i < TC * M1; i += NTH)
     {
         const int n1 = i / TC, c = i % TC;
         buf[c * M1 + n1] = z[(size_t)n1 * M2 + col0 + c];
     }
     __syncthreads();
-    lds_fft<M1, TC, kTfThreads>(buf, t.tw, t.stride);
-    for (int i = threadIdx.x; i < TC * M1; i += kTfThreads)
+    lds_fft<M1, TC, NTH>(buf, t.tw, t.stride);
+    for (int i = threadIdx.x; i < TC * M1; i += NTH)
     {
         const int k1 = i / TC, c = i % TC;
         const double2 v = buf[c * M1 + k1];
@@ -80,21 +101,21 @@ __global__ void __launch_bounds__(kTfThreads) col_kernel(const double2* __restri
     }
 }
 
-template <int M2>
-__global__ void __launch_bounds__(kTfThreads) row_kernel(const double2* __restrict__ y, double2* __restrict__ Z,
-                                                         int M1, TwSel t)
+template <int M2, int NTH = fft_threads(M2, fold_count(M2))>
+__global__ void __launch_bounds__(NTH) row_kernel(const double2* __restrict__ y, double2* __restrict__ Z, int M1,
+                                                  TwSel t)
 {
     constexpr int TR = fold_count(M2);
     extern __shared__ double2 buf[];
-    const int row0 = blockIdx.x * TR;
-    for (int i = threadIdx.x; i < TR * M2; i += kTfThreads)
+    const int row0 = xcd_order(blockIdx.x, gridDim.x) * TR;
+    for (int i = threadIdx.x; i < TR * M2; i += NTH)
     {
         const int r = i / M2, n2 = i % M2;
         buf[r * M2 + n2] = y[(size_t)(row0 + r) * M2 + n2];
     }
     __syncthreads();
-    lds_fft<M2, TR, kTfThreads>(buf, t.tw, t.stride);
-    for (int i = threadIdx.x; i < TR * M2; i += kTfThreads)
+    lds_fft<M2, TR, NTH>(buf, t.tw, t.stride);
+    for (int i = threadIdx.x; i < TR * M2; i += NTH)
     {
         const int k2 = i / TR, r = i % TR;
         Z[(size_t)(row0 + r) + (size_t)M1 * k2] = buf[r * M2 + k2];
@@ -115,16 +136,15 @@ __device__ inline double2 real_bin(const double2* __restrict__ Z, long long k, l
 // One layer m of wavelet_transform_freq.  FROM_Z: the spectrum is formed
 // from the complex FFT Z of the interleaved samples; otherwise src holds
 // the ND/2+1 one-sided bins.
-template <int NT, bool FROM_Z>
-__global__ void __launch_bounds__(kTfThreads) layer_fwd_kernel(const double2* __restrict__ src,
-                                                               double* __restrict__ wdm, int NF,
-                                                               const double* __restrict__ win, TwSel t)
+template <int NT, bool FROM_Z, int NTH = fft_threads(NT, 1)>
+__global__ void __launch_bounds__(NTH) layer_fwd_kernel(const double2* __restrict__ src, double* __restrict__ wdm,
+                                                        int NF, const double* __restrict__ win, TwSel t)
 {
     extern __shared__ double2 buf[];
-    const int m = blockIdx.x;
+    const int m = xcd_order(blockIdx.x, gridDim.x);
     const int center = m * NT / 2;
     const long long M = (long long)NF * NT / 2;
-    for (int jj = threadIdx.x; jj < NT; jj += kTfThreads)
+    for (int jj = threadIdx.x; jj < NT; jj += NTH)
     {
         const int j = jj - NT / 2;
         const int idx = center + j;
@@ -141,8 +161,8 @@ __global__ void __launch_bounds__(kTfThreads) layer_fwd_kernel(const double2* __
         buf[jj] = v;
     }
     __syncthreads();
-    lds_fft<NT, 1, kTfThreads>(buf, t.tw, t.stride);
-    for (int n = threadIdx.x; n < NT; n += kTfThreads)
+    lds_fft<NT, 1, NTH>(buf, t.tw, t.stride);
+    for (int n = threadIdx.x; n < NT; n += NTH)
     {
         const double re = buf[n].x / NT, im = -buf[n].y / NT;
         if (m == 0)
@@ -163,13 +183,13 @@ __global__ void __launch_bounds__(kTfThreads) layer_fwd_kernel(const double2* __
 
 // wavelet_transform_inverse_freq, first half: per layer the prefactors and
 // their forward FFT of NT.
-template <int NT>
-__global__ void __launch_bounds__(kTfThreads) layer_inv_kernel(const double* __restrict__ wdm,
-                                                               double2* __restrict__ F, int NF, TwSel t)
+template <int NT, int NTH = fft_threads(NT, 1)>
+__global__ void __launch_bounds__(NTH) layer_inv_kernel(const double* __restrict__ wdm, double2* __restrict__ F,
+                                                        int NF, TwSel t)
 {
     extern __shared__ double2 buf[];
     const int m = blockIdx.x;
-    for (int n = threadIdx.x; n < NT; n += kTfThreads)
+    for (int n = threadIdx.x; n < NT; n += NTH)
     {
         double2 v = make_double2(0.0, 0.0);
         if (m == 0) v.x = wdm[(2 * n) % NT] * M_SQRT1_2;
@@ -179,8 +199,8 @@ __global__ void __launch_bounds__(kTfThreads) layer_inv_kernel(const double* __r
         buf[n] = v;
     }
     __syncthreads();
-    lds_fft<NT, 1, kTfThreads>(buf, t.tw, t.stride);
-    for (int n = threadIdx.x; n < NT; n += kTfThreads) F[(size_t)m * NT + n] = buf[n];
+    lds_fft<NT, 1, NTH>(buf, t.tw, t.stride);
+    for (int n = threadIdx.x; n < NT; n += NTH) F[(size_t)m * NT + n] = buf[n];
 }
 
 // Second half: each output bin i sums the windowed layer spectra that
@@ -255,26 +275,26 @@ static TwSel tw_for(int n, const double2* tw, const double2* tw3)
 
 #define GH_COL(N)                                                                                        \
     case N:                                                                                              \
-        hipLaunchKernelGGL(col_kernel<N>, dim3(p.M2 / fold_count(N)), dim3(kTfThreads),                  \
+        hipLaunchKernelGGL(col_kernel<N>, dim3(p.M2 / fold_count(N)), dim3(fft_threads(N, fold_count(N))), \
                            sizeof(double2) * N * fold_count(N), s, z, y, p.M2, tw_for(N, tw, tw3));      \
         break;
 #define GH_ROW(N)                                                                                        \
     case N:                                                                                              \
-        hipLaunchKernelGGL(row_kernel<N>, dim3(p.M1 / fold_count(N)), dim3(kTfThreads),                  \
+        hipLaunchKernelGGL(row_kernel<N>, dim3(p.M1 / fold_count(N)), dim3(fft_threads(N, fold_count(N))), \
                            sizeof(double2) * N * fold_count(N), s, y, Z, p.M1, tw_for(N, tw, tw3));      \
         break;
 #define GH_LAYER(N)                                                                                      \
     case N:                                                                                              \
         if (fromZ)                                                                                       \
-            hipLaunchKernelGGL((layer_fwd_kernel<N, true>), dim3(p.NF + 1), dim3(kTfThreads),            \
+            hipLaunchKernelGGL((layer_fwd_kernel<N, true>), dim3(p.NF + 1), dim3(fft_threads(N, 1)),     \
                                sizeof(double2) * N, s, src, out, p.NF, win, tw_for(N, tw, tw3));         \
         else                                                                                             \
-            hipLaunchKernelGGL((layer_fwd_kernel<N, false>), dim3(p.NF + 1), dim3(kTfThreads),           \
+            hipLaunchKernelGGL((layer_fwd_kernel<N, false>), dim3(p.NF + 1), dim3(fft_threads(N, 1)),    \
                                sizeof(double2) * N, s, src, out, p.NF, win, tw_for(N, tw, tw3));         \
         break;
 #define GH_INV(N)                                                                                        \
     case N:                                                                                              \
-        hipLaunchKernelGGL(layer_inv_kernel<N>, dim3(p.NF + 1), dim3(kTfThreads), sizeof(double2) * N, s, \
+        hipLaunchKernelGGL(layer_inv_kernel<N>, dim3(p.NF + 1), dim3(fft_threads(N, 1)), sizeof(double2) * N, s, \
                            pix, F, p.NF, tw_for(N, tw, tw3));                                            \
         break;
 
