@@ -82,12 +82,13 @@ void spline_host(int N, const double* x, const double* y, double* c)
     }
 }
 
-// x-only terms of the recurrence: dx | l | mu (N each)
+// x-only terms of the recurrence: dx | l | mu | 1/l (N each; the
+// reciprocals correctly rounded, for spline_sweep's division)
 static void spline_prep_host(int N, const double* x, double* p)
 {
 #pragma clang fp contract(off)
-    double *dx = p, *l = p + N, *mu = p + 2 * N;
-    for (int i = 0; i < 3 * N; i++) p[i] = 0.0;
+    double *dx = p, *l = p + N, *mu = p + 2 * N, *rl = p + 3 * N;
+    for (int i = 0; i < 4 * N; i++) p[i] = 0.0;
     dx[0] = x[1] - x[0];
     for (int i = 1; i < N - 1; i++) dx[i] = x[i + 1] - x[i];
     l[0] = l[N - 1] = 1.0;
@@ -96,6 +97,7 @@ static void spline_prep_host(int N, const double* x, double* p)
         l[i] = 2.0 * (x[i + 1] - x[i - 1]) - dx[i - 1] * mu[i - 1];
         mu[i] = dx[i] / l[i];
     }
+    for (int i = 0; i < N; i++) rl[i] = 1.0 / l[i];
 }
 
 // ------------------------------------------------------ host: geometry --
@@ -159,9 +161,9 @@ struct Geo
     double Tobs, t0, dt, df, A, ot0, ods;
     const double* ot;      // Norb orbit knots
     const double* oc;      // 9 x 4 x Norb position splines (row 3*sc+coord)
-    const double* prep_o;  // 3 x Norb  dx | l | mu on the orbit knots
+    const double* prep_o;  // 4 x Norb  dx | l | mu | 1/l on the orbit knots
     const double* tssb;    // Norb response times t0 + i Tobs/(Norb-1)
-    const double* prep_s;  // 3 x Norb  on the response times
+    const double* prep_s;  // 4 x Norb  on the response times
     const double* resp;    // Norb x 24: x[3][3]/c, n[3][3], L[3] at tssb
     // per layer count L (slot L; a forced count Lf in slot kForcedSlot):
     const double* ds[kForcedSlot + 1];   // N_ds x 5: t, r0x, r0y, r0z, dx
@@ -226,7 +228,7 @@ static int geo_build(GeoHost& G, const glass_hip_wdm& w, int Norb, const double*
     std::vector<double> ot(ot_in, ot_in + N);
     g.ot0 = ot[0];
     g.ods = ot[1] - ot[0];
-    std::vector<double> oc(9 * 4 * N), prep_o(3 * N), tssb(N), prep_s(3 * N), resp(24 * N, 0.0);
+    std::vector<double> oc(9 * 4 * N), prep_o(4 * N), tssb(N), prep_s(4 * N), resp(24 * N, 0.0);
     for (int s = 0; s < 9; s++) spline_host(N, ot.data(), pos + s * N, oc.data() + 4 * s * N);
     spline_prep_host(N, ot.data(), prep_o.data());
     const double dts = Tobs / (double)(N - 1);
@@ -354,7 +356,13 @@ constexpr int kRespThreads = 256;
 // The sweep scratch of the splines lives in regions dead by then: the two
 // barycentre sweeps use M,Mf (not yet written), the six TDI sweeps use ssb
 // (z) and As + flip (y2), dead after the signed amplitude/phase step.
-__host__ __device__ inline size_t resp_lds_bytes(int N) { return sizeof(double) * 24 * (size_t)N + sizeof(Hdr); }
+// Then the sign-flip scan's knot flags: per channel three bit masks (local
+// minimum, the d2 and d3 tests) of kMaxKnots bits.
+constexpr int kFlagWords = kMaxKnots / 64;
+__host__ __device__ inline size_t resp_lds_bytes(int N)
+{
+    return sizeof(double) * 24 * (size_t)N + sizeof(Hdr) + sizeof(unsigned long long) * 9 * kFlagWords;
+}
 
 // Per-template constants, one thread per template, into each template's
 // scratch header (read by K1 and K2).
@@ -379,24 +387,36 @@ __device__ inline void spline_alpha(int N, const double* __restrict__ prep, cons
     z[i] = 3.0 * ((y[i + 1] - y[i]) / dx[i] - (y[i] - y[i - 1]) / dx[i - 1]);
 }
 
+// a / b correctly rounded from y = 1/b correctly rounded: q = a y is within
+// one ulp of a/b, e = a - b q is exact (fma), q + e y rounds to a/b
+// (Markstein's theorem; no underflow or overflow here).  Three dependent
+// operations instead of a division's chain of ten.
+__device__ inline double div_by_recip(double a, double b, double y)
+{
+    const double q = a * y;
+    return fma(fma(-q, b, a), y, q);
+}
+
 // One lane's tridiagonal sweep of that spline, z holding the alpha_i of
 // spline_alpha: writes z and y2 (N each).  Only the recurrence itself is
-// serial (one division per knot each way it needs).
+// serial; its division by l_i goes through the precomputed 1/l_i.
 __device__ inline void spline_sweep(int N, const double* __restrict__ prep, double* z, double* y2)
 {
 #pragma clang fp contract(off)
-    const double *dx = prep, *l = prep + N, *mu = prep + 2 * N;
+    const double *dx = prep, *l = prep + N, *mu = prep + 2 * N, *rl = prep + 3 * N;
     z[0] = 0.0;
     double zp = 0.0;
+#pragma unroll 4
     for (int i = 1; i < N - 1; i++)
     {
         const double alpha = z[i];
-        zp = (alpha - dx[i - 1] * zp) / l[i];
+        zp = div_by_recip(alpha - dx[i - 1] * zp, l[i], rl[i]);
         z[i] = zp;
     }
     z[N - 1] = 0.0;
     double yn = 0.0;
     y2[N - 1] = 0.0;
+#pragma unroll 4
     for (int i = N - 2; i >= 0; i--)
     {
         yn = z[i] - mu[i] * yn;
@@ -441,6 +461,7 @@ __global__ void __launch_bounds__(kRespThreads) __attribute__((amdgpu_waves_per_
     double* fl = As + 3 * N;         // flip X,Y,Z | pjump X,Y,Z
     double* phsc = fl + 6 * N;       // phase at spacecraft 1
     Hdr* H = (Hdr*)(phsc + N);
+    unsigned long long* flg = (unsigned long long*)(H + 1);  // 9 x kFlagWords
     double* scr = scratch + scratch_doubles(N) * (size_t)e;
     (void)params;
 
@@ -557,36 +578,89 @@ __global__ void __launch_bounds__(kRespThreads) __attribute__((amdgpu_waves_per_
     }
     __syncthreads();
     // sign flips of the amplitude (extract_amplitude_and_phase,
-    // utils/src/glass_math.c:1244-1296), one channel per lane
+    // utils/src/glass_math.c:1244-1296).  The tests at each knot (a local
+    // minimum of |M|, then |d2/d1| < 0.1 and |d3/d1| < 0.1) need no state:
+    // every wave evaluates 64 knots of a channel at once into ballot masks;
+    // one lane per channel then walks the knots in the reference's order
+    // (a d2 hit skips the next knot), holding the running flip and phase
+    // jump in registers.
+    {
+        const int NWd = (N + 63) / 64, lane = tid & 63;
+        for (int b = tid >> 6; b < 3 * NWd; b += kRespThreads / 64)
+        {
+            const int ch = b / NWd, w = b % NWd, i = w * 64 + lane;
+            bool mn = false, t2 = false, t3 = false;
+            if (i >= 1 && i <= N - 2)
+            {
+                const double* A = As + ch * N;
+                mn = A[i] < A[i - 1] && A[i] < A[i + 1];
+                if (mn)
+                {
+                    const double d1 = A[i + 1] + A[i - 1] - 2.0 * A[i];
+                    const double d2 = -A[i + 1] + A[i - 1] - 2.0 * A[i];
+                    const double d3 = -A[i + 1] + A[i - 1] + 2.0 * A[i];
+                    t2 = fabs(d2 / d1) < 0.1;
+                    t3 = fabs(d3 / d1) < 0.1;
+                }
+            }
+            const unsigned long long m0 = __ballot(mn), m2 = __ballot(t2), m3 = __ballot(t3);
+            if (lane == 0)
+            {
+                flg[(ch * 3 + 0) * kFlagWords + w] = m0;
+                flg[(ch * 3 + 1) * kFlagWords + w] = m2;
+                flg[(ch * 3 + 2) * kFlagWords + w] = m3;
+            }
+        }
+    }
+    __syncthreads();
     if (tid < 3)
     {
-        const double* A = As + tid * N;
+        const unsigned long long* fm = flg + tid * 3 * kFlagWords;
         double* flip = fl + tid * N;
         double* pj = fl + (3 + tid) * N;
         flip[0] = 1.0;
         pj[0] = 0.0;
+        double pf = 1.0, pp = 0.0;  // flip[i-1], pj[i-1]
+        int wc = -1;
+        unsigned long long wm = 0, w2 = 0, w3 = 0;
         int i = 1;
         do
         {
-            flip[i] = flip[i - 1];
-            pj[i] = pj[i - 1];
-            if (A[i] < A[i - 1] && A[i] < A[i + 1])
+            double cf = pf, cp = pp;  // flip[i] = flip[i-1]
+            const int w = i >> 6;
+            if (w != wc)
             {
-                const double d1 = A[i + 1] + A[i - 1] - 2.0 * A[i];
-                const double d2 = -A[i + 1] + A[i - 1] - 2.0 * A[i];
-                const double d3 = -A[i + 1] + A[i - 1] + 2.0 * A[i];
-                if (fabs(d2 / d1) < 0.1)
+                wc = w;
+                wm = fm[w];
+                w2 = fm[kFlagWords + w];
+                w3 = fm[2 * kFlagWords + w];
+            }
+            const unsigned long long bit = 1ull << (i & 63);
+            if (wm & bit)
+            {
+                const bool b2 = (w2 & bit) != 0, b3 = (w3 & bit) != 0;
+                if (b2)
                 {
-                    flip[i + 1] = -1.0 * flip[i];
-                    pj[i + 1] = pj[i] + M_PI;
+                    // flip[i+1] = -flip[i], pj[i+1] = pj[i] + pi; i++
+                    flip[i] = cf;
+                    pj[i] = cp;
+                    pf = cf;
+                    pp = cp;
+                    cf = -1.0 * cf;
+                    cp = cp + M_PI;
                     i++;
                 }
-                if (fabs(d3 / d1) < 0.1)
+                if (b3)
                 {
-                    flip[i] = -1.0 * flip[i - 1];
-                    pj[i] = pj[i - 1] + M_PI;
+                    // flip[i] = -flip[i-1], pj[i] = pj[i-1] + pi
+                    cf = -1.0 * pf;
+                    cp = pp + M_PI;
                 }
             }
+            flip[i] = cf;
+            pj[i] = cp;
+            pf = cf;
+            pp = cp;
             i++;
         } while (i < N - 1);
         flip[N - 1] = flip[N - 2];
@@ -606,12 +680,14 @@ __global__ void __launch_bounds__(kRespThreads) __attribute__((amdgpu_waves_per_
     // unwrap_phase (utils/src/glass_math.c:1034-1052)
     if (tid < 3)
     {
+        constexpr double kInvPI2 = 1.0 / PI2;  // correctly rounded (compile time)
         double* ph = Mx + (3 + tid) * N;
         double v = ph[0];
+#pragma unroll 4
         for (int i = 0; i < N; i++)
         {
             double u = ph[i];
-            const double q = rint(fabs(u - v) / PI2);
+            const double q = rint(div_by_recip(fabs(u - v), PI2, kInvPI2));
             if (q > 0.0)
             {
                 if (v > u) u += q * PI2;
