@@ -765,6 +765,40 @@ __device__ inline double2 spectrum(int ch, int k, const double2* U, const double
     return make_double2(E.x + (w.x * O.x - w.y * O.y), E.y + (w.x * O.y + w.y * O.x));
 }
 
+// Exclusive prefix sum of v over the workgroup in thread order, by wave
+// scans (__shfl_up) and one pass over the NTHR/64 wave totals; ws holds
+// NTHR/64 + 1 ints of LDS and ends with the total in ws[NTHR/64].  Ends
+// with a barrier.
+template <int NTHR>
+__device__ inline int block_exclusive_scan(int v, int* ws)
+{
+    static_assert(NTHR % 64 == 0, "wave64 workgroups");
+    constexpr int NW = NTHR / 64;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1)
+    {
+        const int y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) ws[w] = x;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        int sum = 0;
+        for (int i = 0; i < NW; i++)
+        {
+            const int t = ws[i];
+            ws[i] = sum;
+            sum += t;
+        }
+        ws[NW] = sum;
+    }
+    __syncthreads();
+    return ws[w] + x - v;
+}
+
 template <int NT, int L>
 __device__ void layer_body(const Geo& g, const Hdr& h, int e, const LayerArgs& a, double2* U, double2* V,
                            double2* B, int* cnt, const double* __restrict__ scr)
@@ -830,22 +864,8 @@ __device__ void layer_body(const Geo& g, const Hdr& h, int e, const LayerArgs& a
             const int k = c / L + (h.jmin + c % L) * NT;
             mine += (k >= g.kmin && k < g.kmax);
         }
-        cnt[tid] = mine;
-        __syncthreads();
-        if (tid == 0)
-        {
-            int s = 0;
-            for (int t = 0; t < kLayerThreads; t++)
-            {
-                const int v = cnt[t];
-                cnt[t] = s;
-                s += v;
-            }
-            a.nwave[e] = s;
-            cnt[kLayerThreads] = s;
-        }
-        __syncthreads();
-        int pos = cnt[tid];
+        int pos = block_exclusive_scan<kLayerThreads>(mine, cnt);
+        if (tid == 0) a.nwave[e] = cnt[kLayerThreads / 64];
         for (int c = c0; c < c0 + PER && c < NC; c++)
         {
             const int k = c / L + (h.jmin + c % L) * NT;
@@ -853,7 +873,7 @@ __device__ void layer_body(const Geo& g, const Hdr& h, int e, const LayerArgs& a
         }
     }
     __syncthreads();
-    const int Nw = cnt[kLayerThreads];
+    const int Nw = cnt[kLayerThreads / 64];
     const double* win = g.win[L];
     const double norm = sqrt((double)g.NF) / (L + 1.0);
     double* vals = a.values + (size_t)e * 3 * a.vcap;
@@ -1056,22 +1076,8 @@ __device__ void layer_body_big(const Geo& g, const Hdr& h, int e, const LayerArg
             const int k = c / L + (h.jmin + c % L) * NT;
             mine += (k >= g.kmin && k < g.kmax);
         }
-        cnt[tid] = mine;
-        __syncthreads();
-        if (tid == 0)
-        {
-            int sum = 0;
-            for (int t = 0; t < NTHR; t++)
-            {
-                const int v = cnt[t];
-                cnt[t] = sum;
-                sum += v;
-            }
-            a.nwave[e] = sum;
-            cnt[NTHR] = sum;
-        }
-        __syncthreads();
-        int pos = cnt[tid];
+        int pos = block_exclusive_scan<NTHR>(mine, cnt);
+        if (tid == 0) a.nwave[e] = cnt[NTHR / 64];
         for (int c = c0; c < c0 + PER && c < NC; c++)
         {
             const int k = c / L + (h.jmin + c % L) * NT;
@@ -1079,7 +1085,7 @@ __device__ void layer_body_big(const Geo& g, const Hdr& h, int e, const LayerArg
         }
     }
     __syncthreads();
-    const int Nw = cnt[NTHR];
+    const int Nw = cnt[NTHR / 64];
     const double* win = g.win[SLOT];
     const double norm = sqrt((double)g.NF) / (L + 1.0);
     double* vals = a.values + (size_t)e * 3 * a.vcap;
