@@ -53,7 +53,16 @@ constexpr double kAccUnits = 2.81837551648e-19;  // 1/(2 pi c)^2
 constexpr double kOmsUnits = 4.39256635604e-16;  // (2 pi / c)^2
 constexpr double kAccFmax = 0.01;                // Hz
 constexpr int kNThr = 256;
-constexpr int kTile = 8;  // chains per workgroup of the fused likelihood
+#ifndef GH_NOISE_TILE
+#define GH_NOISE_TILE 4
+#endif
+#ifndef GH_NOISE_WAVES
+#define GH_NOISE_WAVES 1
+#endif
+// chains per workgroup of the fused likelihood: each bin's transfer factors
+// and data products are read once per tile; the tile's chains are computed
+// side by side, so their temporaries set the register count
+constexpr int kTile = GH_NOISE_TILE;
 
 struct Basis
 {
@@ -309,21 +318,18 @@ __global__ void noise_prep_kernel(const double* __restrict__ f, const double2* _
 // Fused model -> (+ fixed covariance) -> inverse -> likelihood for kTile
 // chains per workgroup over one chunk of bins; partial sums part[chunk][i].
 // Every bin must have f > 0 (checked at plan creation).
-__global__ __launch_bounds__(kNThr) void noise_logl_kernel(int N, int chunk, const double* __restrict__ basis,
+__global__ __launch_bounds__(kNThr, GH_NOISE_WAVES) void noise_logl_kernel(int N, int chunk, const double* __restrict__ basis,
                                                            const double* __restrict__ prod,
                                                            const double* __restrict__ cadd,
                                                            const double* __restrict__ params, int n,
                                                            double* __restrict__ part)
 {
-    __shared__ double sp[kTile][12];
+    // the tile's parameters are read with workgroup-uniform addresses:
+    // scalar loads into SGPRs (through LDS they took vector registers,
+    // 12 per chain, held across the bin loop)
     const int c0 = blockIdx.y * kTile;
     const int nc = min(kTile, n - c0);
-    for (int i = threadIdx.x; i < kTile * 12; i += kNThr)
-    {
-        int c = i / 12;
-        sp[c][i % 12] = c < nc ? params[(size_t)(c0 + c) * 12 + i % 12] : 0.0;
-    }
-    __syncthreads();
+    const double* sp = params + (size_t)c0 * 12;
     const int b0 = blockIdx.x * chunk, b1 = min(N, b0 + chunk);
     const size_t P = (size_t)N;
     double acc[kTile];
@@ -350,19 +356,24 @@ __global__ __launch_bounds__(kNThr) void noise_logl_kernel(int N, int chunk, con
         {
             if (c >= nc) break;
             double m[6];
-            model_c3(sp[c], b, lowf, m);
+            model_c3(sp + 12 * c, b, lowf, m);
             if (cadd)
             {
 #pragma unroll
                 for (int e = 0; e < 6; e++) m[e] += a[e];
             }
             const double cxx = m[0], cyy = m[1], czz = m[2], cxy = m[3], cxz = m[4], cyz = m[5];
-            const double d1 = cxx;
-            const double d2 = cyy - cxy * cxy / d1;
-            const double t = cyz - cxy * cxz / d1;
-            const double d3 = (czz - cxz * cxz / d1) - (t * t) / d2;
-            const double ldet = log(d1) + log(d2) + log(d3);
-            const double r = 1.0 / (d1 * d2 * d3);
+            // the pivots of the reference's elimination; log det C and
+            // 1/det from their product (one logarithm and three divisions
+            // per chain and bin; det ~ 1e-120 for LISA noise, far from the
+            // double range's ends)
+            const double d1 = cxx, i1 = 1.0 / d1;
+            const double d2 = cyy - cxy * cxy * i1;
+            const double t = cyz - cxy * cxz * i1;
+            const double d3 = (czz - cxz * cxz * i1) - (t * t) / d2;
+            const double det = d1 * d2 * d3;
+            const double ldet = log(det);
+            const double r = 1.0 / det;
             const double ixx = (cyy * czz - cyz * cyz) * r, iyy = (czz * cxx - cxz * cxz) * r;
             const double izz = (cxx * cyy - cxy * cxy) * r, ixy = (cxz * cyz - czz * cxy) * r;
             const double ixz = (cxy * cyz - cxz * cyy) * r, iyz = (cxy * cxz - cxx * cyz) * r;
