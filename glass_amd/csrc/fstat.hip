@@ -172,11 +172,20 @@ __global__ __launch_bounds__(NTHR) void fstat_kernel(const int* __restrict__ lis
     const double theta = C.cell[(size_t)e * 4 + 2], phi = C.cell[(size_t)e * 4 + 3];
     const long q = (long)(f0 * D.T);
     double2 hx[S::PER][4], ha[S::PER][4], he[S::PER][4];
-    for (int fi = 0; fi < 4; fi++)
+    // the four filters' constants, one lane each (the same code on four lanes
+    // of the first wave: one setup's latency instead of four in turn)
+    __shared__ UcbConst sc4[4];
+    if (threadIdx.x < 4)
     {
+        const int fi = threadIdx.x;
         double p[8] = {f0 * D.T,    cos(theta),   phi,          D.fp[fi][0],
                        D.fp[fi][1], D.fp[fi][2], D.fp[fi][3], fdot * (D.T * D.T)};
-        const UcbConst& k = ucb_setup_shared(p, 8, D.T, D.t0, D.L, D.fstar);
+        ucb_setup(p, 8, D.T, D.t0, D.L, D.fstar, sc4[fi]);
+    }
+    __syncthreads();
+    for (int fi = 0; fi < 4; fi++)
+    {
+        const UcbConst& k = sc4[fi];
         double2 dl[S::PER][6];
         fd_link_spectra<N, NTHR>(k, lds, D.pos, kPosNF / N, D.tw, kTwN / N, dl);
 #pragma unroll
@@ -220,18 +229,36 @@ __global__ __launch_bounds__(NTHR) void fstat_kernel(const int* __restrict__ lis
                 }
         }
     }
+    // the 28 workgroup sums in one pass (one barrier instead of 56)
     constexpr int B = NTHR < 64 ? 64 : NTHR;
+    __shared__ double tot[28];
+    {
+        double v[28];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+        {
+            v[i] = nx[i];
+            v[4 + i] = nae[i];
+        }
+#pragma unroll
+        for (int i = 0; i < 10; i++)
+        {
+            v[8 + i] = mx[i];
+            v[18 + i] = mae[i];
+        }
+        block_sum_many<B, 28>(v, tot);
+    }
+    if (threadIdx.x != 0) return;
     for (int i = 0; i < 4; i++)
     {
-        nx[i] = block_sum<B>(nx[i]);
-        nae[i] = block_sum<B>(nae[i]);
+        nx[i] = tot[i];
+        nae[i] = tot[4 + i];
     }
     for (int i = 0; i < 10; i++)
     {
-        mx[i] = block_sum<B>(mx[i]);
-        mae[i] = block_sum<B>(mae[i]);
+        mx[i] = tot[8 + i];
+        mae[i] = tot[18 + i];
     }
-    if (threadIdx.x != 0) return;
     double MX[4][4], MA[4][4];
     int t = 0;
     for (int i = 0; i < 4; i++)
