@@ -81,11 +81,16 @@ __global__ __launch_bounds__(NTHR) void adjoint_kernel(DerivArgs A, int want_gra
     const int e = blockIdx.x;
     const int t = threadIdx.x;
     const double* p = A.params + (size_t)(A.pidx ? A.pidx[e] : e) * 8;
-    const UcbConst& c = ucb_setup_shared(p, 8, A.T, A.t0, A.L, A.fstar);
-    if (t == 0) ucb_diff_setup(p, A.T, sdd);
+    // the template constants and their derivatives on lanes of two waves,
+    // so the two serial setups overlap (one wave at 32/64 bins: in turn)
+    __shared__ UcbConst sc;
+    if (t == 0) ucb_setup(p, 8, A.T, A.t0, A.L, A.fstar, sc);
+    if (t == (NTHR >= 128 ? 64 : 0)) ucb_diff_setup(p, A.T, sdd);
+    __syncthreads();
+    const UcbConst& c = sc;
     const int posstride = 4096 / N;
     // forward: the six link spectra of this thread's bins (ends with a
-    // barrier, which also publishes sdd)
+    // barrier)
     double2 d[S::PER][6];
     fd_link_spectra<N, NTHR>(c, lds, A.pos, posstride, A.tw, kTwN / N, d);
     const UcbDiff& dd = sdd;
