@@ -727,7 +727,10 @@ __global__ void __launch_bounds__(kRespThreads) __attribute__((amdgpu_waves_per_
 
 // ---------------------------------------------------------- K2: layers --
 
-constexpr int kLayerThreads = 256;
+// 512 threads: the kernel's 74 KB of LDS (NT 512) admits two workgroups
+// per CU, so 512-thread workgroups give four waves per SIMD where 256-thread
+// ones gave two (111 VGPRs allow four)
+constexpr int kLayerThreads = 512;
 
 struct LayerArgs
 {
