@@ -42,12 +42,11 @@ __device__ inline void block_sum_vec(double (&v)[kNSum], double* out)
     constexpr int NW = (NTHR + 63) / 64;
     __shared__ double part[NW][kNSum];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-#pragma unroll
-    for (int k = 0; k < kNSum; k++)
-    {
-        double x = wave_sum(v[k]);
-        if (lane == 0) part[w][k] = x;
-    }
+    // the 45 wave sums by transposition: lane k ends with value k's sum
+    constexpr int GROUP = 64 / pow2_at_least(kNSum);  // lanes holding one sum
+    const double x = wave_sum_transpose<kNSum>(v);
+    const int k = lane / GROUP;
+    if (k < kNSum && lane % GROUP == 0) part[w][k] = x;
     __syncthreads();
     if (threadIdx.x < kNSum)
     {
@@ -101,18 +100,32 @@ __device__ inline void deriv_pass(const UcbConst& c, const UcbDiff& dd, double2*
     __syncthreads();
 }
 
-template <int N, int NTHR, int S0>
+// Passes over the sets below S1, the highest first: the tangent sets (2-5,
+// the costly samples) run while few TDI values of earlier passes are held in
+// registers, the basis sets (0, 1) last.
+template <int N, int NTHR, int S1>
 __device__ inline void deriv_passes(const UcbConst& c, const UcbDiff& dd, double2* lds, const DerivArgs& A,
                                     double2 (&H)[FdShape<N, NTHR>::PER][6][3])
 {
     constexpr int CAP = (deriv_lds_budget(N) / (96 * N)) < 1 ? 1 : (deriv_lds_budget(N) / (96 * N));
-    constexpr int NS = (6 - S0) < CAP ? (6 - S0) : CAP;
-    deriv_pass<N, NTHR, S0, NS>(c, dd, lds, A, H);
-    if constexpr (S0 + NS < 6) deriv_passes<N, NTHR, S0 + NS>(c, dd, lds, A, H);
+    constexpr int NS = S1 < CAP ? S1 : CAP;
+    deriv_pass<N, NTHR, S1 - NS, NS>(c, dd, lds, A, H);
+    if constexpr (S1 - NS > 0) deriv_passes<N, NTHR, S1 - NS>(c, dd, lds, A, H);
 }
 
+// Waves per SIMD the register allocation must allow (the launch bound's
+// second argument; 1 = the compiler's choice).  Only 256 bins can use more:
+// their 50 KB of LDS fits three workgroups per CU, the larger and smaller
+// sizes are held to two waves per SIMD by LDS.  Three waves cap the kernel
+// at 168 VGPRs (from 195, with 144 B of spill) and run the 256-bin Fisher
+// batch about 2% faster.
+#ifndef GH_DERIV_W256
+#define GH_DERIV_W256 3
+#endif
+constexpr int deriv_min_waves(int N) { return N == 256 ? GH_DERIV_W256 : 1; }
+
 template <int N, int NTHR>
-__global__ __launch_bounds__(NTHR) void deriv_kernel(DerivArgs A)
+__global__ __launch_bounds__(NTHR, deriv_min_waves(N)) void deriv_kernel(DerivArgs A)
 {
     using S = FdShape<N, NTHR>;
     constexpr int CAP = (deriv_lds_budget(N) / (96 * N)) < 1 ? 1 : (deriv_lds_budget(N) / (96 * N));
@@ -132,7 +145,7 @@ __global__ __launch_bounds__(NTHR) void deriv_kernel(DerivArgs A)
     __syncthreads();
     const UcbDiff& dd = sdd;
     double2 H[S::PER][6][3];
-    deriv_passes<N, NTHR, 0>(c, dd, lds, A, H);
+    deriv_passes<N, NTHR, 6>(c, dd, lds, A, H);
 
     const double2* R = A.iroff ? A.resid + A.iroff[e] : A.resid + (size_t)e * (A.rstride / 2);
     const double* iC = A.iioff ? A.invC + A.iioff[e] : A.invC + (size_t)e * A.istride;

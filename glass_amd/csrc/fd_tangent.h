@@ -98,7 +98,7 @@ __device__ inline double2 cmulc(double2 a, double2 b) { return make_double2(a.x 
 // Sets 0, 1: the plus / cross basis (the sample is DP s0 + DC s1, complex
 // products); 2..5: d/d(f0 T), d/dcos(theta), d/dphi, d/d(fdot T^2).
 template <int S0, int NS, typename Out>
-__device__ inline void ucb_sample_sets_to(const UcbConst& c, const UcbDiff& dd, double t,
+__device__ __forceinline__ void ucb_sample_sets_to(const UcbConst& c, const UcbDiff& dd, double t,
                                           const double* __restrict__ pos, Out&& out)
 {
     constexpr bool W01 = S0 <= 1 && S0 + NS > 0;  // any basis set
@@ -220,7 +220,7 @@ __device__ inline void ucb_sample_sets_to(const UcbConst& c, const UcbDiff& dd, 
 
 // The sets [S0, S0+NS) of one sample into lds[(set - S0)*6*stride + link*stride + j].
 template <int S0, int NS>
-__device__ inline void ucb_sample_sets(const UcbConst& c, const UcbDiff& dd, double t, const double* __restrict__ pos,
+__device__ __forceinline__ void ucb_sample_sets(const UcbConst& c, const UcbDiff& dd, double t, const double* __restrict__ pos,
                                        double2* lds, int stride, int j)
 {
     ucb_sample_sets_to<S0, NS>(c, dd, t, pos,
