@@ -115,6 +115,16 @@ GH_HD inline void sincos_red(double x, double* s, double* c)
     *c = bad ? __builtin_nan("") : co;
 }
 
+// a / b correctly rounded from y = 1/b correctly rounded: q = a y is within
+// one ulp of a/b, e = a - b q is exact (fma), q + e y rounds to a/b
+// (Markstein's theorem; no underflow or overflow).  Three dependent
+// operations instead of a division's chain of ten.
+GH_HD inline double div_by_recip(double a, double b, double y)
+{
+    const double q = a * y;
+    return fma(fma(-q, b, a), y, q);
+}
+
 // Double-double values hi + lo (|lo| <= ulp(hi)/2), passed by value.  The
 // error-free sum / product and everything built on them keep contraction
 // off: fusing a product into a later sum (across inlined calls) would drop

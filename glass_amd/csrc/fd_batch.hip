@@ -211,9 +211,19 @@ __device__ __forceinline__ void eval_item(int w, double2* lds, const int* __rest
         const double2* R = B.resid + (size_t)B.c_roff[c] * 3;
         const double* iC = B.invC + (size_t)B.g_off[g] * 6;
         double acc = 0.0, s2 = 0.0;
+        // the old template's values at bin m, loaded before the bin's
+        // transform (their latency, an HBM read per source, overlaps the FFT)
+        double2 hpre[3];
+        auto pre = [&](int m) {
+            const int j = imin_n + m, mo = j - imin_o;
+            const bool inold = j >= 0 && j < W && mo >= 0 && mo < bw_o;
+#pragma unroll
+            for (int ch = 0; ch < 3; ch++) hpre[ch] = inold ? ho[ch * cap + mo] : make_double2(0.0, 0.0);
+        };
         // new-template bins (and the old template where they overlap)
         fd_link_bins<N, NTHR, fd_split(N, NTHR), fd_split(N, NTHR)>(
-            k, lds, opaque(batch_pos<N, NTHR>(B)), kPosN / N, opaque(B.tw), kTwN / N, [&](int m, const double2* dl) {
+            k, lds, opaque(batch_pos<N, NTHR>(B)), kPosN / N, opaque(B.tw), kTwN / N, pre,
+            [&](int m, const double2* dl) {
                 double2 v[5];
                 fd_tdi<N>(k, m, B.fmt, dl, v);
                 if (it.stg)
@@ -226,12 +236,10 @@ __device__ __forceinline__ void eval_item(int w, double2* lds, const int* __rest
                 if (j >= 0 && j < W)
                 {
                     double2 d[3], r[3];
-                    const int mo = j - imin_o;
-                    const bool inold = mo >= 0 && mo < bw_o;
 #pragma unroll
                     for (int ch = 0; ch < 3; ch++)
                     {
-                        double2 h0 = inold ? ho[ch * cap + mo] : make_double2(0.0, 0.0);
+                        const double2 h0 = hpre[ch];  // zero outside the old template
                         d[ch] = make_double2(v[ch].x - h0.x, v[ch].y - h0.y);
                         r[ch] = R[(size_t)ch * W + j];
                     }

@@ -136,7 +136,7 @@ __device__ __forceinline__ void ucb_sample_sets_to(const UcbConst& c, const UcbD
         double xi, a2, fi, ae;
         {
 #pragma clang fp contract(off)
-            const double kdotx = (px * c.k[0] + py * c.k[1] + pz * c.k[2]) / kCLIGHT;
+            const double kdotx = div_by_recip(px * c.k[0] + py * c.k[1] + pz * c.k[2], kCLIGHT, kInvCLIGHT);
             xi = t - kdotx;
             fi = c.f0;
             a2 = c.pi2f0 * xi - c.phi0 - dft;
@@ -151,8 +151,8 @@ __device__ __forceinline__ void ucb_sample_sets_to(const UcbConst& c, const UcbD
         const double fon = fi * c.invfstar;
         // derivatives of xi, arg2, f_i, aevol: index 0 -> f0 T, 1 -> cos(theta),
         // 2 -> phi, 3 -> fdot T^2
-        const double dxi1 = -(px * dd.dk1[0] + py * dd.dk1[1] + pz * dd.dk1[2]) / kCLIGHT;
-        const double dxi2 = -(px * dd.dk2[0] + py * dd.dk2[1] + pz * dd.dk2[2]) / kCLIGHT;
+        const double dxi1 = -div_by_recip(px * dd.dk1[0] + py * dd.dk1[1] + pz * dd.dk1[2], kCLIGHT, kInvCLIGHT);
+        const double dxi2 = -div_by_recip(px * dd.dk2[0] + py * dd.dk2[1] + pz * dd.dk2[2], kCLIGHT, kInvCLIGHT);
         const double da2dxi = kPI2 * (dd.f0 + c.dfdt * xi);
         const double da2[4] = {kPI2 * xi * dd.invT, da2dxi * dxi1, da2dxi * dxi2, kPI * xi * xi * dd.invT2};
         const double dfo[4] = {dd.invT * c.invfstar, c.dfdt * dxi1 * c.invfstar, c.dfdt * dxi2 * c.invfstar,

@@ -215,9 +215,13 @@ __device__ inline void fd_link_spectra(const UcbConst& c, double2* lds, const do
 // only): pos is the N-sample structure-of-arrays block (9 rows of N
 // samples) and posstride is unused; else pos is the record table read with
 // stride posstride.
-template <int N, int NTHR, bool SPLIT, bool SOA = false, typename F>
+// pre(m) is called for each bin m before f(m, d) of that bin, as early as
+// the shape allows: in the split shape at the start of the bin's pass,
+// before its transform, so that global loads the caller issues there for
+// bin m are in flight during the FFT stages.
+template <int N, int NTHR, bool SPLIT, bool SOA = false, typename Pre, typename F>
 __device__ inline void fd_link_bins(const UcbConst& c, double2* lds, const double* __restrict__ pos, int posstride,
-                                    const double2* __restrict__ tw, int twstride, F&& f)
+                                    const double2* __restrict__ tw, int twstride, Pre&& pre, F&& f)
 {
     static_assert(SPLIT || !SOA, "structure-of-arrays positions: split shape only");
     if constexpr (!SPLIT)
@@ -229,7 +233,11 @@ __device__ inline void fd_link_bins(const UcbConst& c, double2* lds, const doubl
         for (int k = 0; k < S::PER; k++)
         {
             const int m = threadIdx.x + k * NTHR;
-            if (m < N) f(m, dl[k]);
+            if (m < N)
+            {
+                pre(m);
+                f(m, dl[k]);
+            }
         }
     }
     else
@@ -274,9 +282,10 @@ __device__ inline void fd_link_bins(const UcbConst& c, double2* lds, const doubl
 #pragma unroll
         for (int pass = 0; pass < 2; pass++)
         {
-            lds_fft<H, 6, NTHR, SW>(lds, opaque(tw), 2 * twstride);
             // bin m = 2t + pass is FFT index (m + H) mod N, of parity pass
             const int m = 2 * t + pass;
+            pre(m);
+            lds_fft<H, 6, NTHR, SW>(lds, opaque(tw), 2 * twstride);
             const int s = lds_slot<SW>(((m + H) & (N - 1)) >> 1);
             double2 d[6];
 #pragma unroll
@@ -301,6 +310,13 @@ __device__ inline void fd_link_bins(const UcbConst& c, double2* lds, const doubl
             if (!pass) __syncthreads();
         }
     }
+}
+
+template <int N, int NTHR, bool SPLIT, bool SOA = false, typename F>
+__device__ inline void fd_link_bins(const UcbConst& c, double2* lds, const double* __restrict__ pos, int posstride,
+                                    const double2* __restrict__ tw, int twstride, F&& f)
+{
+    fd_link_bins<N, NTHR, SPLIT, SOA>(c, lds, pos, posstride, tw, twstride, [](int) {}, f);
 }
 
 // TDI of template bin m from its six link values d (link order 12, 21, 31,

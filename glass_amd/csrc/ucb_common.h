@@ -21,6 +21,7 @@ namespace glass {
 constexpr double kPI2 = 6.283185307179586;
 constexpr double kPI = 3.14159265358979323846;
 constexpr double kCLIGHT = 299792458.;
+constexpr double kInvCLIGHT = 1.0 / kCLIGHT;  // correctly rounded (compile time)
 constexpr double kYEAR = 3.15581497632e7;
 constexpr double kAU = 1.49597870700e11;
 constexpr double kSQ3 = 1.73205080757;
@@ -143,7 +144,8 @@ __device__ inline void ucb_slow_sample_to(const UcbConst& c, double t, const dou
         double a2, fon, ae;
         {
 #pragma clang fp contract(off)
-            double kdotx = (px * c.k[0] + py * c.k[1] + pz * c.k[2]) / kCLIGHT;
+            // the quotient by c, as IEEE division gives it (div_by_recip)
+            double kdotx = div_by_recip(px * c.k[0] + py * c.k[1] + pz * c.k[2], kCLIGHT, kInvCLIGHT);
             double xi = t - kdotx;
             double fi = c.f0;
             a2 = c.pi2f0 * xi - c.phi0 - dft;

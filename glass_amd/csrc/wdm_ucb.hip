@@ -387,16 +387,6 @@ __device__ inline void spline_alpha(int N, const double* __restrict__ prep, cons
     z[i] = 3.0 * ((y[i + 1] - y[i]) / dx[i] - (y[i] - y[i - 1]) / dx[i - 1]);
 }
 
-// a / b correctly rounded from y = 1/b correctly rounded: q = a y is within
-// one ulp of a/b, e = a - b q is exact (fma), q + e y rounds to a/b
-// (Markstein's theorem; no underflow or overflow here).  Three dependent
-// operations instead of a division's chain of ten.
-__device__ inline double div_by_recip(double a, double b, double y)
-{
-    const double q = a * y;
-    return fma(fma(-q, b, a), y, q);
-}
-
 // One lane's tridiagonal sweep of that spline, z holding the alpha_i of
 // spline_alpha: writes z and y2 (N each).  Only the recurrence itself is
 // serial; its division by l_i goes through the precomputed 1/l_i.

@@ -1,7 +1,8 @@
 """The wavelet response stage divides by a correctly rounded reciprocal with
 one FMA correction (q = a y, q' = q + (a - b q) y with y = RN(1/b)), in
-place of a division, on its serial spline sweeps and phase unwrap, and the
-templates are claimed unchanged bit for bit.  Markstein's theorem says q'
+place of a division, on its serial spline sweeps and phase unwrap, and so
+do the light-time delays k.x / c of every slow sample (FD, tangent and
+wavelet templates); the templates are claimed unchanged bit for bit.  Markstein's theorem says q'
 is the correctly rounded a/b (round to nearest, no underflow or overflow);
 this checks it against IEEE division, compiled with the system C compiler's
 fma, on random operands across 120 binades, on divisors near 1-2 (the
@@ -56,6 +57,19 @@ long check(long n) {
     }
     return bad;
 }
+/* the light-time quotient k.x / c: numerators across the positions' range
+   (|k.x| up to ~2e11 m, down to cancellation near 0) */
+long check_c(long n) {
+    long bad = 0;
+    const double C = 299792458.;
+    volatile double yv = 1.0 / C;
+    const double y = yv;
+    for (long k = 0; k < n; k++) {
+        double a = rnd(-40, 45);
+        if (div_by_recip(a, C, y) != a / C) bad++;
+    }
+    return bad;
+}
 '''
 
 
@@ -69,11 +83,17 @@ def lib(tmp_path_factory):
     L = ctypes.CDLL(str(so))
     L.check.restype = ctypes.c_long
     L.check.argtypes = [ctypes.c_long]
+    L.check_c.restype = ctypes.c_long
+    L.check_c.argtypes = [ctypes.c_long]
     return L
 
 
 def test_reciprocal_fma_division_is_correctly_rounded(lib):
     assert lib.check(10_000_000) == 0
+
+
+def test_light_time_quotient_is_correctly_rounded(lib):
+    assert lib.check_c(20_000_000) == 0
 
 
 def test_the_check_sees_a_plain_reciprocal_product(tmp_path):
