@@ -797,7 +797,37 @@ def globalfit_leg(gl, synth, steps=3, seed=4, cpu_seconds=2.0, nthreads=0, seria
     e1.record()
     torch.cuda.synchronize()
     ms_der = e0.elapsed_time(e1) / steps
+    # per-class kernel times of the eval part (HIP events around each class
+    # kernel on its launch stream, the classes in turn): the roofline of the
+    # dominant class kernel, and the algorithmic FP64 rate of all classes
+    plan.profile(True)
+    for _ in range(steps):
+        plan.eval(E)
+    kms, klaunch, kcount = plan.profile_read()
+    plan.profile(False)
+    torch.cuda.synchronize()
+    kd = int(np.argmax(kms))
+    Nd = 32 << kd
+    avg_ms = kms[kd] / klaunch[kd]
+    ach_tf = kcount[kd] * wave_flops(Nd) / (avg_ms * 1e-3) / 1e12
+    ach_gbs = kcount[kd] * eval_bytes(Nd) / (avg_ms * 1e-3) / 1e9
+    t = tiles[kd]
+    sweep_flops = sum(float(kcount[k]) * wave_flops(32 << k) for k in range(8) if klaunch[k])
+    roofline = {"bound": "fp64-valu", "kernel": f"eval_kernel<{Nd},{t[0]},{'true' if t[1] else 'false'},{t[2]}>",
+                "achieved": round(ach_tf, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(ach_tf / FP64_PEAK_TFLOPS, 5), "traffic": None,
+                "avg_launch_ms": round(avg_ms, 4), "evals_per_launch": int(kcount[kd]),
+                "hbm_algorithmic_GBs": round(ach_gbs, 2), "hbm_frac": round(ach_gbs / HBM_PEAK_GBS, 5),
+                "classes": {str(32 << k): {"evals": int(kcount[k]), "ms_per_launch": round(kms[k] / klaunch[k], 4)}
+                            for k in range(8) if klaunch[k]},
+                "eval_sweep_achieved": round(sweep_flops / (ms_eval * 1e-3) / 1e12, 4),
+                "eval_sweep_frac": round(sweep_flops / (ms_eval * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 5),
+                "note": "achieved = evals_per_launch x wave_flops(N) / the class kernel's launch duration; "
+                        "eval_sweep_* = the algorithmic FP64 of every class / the eval-only graph sweep time"}
+    log(f"globalfit: dominant eval class BW {Nd} {avg_ms:.3f} ms/launch, {ach_tf:.2f} TFLOP/s "
+        f"({ach_tf / FP64_PEAK_TFLOPS:.3f} of FP64 peak); all classes {roofline['eval_sweep_achieved']:.2f} TFLOP/s")
     res = dict(ms=ms, ms_eval=ms_eval, ms_derivs=ms_der, E=E, nF=nF, nan_fisher=nan_f, chains=len(cfg["chain_group"]),
+               roofline=roofline,
                segments=len(cfg["group_W"]), bins=int(cfg["group_W"].sum()), max_src=cfg["max_src"],
                device_GB=total / 1e9, template_GB=tb / 1e9, free_GB_before=free0 / 1e9, t_gen=t_gen,
                t_plan=t_plan, t_tune=t_tune,
@@ -1241,7 +1271,7 @@ def main():
                 "template_GB": round(gl_["template_GB"], 1), "tiles": gl_["tiles"],
                 "setup_s": {"inputs": round(gl_["t_gen"], 1), "plan": round(gl_["t_plan"], 1),
                             "autotune": round(gl_["t_tune"], 1)},
-                "parity": gl_.get("parity"),
+                "parity": gl_.get("parity"), "roofline": gl_["roofline"],
                 "config": "20,000 binaries (f0 log-uniform 0.1-25 mHz, A log-uniform 1e-24..1e-21), Tobs 4 yr, "
                           "4,000 density-weighted segments of 64-2048 bins, 8 temperatures x 32 chains per "
                           "segment; one proposal per chain + exact Fisher of one source on every 8th chain, "
