@@ -93,22 +93,6 @@ __device__ inline const UcbConst& ucb_setup_shared(const double* p, int np, doub
     return sc;
 }
 
-// Per-template constants computed beforehand (one thread per template, e.g.
-// while aligning the proposals), copied into LDS by the workgroup.  Same
-// synchronisation as ucb_setup_shared.
-__device__ inline const UcbConst& ucb_load_shared(const UcbConst* __restrict__ src)
-{
-    static_assert(sizeof(UcbConst) % sizeof(double) == 0, "UcbConst copied as doubles");
-    __shared__ UcbConst sc;
-    __syncthreads();
-    constexpr int nw = sizeof(UcbConst) / sizeof(double);
-    const double* s = reinterpret_cast<const double*>(src);
-    double* d = reinterpret_cast<double*>(&sc);
-    for (int i = threadIdx.x; i < nw; i += blockDim.x) d[i] = s[i];
-    __syncthreads();
-    return sc;
-}
-
 // Slow samples of one phase into LDS: sample n = P*j + p at LDS slot j.
 template <int N, int NTHR, int P>
 __device__ inline void fd_samples_to_lds(const UcbConst& c, double2* lds, const double* __restrict__ pos,
