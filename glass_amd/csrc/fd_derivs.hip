@@ -140,9 +140,13 @@ __global__ __launch_bounds__(NTHR, deriv_min_waves(N)) void deriv_kernel(DerivAr
         e = A.list[e];
     }
     const double* p = A.params + (size_t)(A.pidx ? A.pidx[e] : e) * 8;
-    const UcbConst& c = ucb_setup_shared(p, 8, A.T, A.t0, A.L, A.fstar);
-    if (threadIdx.x == 0) ucb_diff_setup(p, A.T, sdd);
+    // the template constants and their derivatives set up at once on two
+    // waves (one workgroup per source: nothing of a previous source to wait for)
+    __shared__ UcbConst sc;
+    if (threadIdx.x == 0) ucb_setup(p, 8, A.T, A.t0, A.L, A.fstar, sc);
+    if (threadIdx.x == (NTHR >= 128 ? 64 : 0)) ucb_diff_setup(p, A.T, sdd);
     __syncthreads();
+    const UcbConst& c = sc;
     const UcbDiff& dd = sdd;
     double2 H[S::PER][6][3];
     deriv_passes<N, NTHR, 6>(c, dd, lds, A, H);
