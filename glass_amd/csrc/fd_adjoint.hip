@@ -87,7 +87,9 @@ __global__ __launch_bounds__(NTHR) void adjoint_kernel(DerivArgs A, int want_gra
     if (t == 0) ucb_setup(p, 8, A.T, A.t0, A.L, A.fstar, sc);
     if (t == (NTHR >= 128 ? 64 : 0)) ucb_diff_setup(p, A.T, sdd);
     __syncthreads();
-    const UcbConst& c = sc;
+    UcbConst cr = sc;  // into scalar registers for the passes
+    make_uniform(cr);
+    const UcbConst& c = cr;
     const int posstride = 4096 / N;
     // forward: the six link spectra of this thread's bins (ends with a
     // barrier)
