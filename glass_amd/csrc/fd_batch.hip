@@ -269,12 +269,27 @@ __device__ __forceinline__ void eval_item(int w, double2* lds, const int* __rest
             load_w(iC, W, j, wq);
             acc += bform(r, d, wq) - 0.5 * bform(d, d, wq);
         }
-        acc = block_sum<NTHR < 64 ? 64 : NTHR>(acc);
-        s2 = block_sum<NTHR < 64 ? 64 : NTHR>(s2);
-        if (threadIdx.x == 0)
+        // both sums over the workgroup behind one barrier (block_sum's
+        // order: wave sums, then the waves' partials in turn); thread 0
+        // writes the logL, thread 1 the snr.  The partials are next written
+        // by the following item's reduction, many barriers later.
+        constexpr int NW = (NTHR < 64 ? 64 : NTHR) / 64;
+        __shared__ double part[NW][2];
+        const double wa = wave_sum(acc), ws = wave_sum(s2);
+        if ((threadIdx.x & 63) == 0)
         {
-            logl_out[e] = B.logl[c] + acc;
-            snr_out[e] = sqrt(s2);
+            part[threadIdx.x >> 6][0] = wa;
+            part[threadIdx.x >> 6][1] = ws;
+        }
+        __syncthreads();
+        if (threadIdx.x < 2)
+        {
+            double r = 0.0;
+            for (int i = 0; i < NW; i++) r += part[i][threadIdx.x];
+            if (threadIdx.x == 0)
+                logl_out[e] = B.logl[c] + r;
+            else
+                snr_out[e] = sqrt(r);
         }
     }
 }
