@@ -226,7 +226,9 @@ __device__ __forceinline__ void eval_item(int w, double2* lds, const int* __rest
             for (int ch = 0; ch < 3; ch++) hpre[ch] = inold ? ho[ch * cap + mo] : make_double2(0.0, 0.0);
         };
         // new-template bins (and the old template where they overlap)
-        fd_link_bins<N, NTHR, fd_split(N, NTHR), fd_split(N, NTHR)>(
+        // no tail barrier: the reduction's barrier below separates this
+        // item's last LDS reads from the next item's writes
+        fd_link_bins<N, NTHR, fd_split(N, NTHR), fd_split(N, NTHR), false>(
             k, lds, opaque(batch_pos<N, NTHR>(B)), kPosN / N, opaque(B.tw), kTwN / N, pre,
             [&](int m, const double2* dl) {
                 double2 v[5];

@@ -195,15 +195,17 @@ __device__ inline void fd_link_spectra(const UcbConst& c, double2* lds, const do
 }
 
 // f(m, d) for every output bin m of this thread, d[l] = link l at bin m
-// as fd_link_spectra leaves it.  Ends with a barrier.  SOA (split shape
+// as fd_link_spectra leaves it.  Ends with a barrier (TAIL below).  SOA (split shape
 // only): pos is the N-sample structure-of-arrays block (9 rows of N
 // samples) and posstride is unused; else pos is the record table read with
 // stride posstride.
 // pre(m) is called for each bin m before f(m, d) of that bin, as early as
 // the shape allows: in the split shape at the start of the bin's pass,
 // before its transform, so that global loads the caller issues there for
-// bin m are in flight during the FFT stages.
-template <int N, int NTHR, bool SPLIT, bool SOA = false, typename Pre, typename F>
+// bin m are in flight during the FFT stages.  TAIL = false (split shape):
+// no barrier after the last LDS reads; the caller then passes a barrier of
+// its own before anything writes lds again.
+template <int N, int NTHR, bool SPLIT, bool SOA = false, bool TAIL = true, typename Pre, typename F>
 __device__ inline void fd_link_bins(const UcbConst& c, double2* lds, const double* __restrict__ pos, int posstride,
                                     const double2* __restrict__ tw, int twstride, Pre&& pre, F&& f)
 {
@@ -278,7 +280,7 @@ __device__ inline void fd_link_bins(const UcbConst& c, double2* lds, const doubl
                 const double2 v = lds[l * H + s];
                 d[l] = make_double2(v.x * inv2N, v.y * inv2N);
             }
-            __syncthreads();
+            if (!pass || TAIL) __syncthreads();
             if (!pass)
             {
                 // the odd-bin series takes the LDS before the even bins'
