@@ -1061,7 +1061,16 @@ def main():
                                   "on a 256 MiB copy); bytes per launch (L2/MALL reuse keeps it below the "
                                   "algorithmic bytes)",
                 "avg_launch_ms": round(avg_ms, 4), "evals_per_launch": int(kcount[kd]),
-                "hbm_algorithmic_GBs": round(ach_gbs, 2), "hbm_frac": round(ach_gbs / HBM_PEAK_GBS, 5)}
+                "hbm_algorithmic_GBs": round(ach_gbs, 2), "hbm_frac": round(ach_gbs / HBM_PEAK_GBS, 5),
+                "dominant": "the class kernel with the longest total time in the per-class timing pass",
+                "classes": {str(32 << k): {
+                    "kernel": f"eval_kernel<{32 << k},{tiles_now[k][0]},{'true' if tiles_now[k][1] else 'false'},"
+                              f"{tiles_now[k][2]}>",
+                    "evals": int(kcount[k]), "ms_per_launch": round(kms[k] / klaunch[k], 4),
+                    "achieved": round(kcount[k] * wave_flops(32 << k) / (kms[k] / klaunch[k] * 1e-3) / 1e12, 4),
+                    "frac": round(kcount[k] * wave_flops(32 << k) / (kms[k] / klaunch[k] * 1e-3) / 1e12
+                                  / FP64_PEAK_TFLOPS, 5)}
+                    for k in range(8) if klaunch[k] and kcount[k]}}
 
     if args.copy_calibrate:
         a = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")  # 256 MiB

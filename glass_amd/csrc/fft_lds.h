@@ -109,7 +109,22 @@ __device__ inline void radix_butterfly(const double2 (&v)[R], double2 (&y)[R])
 // multiple of N/R) every butterfly of a thread has the same index j and
 // shares one set.  Twiddle indices are in range for any b (j < N/R), so
 // the loads need no guard.
-template <int N, int NL, int R, int NTHR, bool SWZ = false>
+// Orders a wave's LDS accesses before and after it, for the other lanes of
+// the same wave: the compiler keeps loads and stores on their side, and the
+// LDS serves one wave's instructions in issue order, so no instruction is
+// emitted (wavefront-scope fences).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// WAVE: every series of the stage is read and written by the lanes of one
+// wave, the same wave as in the neighbouring stages (lds_fft decides), so
+// the stage's two synchronisations are wave-level; LAST: its final one is a
+// workgroup barrier all the same (the caller reads across waves after it).
+template <int N, int NL, int R, int NTHR, bool SWZ = false, bool WAVE = false, bool LAST = true>
 __device__ __forceinline__ void stockham_stage(double2* buf, int Ns, const double2* __restrict__ tw, int twstride,
                                                int tid = threadIdx.x)
 {
@@ -144,7 +159,10 @@ __device__ __forceinline__ void stockham_stage(double2* buf, int Ns, const doubl
             for (int m = 0; m < R; m++) v[p][m] = buf[l * N + lds_slot_add<SWZ>(sj, m * (N / R))];
         }
     }
-    __syncthreads();
+    if constexpr (WAVE)
+        wave_lds_sync();
+    else
+        __syncthreads();
 #pragma unroll
     for (int p = 0; p < PER; p++)
     {
@@ -164,7 +182,10 @@ __device__ __forceinline__ void stockham_stage(double2* buf, int Ns, const doubl
             for (int m = 0; m < R; m++) buf[l * N + lds_slot_add<SWZ>(sb, m * Ns)] = y[m];
         }
     }
-    __syncthreads();
+    if constexpr (WAVE && !LAST)
+        wave_lds_sync();
+    else
+        __syncthreads();
 }
 
 // Full forward FFT of NL length-N series in LDS. twstride = NTW / N.
@@ -196,10 +217,18 @@ __device__ __forceinline__ void lds_fft(double2* buf, const double2* __restrict_
         stockham_stage<N, NL, 2, NTHR, SWZ>(buf, Ns, tw, twstride, tid);
         Ns *= 2;
     }
+    // radix-4 stages only, N/4 <= 64 butterflies per series: butterfly b =
+    // tid + p*NTHR of series b / (N/4) lies in one wave (N/4 divides 64 and
+    // NTHR), the same wave in every stage, so only the last stage ends with
+    // a workgroup barrier (the caller has synchronised the input)
+    constexpr bool WAVE = !R3 && !(LOG2 & 1) && N / 4 <= 64 && NTHR % 64 == 0;
 #pragma unroll
     for (int s = 0; s < LOG2 / 2; s++)
     {
-        stockham_stage<N, NL, 4, NTHR, SWZ>(buf, Ns, tw, twstride, tid);
+        if (s == LOG2 / 2 - 1)
+            stockham_stage<N, NL, 4, NTHR, SWZ, WAVE, true>(buf, Ns, tw, twstride, tid);
+        else
+            stockham_stage<N, NL, 4, NTHR, SWZ, WAVE, false>(buf, Ns, tw, twstride, tid);
         Ns *= 4;
     }
 }
