@@ -36,8 +36,8 @@ FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per launch of the batched per-class kernels, measured with rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE on this workload and scaled by the same counters on a 256 MiB device copy (fetch x2.00,
 # write x1.00): profiles/eval_kernel_batched_fd_sweep.md.  Keyed by template bins N.
-MEASURED_TRAFFIC = {512: 2.3546e9, 1024: 1.8288e9, 2048: 1.5617e9, 256: 8.2987e8, 4096: 3.0662e8,
-                    128: 2.3359e8, 64: 7.1662e7, 32: 1.8162e6}
+MEASURED_TRAFFIC = {512: 2.3859e9, 1024: 1.8438e9, 2048: 1.5761e9, 256: 8.5133e8, 4096: 1.6860e8,
+                    128: 2.4454e8, 64: 7.8533e7, 32: 1.8974e6}
 
 # HBM bytes per launch of logl_chunk_kernel on the inner-product leg (rocprofv3 --pmc FETCH_SIZE and
 # WRITE_SIZE, same calibration): profiles/logl_chunk_kernel_batched_inner_product.md.
