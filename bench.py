@@ -806,7 +806,11 @@ def globalfit_leg(gl, synth, steps=3, seed=4, cpu_seconds=2.0, nthreads=0, seria
     kms, klaunch, kcount = plan.profile_read()
     plan.profile(False)
     torch.cuda.synchronize()
-    kd = int(np.argmax(kms))
+    # dominant class: the largest share of the algorithmic FP64 work, as in
+    # the sweep's roofline; the longest class beside it
+    work = np.array([float(kcount[k]) * wave_flops(32 << k) if klaunch[k] else 0.0 for k in range(8)])
+    kd = int(np.argmax(work))
+    kt = int(np.argmax(kms))
     Nd = 32 << kd
     avg_ms = kms[kd] / klaunch[kd]
     ach_tf = kcount[kd] * wave_flops(Nd) / (avg_ms * 1e-3) / 1e12
@@ -820,11 +824,15 @@ def globalfit_leg(gl, synth, steps=3, seed=4, cpu_seconds=2.0, nthreads=0, seria
                 "hbm_algorithmic_GBs": round(ach_gbs, 2), "hbm_frac": round(ach_gbs / HBM_PEAK_GBS, 5),
                 "classes": {str(32 << k): {"evals": int(kcount[k]), "ms_per_launch": round(kms[k] / klaunch[k], 4)}
                             for k in range(8) if klaunch[k]},
+                "longest_class": {"N": 32 << kt, "ms_per_launch": round(kms[kt] / klaunch[kt], 4),
+                                  "frac": round(kcount[kt] * wave_flops(32 << kt) / (kms[kt] / klaunch[kt] * 1e-3)
+                                                / 1e12 / FP64_PEAK_TFLOPS, 5)},
                 "eval_sweep_achieved": round(sweep_flops / (ms_eval * 1e-3) / 1e12, 4),
                 "eval_sweep_frac": round(sweep_flops / (ms_eval * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 5),
-                "note": "achieved = evals_per_launch x wave_flops(N) / the class kernel's launch duration; "
+                "note": "dominant = the class with the largest algorithmic FP64 work; "
+                        "achieved = evals_per_launch x wave_flops(N) / the class kernel's launch duration; "
                         "eval_sweep_* = the algorithmic FP64 of every class / the eval-only graph sweep time"}
-    log(f"globalfit: dominant eval class BW {Nd} {avg_ms:.3f} ms/launch, {ach_tf:.2f} TFLOP/s "
+    log(f"globalfit: dominant eval class (work) BW {Nd} {avg_ms:.3f} ms/launch, {ach_tf:.2f} TFLOP/s "
         f"({ach_tf / FP64_PEAK_TFLOPS:.3f} of FP64 peak); all classes {roofline['eval_sweep_achieved']:.2f} TFLOP/s")
     res = dict(ms=ms, ms_eval=ms_eval, ms_derivs=ms_der, E=E, nF=nF, nan_fisher=nan_f, chains=len(cfg["chain_group"]),
                roofline=roofline,
@@ -1043,7 +1051,11 @@ def main():
         if klaunch[k]:
             log(f"  BW {32 << k:5d}: {kcount[k]:7d} evals, {kms[k] / klaunch[k]:8.3f} ms/launch")
 
-    kd = int(np.argmax(kms))
+    # the dominant class: the largest share of the sweep's algorithmic FP64
+    # work (stable run to run; the longest class is reported beside it)
+    work = np.array([float(kcount[k]) * wave_flops(32 << k) if klaunch[k] else 0.0 for k in range(8)])
+    kd = int(np.argmax(work))
+    kt = int(np.argmax(kms))
     Nd = 32 << kd
     avg_ms = kms[kd] / klaunch[kd]
     flops = kcount[kd] * wave_flops(Nd)
@@ -1062,7 +1074,14 @@ def main():
                                   "algorithmic bytes)",
                 "avg_launch_ms": round(avg_ms, 4), "evals_per_launch": int(kcount[kd]),
                 "hbm_algorithmic_GBs": round(ach_gbs, 2), "hbm_frac": round(ach_gbs / HBM_PEAK_GBS, 5),
-                "dominant": "the class kernel with the longest total time in the per-class timing pass",
+                "dominant": "the class kernel with the largest share of the sweep's algorithmic FP64 work "
+                            "(evaluations x wave_flops(N)); longest_class: the one with the longest time in "
+                            "the per-class timing pass",
+                "longest_class": {"kernel": f"eval_kernel<{32 << kt},{tiles_now[kt][0]},"
+                                            f"{'true' if tiles_now[kt][1] else 'false'},{tiles_now[kt][2]}>",
+                                  "ms_per_launch": round(kms[kt] / klaunch[kt], 4),
+                                  "frac": round(kcount[kt] * wave_flops(32 << kt) / (kms[kt] / klaunch[kt] * 1e-3)
+                                                / 1e12 / FP64_PEAK_TFLOPS, 5)},
                 "classes": {str(32 << k): {
                     "kernel": f"eval_kernel<{32 << k},{tiles_now[k][0]},{'true' if tiles_now[k][1] else 'false'},"
                               f"{tiles_now[k][2]}>",
@@ -1071,6 +1090,9 @@ def main():
                     "frac": round(kcount[k] * wave_flops(32 << k) / (kms[k] / klaunch[k] * 1e-3) / 1e12
                                   / FP64_PEAK_TFLOPS, 5)}
                     for k in range(8) if klaunch[k] and kcount[k]}}
+    log(f"roofline: {roofline['kernel']} {avg_ms:.3f} ms/launch, {ach_tf:.2f} TFLOP/s = {roofline['frac']:.3f} of "
+        f"FP64 peak (largest algorithmic work); longest class {roofline['longest_class']['kernel']} "
+        f"{roofline['longest_class']['ms_per_launch']:.3f} ms, {roofline['longest_class']['frac']:.3f}")
 
     if args.copy_calibrate:
         a = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")  # 256 MiB
