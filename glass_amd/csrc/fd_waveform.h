@@ -13,6 +13,8 @@
 //    half-length transforms run in turn (48 N bytes of LDS).  Bins are
 //    handed out per pass, one per thread (m = 2t, then 2t + 1), so only
 //    one bin's six link values are live at a time.
+//  * N = 2048 with NTHR = 512 (fd_link_bins): the split step on two sample
+//    pairs per thread, the two 1024-point half-length transforms in turn.
 //  * N = P*NS (N >= 2048, NS = 1024): decimation in time.  The samples
 //    n = P*j + p of each phase p form an NS-point series; its six FFTs run
 //    in LDS in turn and each thread folds X_{j+NS r} += w_N^{p j} w_P^{p r}
@@ -210,7 +212,82 @@ __device__ inline void fd_link_bins(const UcbConst& c, double2* lds, const doubl
                                     const double2* __restrict__ tw, int twstride, Pre&& pre, F&& f)
 {
     static_assert(SPLIT || !SOA, "structure-of-arrays positions: split shape only");
-    if constexpr (!SPLIT)
+    // 2048 bins on 512 threads: the split step on two sample pairs per
+    // thread (n = t, t + 512 with partners n + 1024), the odd-bin series in
+    // registers during the even-bin transform; no decimated accumulators
+    // (3% faster than the decimated form, 173 VGPRs instead of ~235)
+    constexpr bool SPLIT2 = !SPLIT && N == 2048 && NTHR == 512;
+    if constexpr (SPLIT2)
+    {
+        constexpr int H = N / 2, Q = H / NTHR;
+        const double inv2N = 1. / (double)(2 * N);
+        const int t = threadIdx.x;
+        double2 ya[Q][3], yb[Q][3];
+#pragma unroll
+        for (int q = 0; q < Q; q++)
+        {
+            const int n = t + q * NTHR;
+            ucb_slow_sample(c, ucb_sample_time(c.T, c.t0, n, N), pos + (size_t)n * posstride * 9, lds, H, n);
+            const double2 w = tw[n * twstride];
+            ucb_slow_sample_to<false>(c, ucb_sample_time(c.T, c.t0, n + H, N), pos + (size_t)(n + H) * posstride * 9,
+                                      [&](int l, double2 x1) {
+                                          const double2 x0 = lds[l * H + n];
+                                          lds[l * H + n] = make_double2(x0.x + x1.x, x0.y + x1.y);
+                                          const double2 dv = cmul(make_double2(x0.x - x1.x, x0.y - x1.y), w);
+                                          if (l < 3)
+                                          {
+                                              ya[q][0] = ya[q][1];
+                                              ya[q][1] = ya[q][2];
+                                              ya[q][2] = dv;
+                                          }
+                                          else
+                                          {
+                                              yb[q][0] = yb[q][1];
+                                              yb[q][1] = yb[q][2];
+                                              yb[q][2] = dv;
+                                          }
+                                      });
+        }
+        __syncthreads();
+#pragma unroll
+        for (int pass = 0; pass < 2; pass++)
+        {
+            lds_fft<H, 6, NTHR>(lds, opaque(tw), 2 * twstride);
+            double2 d[Q][6];
+#pragma unroll
+            for (int q = 0; q < Q; q++)
+            {
+                // bin m = 2n + pass is FFT index (m + H) mod N, of parity pass
+                const int m = 2 * (t + q * NTHR) + pass;
+                const int sl = ((m + H) & (N - 1)) >> 1;
+#pragma unroll
+                for (int l = 0; l < 6; l++)
+                {
+                    const double2 v = lds[l * H + sl];
+                    d[q][l] = make_double2(v.x * inv2N, v.y * inv2N);
+                }
+            }
+            if (!pass || TAIL) __syncthreads();
+            if (!pass)
+#pragma unroll
+                for (int q = 0; q < Q; q++)
+#pragma unroll
+                    for (int l = 0; l < 3; l++)
+                    {
+                        lds[l * H + t + q * NTHR] = ya[q][l];
+                        lds[(l + 3) * H + t + q * NTHR] = yb[q][l];
+                    }
+#pragma unroll
+            for (int q = 0; q < Q; q++)
+            {
+                const int m = 2 * (t + q * NTHR) + pass;
+                pre(m);
+                f(m, d[q]);
+            }
+            if (!pass) __syncthreads();
+        }
+    }
+    else if constexpr (!SPLIT)
     {
         using S = FdShape<N, NTHR>;
         double2 dl[S::PER][6];
